@@ -1,0 +1,86 @@
+/*
+ * rsbd8_oracle.h -- CPU restatement of the Radixsortbidi8 (rsbd8) stable LSD radix sort.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP product in
+ * radixsortbidi8_amd/: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it. Nothing in the product links it, and the product never falls back to it.
+ *
+ * Citations are file:line into the reference repository (Radixsortbidi8.hpp = "H").
+ *
+ * Flag bits (the five compile-time booleans the reference derives from (mode, T, direction),
+ * H:59778-59904):
+ *   RSBD8O_ABS  = isabsvalue     RSBD8O_SIGN = issignmode     RSBD8O_FLTP = isfltpmode
+ *   RSBD8O_DESC = isdescsort     RSBD8O_REV  = isrevorder (already filtered through
+ *                                isrevorderwithoutindirection / isrevorderwithindirection)
+ */
+#ifndef RSBD8_ORACLE_H
+#define RSBD8_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+	RSBD8O_ABS = 1u,
+	RSBD8O_SIGN = 2u,
+	RSBD8O_FLTP = 4u,
+	RSBD8O_DESC = 8u,
+	RSBD8O_REV = 16u
+};
+
+/* key type codes, shared with the product's C-ABI (include/rsbd8_hip.h) */
+enum {
+	RSBD8O_U8 = 0, RSBD8O_U16 = 1, RSBD8O_U32 = 2, RSBD8O_U64 = 3,
+	RSBD8O_I8 = 4, RSBD8O_I16 = 5, RSBD8O_I32 = 6, RSBD8O_I64 = 7,
+	RSBD8O_F32 = 8, RSBD8O_F64 = 9
+};
+
+/* mode decoding of H:59778-59904 for a plain key type code (enums pass their unsigned
+ * carrier, as std::is_signed_v is false for enums). indirect selects
+ * isrevorderwithindirection (H:59872) over isrevorderwithoutindirection (H:59877). */
+unsigned rsbd8o_decode(int keytype, unsigned mode, unsigned direction, int indirect);
+
+/* closed-form sortable key of a W-bit pattern x (W in {8,16,32,64}); the descending bit is
+ * not applied. Follows filterinput H:10000-10027 plus the virtual top-bit flip of the offsets
+ * transform H:11649-11701. */
+uint64_t rsbd8o_sortkey(unsigned width_bits, unsigned flags, uint64_t x);
+
+/* The reference's single-threaded algorithm: one counting pass over all digits with the
+ * reference's digit geometry (H:1848-1869), per-digit bidirectional offsets with pass skip
+ * (H:11626-11809), and stable bidirectional scatter passes (H:31649-31982). keys: width_bits/8
+ * bytes each. pin may be NULL (then the payload is the source index when pout != NULL).
+ * Returns the number of executed scatter passes, or -1 on bad arguments/allocation failure. */
+int rsbd8o_radixsort(unsigned width_bits, unsigned flags, size_t n, const void *in, void *out,
+	const uint32_t *pin, uint32_t *pout);
+
+/* Independent formulation used to pin rsbd8o_radixsort: stable top-down merge sort on
+ * (sortkey, logical index) with the same direction rules (H:252-275). */
+int rsbd8o_mergesort(unsigned width_bits, unsigned flags, size_t n, const void *in, void *out,
+	uint32_t *pout);
+
+/* Indirection through records: key = *(T*)(base + idx[i]*stride + offset) (the type+offset
+ * wrapper H:61360-61393 with pointers base + idx*stride). Writes the sorted indices. */
+int rsbd8o_sort_indirect(unsigned width_bits, unsigned flags, size_t n, const uint8_t *base,
+	size_t stride, size_t offset, const uint32_t *idx, uint32_t *idx_out);
+
+/* Independent segmented sort: every segment [seg[s], seg[s+1]) sorted on its own with
+ * rsbd8o_radixsort (the reference runs one radixsortcopy per segment, H:60468). */
+int rsbd8o_sort_segmented(unsigned width_bits, unsigned flags, size_t nseg, const uint64_t *seg,
+	const void *in, void *out);
+
+/* 8-bit-digit histograms (digit d of pass p = (sortkey >> 8p) & 255, descending applied as a
+ * complement), counts[p*256 + d]; used to check a device histogram. */
+void rsbd8o_histogram8(unsigned width_bits, unsigned flags, size_t n, const void *in, uint64_t *counts);
+
+/* Multithreaded variant of rsbd8o_radixsort (pthreads; counting and scatter sliced across
+ * threads with per-thread offsets, as the reference's split counting H:28919 plus slice
+ * scatters); used as the timed CPU baseline. */
+int rsbd8o_radixsort_mt(unsigned width_bits, unsigned flags, size_t n, const void *in, void *out,
+	unsigned threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
