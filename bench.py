@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Headline benchmark: stable LSD radix sort of 2^28 uniform u64 keys on one MI355X.
+
+One step = one complete sort (rsbd8::radixsortcopynoalloc semantics: histogram of every digit,
+plan, every executed scatter pass) of the device-resident input into the output array.
+Prints one JSON line: sorted GKeys/s (value), the HBM roofline of the dominant kernel (the
+onesweep scatter pass, timed per launch with HIP events on its stream), the whole-sort roofline
+fraction, and the multithreaded CPU baseline (the oracle's sliced LSD sort) timed on this host.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "sorted GKeys/s & % HBM roofline (u64 2^28); learned vs fixed plan speedup"
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline(sample_log2: int, budget_s: float):
+    import numpy as np
+
+    import oracle
+    threads = max(1, min(16, os.cpu_count() or 1))
+    n = 1 << sample_log2
+    rng = np.random.default_rng(0x5EED0001)
+    keys = rng.integers(0, np.iinfo(np.uint64).max, n, dtype=np.uint64, endpoint=True)
+    out = np.empty_like(keys)
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 3 or (time.perf_counter() - t_start < budget_s and len(times) < 7):
+        t0 = time.perf_counter()
+        oracle.radixsort_mt(keys, 0, threads, out)
+        times.append(time.perf_counter() - t0)
+    assert np.array_equal(out, np.sort(keys)), "cpu baseline produced an unsorted result"
+    med = statistics.median(times)
+    return {"value": round(n / med / 1e9, 4), "unit": "GKeys/s", "cores": threads, "kind": "port",
+            "sample": f"2^{sample_log2} uniform u64 keys (same distribution), oracle LSD sort with the "
+                      f"reference's 11-bit digit plan sliced over {threads} threads, median of {len(times)}"}
+
+
+def traffic_from_profile(n_log2: int):
+    """HBM bytes per scatter launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE), as
+    recorded in profiles/ by the counter runs; None when no such record exists."""
+    path = os.path.join(ROOT, "profiles", f"onesweep_u64_2p{n_log2}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    return rec.get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n-log2", type=int, default=28)
+    ap.add_argument("--cpu-sample-log2", type=int, default=26)
+    ap.add_argument("--cpu-budget-s", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tile-config", type=int, default=-1)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: every run here has exactly one MI355X; "
+              "--gpus N with N > 1 is not supported (the multi-GPU sharding is described in DESIGN.md)",
+              file=sys.stderr, flush=True)
+        sys.exit(2)
+
+    import torch
+
+    import radixsortbidi8_amd as r
+
+    n = 1 << args.n_log2
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    gen = torch.Generator(device=dev).manual_seed(0x5EED0001)
+    keys = torch.randint(-(1 << 63), (1 << 63) - 1, (n,), dtype=torch.int64, device=dev, generator=gen).view(torch.uint64)
+    out = torch.empty_like(keys)
+    buf = torch.empty_like(keys)
+    ws = r.Workspace(r.U64, n)
+    if args.tile_config >= 0:
+        r.set_tile_config(args.tile_config)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        r.radixsortcopynoalloc(keys, out, buf, ws)
+    torch.cuda.synchronize()
+    print(f"bench: warmup done (n=2^{args.n_log2})", file=sys.stderr, flush=True)
+
+    r.profile_begin()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r.radixsortcopynoalloc(keys, out, buf, ws)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = r.profile_end()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+
+    # verify the last result: monotone under the sign-flipped int64 view, and a permutation
+    flip = out.view(torch.int64) ^ torch.tensor(-(1 << 63), dtype=torch.int64, device=dev)
+    sorted_ok = bool((flip[1:] >= flip[:-1]).all().item())
+    ref = torch.sort(keys.view(torch.int64) ^ torch.tensor(-(1 << 63), dtype=torch.int64, device=dev)).values
+    perm_ok = bool(torch.equal(ref, flip))
+    del ref, flip
+    print(f"bench: timed {args.steps} steps, verified sorted={sorted_ok} permutation={perm_ok}", file=sys.stderr,
+          flush=True)
+
+    k = 8
+    scatter_ms, scatter_launches = prof["scatter"]
+    passes = round(scatter_launches / args.steps) if args.steps else 0
+    avg_pass_ms = scatter_ms / max(1, scatter_launches)
+    bytes_per_pass = n * 2 * k
+    achieved = bytes_per_pass / (avg_pass_ms * 1e-3) / 1e9
+    sort_bytes = n * k + passes * n * 2 * k + (2 * n * k if passes == 0 else 0)
+    sort_gbs = sort_bytes / (ms_per_step * 1e-3) / 1e9
+    traffic = traffic_from_profile(args.n_log2)
+    line = {
+        "metric": METRIC,
+        "value": round(n / (ms_per_step * 1e-3) / 1e9, 4),
+        "unit": "GKeys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": "2^28 uniform u64 keys (seeded), stable ascending sort, keys only, device-resident copy in->out",
+            "n": n, "key": "u64", "direction": "ascfwdorder", "mode": "native",
+            "plan": f"8-bit digits, {passes} executed onesweep passes",
+            "tile_config": r.TILE_CONFIGS[args.tile_config] if args.tile_config >= 0 else "auto",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_onesweep (one scatter pass)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": bytes_per_pass,
+            "avg_launch_ms": round(avg_pass_ms, 4),
+        },
+        "sort_roofline": {
+            "algorithmic_bytes": sort_bytes,
+            "achieved_gbs": round(sort_gbs, 1),
+            "frac": round(sort_gbs / HBM_PEAK_GBS, 4),
+            "hist_ms": round(prof["hist"][0] / max(1, prof["hist"][1]), 4),
+        },
+        "verified": sorted_ok and perm_ok,
+        "learned_vs_fixed": None,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(args.cpu_sample_log2, args.cpu_budget_s)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,141 @@
+/*
+ * rsbd8_hip.h -- C ABI of the MI355X (gfx950) rsbd8 radix sort.
+ *
+ * Every entry point is a drop-in for one of the reference's sorting templates (Radixsortbidi8.hpp,
+ * cited as H:line) with the compile-time template arguments turned into run-time values:
+ *   - the key type T becomes a key type code (an enum passes its unsigned underlying type, as
+ *     the reference sorts enums by their unsigned bit pattern, H:2893 and H:59779);
+ *   - sortingmode/sortingdirection become the five decoded flags of H:59778-59904
+ *     (rsbd8_hip_decode performs that decoding from the raw enum values);
+ *   - allowedthreads and pfuturesplaceholder have no counterpart (the GPU grid replaces the CPU
+ *     threads), and pzeroedindices becomes a device workspace of rsbd8_hip_workspace_size bytes
+ *     that must be zero before its first use (rsbd8_hip_workspace_init) and is left ready for
+ *     reuse by every sort.
+ * Array arguments are device pointers (hipMalloc'd or otherwise GPU-accessible) unless stated.
+ * All sorts are stream-ordered and asynchronous unless stated; calls on one workspace must not
+ * overlap. Return value: 0 on success, a negative RSBD8_HIP_E* code otherwise
+ * (rsbd8_hip_last_error gives the message).
+ */
+#ifndef RSBD8_HIP_H
+#define RSBD8_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *rsbd8_hip_stream; /* a hipStream_t; NULL is the null stream */
+
+enum rsbd8_hip_keytype {
+	RSBD8_HIP_U8 = 0, RSBD8_HIP_U16 = 1, RSBD8_HIP_U32 = 2, RSBD8_HIP_U64 = 3,
+	RSBD8_HIP_I8 = 4, RSBD8_HIP_I16 = 5, RSBD8_HIP_I32 = 6, RSBD8_HIP_I64 = 7,
+	RSBD8_HIP_F32 = 8, RSBD8_HIP_F64 = 9
+};
+
+/* decoded flags: isabsvalue, issignmode, isfltpmode, isdescsort, isrevorder */
+enum {
+	RSBD8_HIP_ABS = 1, RSBD8_HIP_SIGN = 2, RSBD8_HIP_FLTP = 4, RSBD8_HIP_DESC = 8, RSBD8_HIP_REV = 16
+};
+
+/* raw enum values of rsbd8::sortingmode (H:118-156) and rsbd8::sortingdirection (H:158-171) */
+enum {
+	RSBD8_HIP_MODE_NATIVE = 64, RSBD8_HIP_MODE_NATIVEABS = 65, RSBD8_HIP_MODE_NATIVETIEREDABS = 66,
+	RSBD8_HIP_MODE_FORCEUNSIGNED = 0, RSBD8_HIP_MODE_FORCESIGNED = 2, RSBD8_HIP_MODE_FORCEABSSIGNED = 3,
+	RSBD8_HIP_MODE_FORCEFLOATINGP = 6, RSBD8_HIP_MODE_FORCEABSFLOATINGP = 7,
+	RSBD8_HIP_MODE_SPECIALFLOATINGP = 4, RSBD8_HIP_MODE_FORCETIEREDABSFLOATINGP = 5,
+	RSBD8_HIP_MODE_FORCETIEREDABSSIGNED = 1,
+	RSBD8_HIP_ASCFWDORDER = 0, RSBD8_HIP_DSCFWDORDER = 1, RSBD8_HIP_ASCREVORDER = 2, RSBD8_HIP_DSCREVORDER = 3
+};
+
+/* workspace layouts for rsbd8_hip_workspace_size */
+enum { RSBD8_HIP_LAYOUT_KEYS = 0, RSBD8_HIP_LAYOUT_PAIRS = 1, RSBD8_HIP_LAYOUT_INDIRECT = 2 };
+
+enum {
+	RSBD8_HIP_OK = 0, RSBD8_HIP_EINVAL = -1, RSBD8_HIP_ENOMEM = -2, RSBD8_HIP_EDEVICE = -3
+};
+
+/* helper::isabsvalue, issignmode, isfltpmode (H:59778-59865), isdescsort (H:59867-59870),
+ * isrevorderwithindirection (H:59872-59875) when indirect, else isrevorderwithoutindirection
+ * (H:59877-59904). mode and direction are the raw enum values. */
+int rsbd8_hip_decode(int keytype, unsigned mode, unsigned direction, int indirect, unsigned *flags);
+
+/* The pzeroedindices equivalent (allocatearray H:59921-60065 sizes it on the CPU side). */
+size_t rsbd8_hip_workspace_size(int keytype, size_t count, int layout);
+int rsbd8_hip_workspace_init(void *workspace, size_t workspace_bytes, rsbd8_hip_stream stream);
+
+/* rsbd8::radixsortcopynoalloc without indirection: multi-part H:60121-60153, single-part
+ * H:60186-60242 (8-bit keys: buffer may be NULL, as the reference's dummy-buffer overload
+ * ignores it). input is not written. */
+int rsbd8_hip_radixsortcopynoalloc(int keytype, unsigned flags, size_t count, void *workspace,
+	size_t workspace_bytes, const void *input, void *output, void *buffer, rsbd8_hip_stream stream);
+
+/* rsbd8::radixsortnoalloc without indirection: multi-part H:60155-60184, single-part
+ * H:60244-60395. The result lands in input, or in buffer when movetobuffer is set; the other
+ * array holds garbage afterwards (H:188). buffer may be NULL for 8-bit keys in the bijective
+ * modes, as in the reference's buffer-less single-part overload. */
+int rsbd8_hip_radixsortnoalloc(int keytype, unsigned flags, size_t count, void *workspace,
+	size_t workspace_bytes, void *input, void *buffer, int movetobuffer, rsbd8_hip_stream stream);
+
+/* rsbd8::radixsortcopy without indirection (H:60468-60527, H:60529-60588): allocates and frees
+ * its own workspace and buffer; input/output may be host or device memory (host arrays are
+ * staged through device memory). Synchronous. Returns RSBD8_HIP_ENOMEM where the reference
+ * returns false (H:60463). */
+int rsbd8_hip_radixsortcopy(int keytype, unsigned flags, size_t count, const void *input, void *output,
+	rsbd8_hip_stream stream);
+
+/* rsbd8::radixsort without indirection (H:60397-60466), in place; host or device memory.
+ * Synchronous. */
+int rsbd8_hip_radixsort(int keytype, unsigned flags, size_t count, void *input, rsbd8_hip_stream stream);
+
+/* Indirection over an array of device pointers, the T** and type+offset overloads
+ * (H:60590-60981, H:61359-61738) with member pointers expressed as byte offsets
+ * (memberobjectgenerator H:656-668, getoffsetof H:711-720):
+ *   first level:  key = *(T *)((char *)input[i] + indirection1)
+ *   second level: key = ((T *)(*(char **)((char *)input[i] + indirection1) + indirection2))[index2]
+ * (indirectinput1/indirectinput2 H:2745-2821). Keys are read once; the pointers are sorted.
+ * flags should carry RSBD8_HIP_REV as isrevorderwithindirection decodes it. */
+int rsbd8_hip_radixsortcopynoalloc_indirect(int keytype, unsigned flags, size_t count, void *workspace,
+	size_t workspace_bytes, const void *const *input, void **output, ptrdiff_t indirection1, int second_level,
+	ptrdiff_t indirection2, size_t index2, rsbd8_hip_stream stream);
+
+/* In-place form of the above (radixsortnoalloc with indirection, H:61017-61045, H:61395-61427):
+ * the sorted pointers land in input, or in buffer when movetobuffer is set. */
+int rsbd8_hip_radixsortnoalloc_indirect(int keytype, unsigned flags, size_t count, void *workspace,
+	size_t workspace_bytes, void **input, void **buffer, int movetobuffer, ptrdiff_t indirection1,
+	int second_level, ptrdiff_t indirection2, size_t index2, rsbd8_hip_stream stream);
+
+/* Records addressed by 32-bit indices: the type+offset overload (H:61360-61393) applied to the
+ * pointers base + idx_in[i] * stride, returning the sorted indices instead of pointers. */
+int rsbd8_hip_sort_indices(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
+	const void *base, size_t stride, size_t key_offset, const uint32_t *idx_in, uint32_t *idx_out,
+	rsbd8_hip_stream stream);
+
+/* Keys with a 32-bit payload (the engine under the indirect sorts): a stable sort of keys_in
+ * that moves payload_in along (payload_in NULL: the payload is the source index).
+ * keys_buffer/payload_buffer are count-element scratch; no array may alias another. */
+int rsbd8_hip_sort_pairs(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
+	const void *keys_in, void *keys_out, void *keys_buffer, const uint32_t *payload_in, uint32_t *payload_out,
+	uint32_t *payload_buffer, rsbd8_hip_stream stream);
+
+/* Process-wide tile configuration of the scatter passes: 0..4 selects one of
+ * {256x8, 256x16, 512x8, 512x16, 256x24} threads x keys per thread, -1 restores the automatic
+ * choice. Returns the previous setting. (No reference counterpart: the reference's static
+ * thread thresholds H:12948-13152 are its closest analogue.) */
+int rsbd8_hip_set_tile_config(int cfg);
+
+/* Per-launch HIP-event timing of the sorts issued between begin and end, on their own streams
+ * (no reference counterpart; the reference's benchmark times whole calls with rdtsc,
+ * seqptests/WindowsProject1/WindowsProject1.cpp:912-937). end synchronizes and returns, per kind,
+ * the summed milliseconds and launch counts: 0 histogram, 1 executed scatter passes, 2 scatter
+ * launches of skipped (uniform-digit) passes. */
+int rsbd8_hip_profile_begin(void);
+int rsbd8_hip_profile_end(double *ms, unsigned *launches, int nkinds);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *rsbd8_hip_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,463 @@
+// rsbd8_capi.hip -- the extern "C" boundary (include/rsbd8_hip.h) over the gfx950 sort engine,
+// plus the gather/permute kernels of the indirection layouts.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/rsbd8_hip.h"
+#include "rsbd8_engine.hpp"
+
+namespace rsbd8hip {
+
+namespace {
+
+// ---------------------------------------------------------------------------------------------
+// gather / permute kernels
+
+template <typename K>
+__device__ __forceinline__ K load_unaligned(const unsigned char *p)
+{
+	K v = 0;
+#pragma unroll
+	for (int b = 0; b < static_cast<int>(sizeof(K)); ++b) v |= static_cast<K>(p[b]) << (8 * b);
+	return v;
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_gather_index(const unsigned char *__restrict__ base, uint64_t stride,
+	uint64_t offset, const uint32_t *__restrict__ idx, K *__restrict__ keys, uint64_t n, int aligned)
+{
+	const uint64_t step = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+	for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+		const unsigned char *p = base + static_cast<uint64_t>(idx[i]) * stride + offset;
+		keys[i] = aligned ? *reinterpret_cast<const K *>(p) : load_unaligned<K>(p);
+	}
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_gather_pointer(const unsigned char *const *__restrict__ ptrs, int64_t off1,
+	int second_level, int64_t off2, uint64_t index2, K *__restrict__ keys, uint64_t n)
+{
+	const uint64_t step = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+	for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) {
+		const unsigned char *p = ptrs[i] + off1;
+		if (second_level) {
+			const unsigned char *q = (reinterpret_cast<uintptr_t>(p) & 7u)
+				? reinterpret_cast<const unsigned char *>(load_unaligned<uint64_t>(p))
+				: *reinterpret_cast<const unsigned char *const *>(p);
+			p = q + off2 + index2 * sizeof(K);
+		}
+		keys[i] = (reinterpret_cast<uintptr_t>(p) & (sizeof(K) - 1)) ? load_unaligned<K>(p) : *reinterpret_cast<const K *>(p);
+	}
+}
+
+__global__ __launch_bounds__(256) void k_permute_ptr(const void *const *__restrict__ src, const uint32_t *__restrict__ perm,
+	void **__restrict__ dst, uint64_t n)
+{
+	const uint64_t step = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+	for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step)
+		dst[i] = const_cast<void *>(src[perm[i]]);
+}
+
+__global__ __launch_bounds__(256) void k_permute_u32(const uint32_t *__restrict__ src, const uint32_t *__restrict__ perm,
+	uint32_t *__restrict__ dst, uint64_t n)
+{
+	const uint64_t step = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+	for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += step) dst[i] = src[perm[i]];
+}
+
+inline dim3 grid_for(uint64_t n)
+{
+	const uint64_t g = (n + 255) / 256;
+	const uint64_t cap = 8ull * device_cu_count();
+	return dim3(static_cast<unsigned>(g < 1 ? 1 : (g > cap ? cap : g)));
+}
+
+} // namespace
+
+hipError_t gather_by_index(int kb, uint64_t n, const void *base, uint64_t stride, uint64_t offset, const uint32_t *idx,
+	void *keys, hipStream_t s)
+{
+	const auto *b = static_cast<const unsigned char *>(base);
+	const int aligned = ((reinterpret_cast<uintptr_t>(b) + offset) % kb == 0) && (stride % kb == 0);
+	switch (kb) {
+	case 1: hipLaunchKernelGGL(k_gather_index<uint8_t>, grid_for(n), dim3(256), 0, s, b, stride, offset, idx, static_cast<uint8_t *>(keys), n, aligned); break;
+	case 2: hipLaunchKernelGGL(k_gather_index<uint16_t>, grid_for(n), dim3(256), 0, s, b, stride, offset, idx, static_cast<uint16_t *>(keys), n, aligned); break;
+	case 4: hipLaunchKernelGGL(k_gather_index<uint32_t>, grid_for(n), dim3(256), 0, s, b, stride, offset, idx, static_cast<uint32_t *>(keys), n, aligned); break;
+	case 8: hipLaunchKernelGGL(k_gather_index<uint64_t>, grid_for(n), dim3(256), 0, s, b, stride, offset, idx, static_cast<uint64_t *>(keys), n, aligned); break;
+	default: return hipErrorInvalidValue;
+	}
+	return hipGetLastError();
+}
+
+hipError_t gather_by_pointer(int kb, uint64_t n, const void *const *ptrs, int64_t off1, int second_level, int64_t off2,
+	uint64_t index2, void *keys, hipStream_t s)
+{
+	const auto *p = reinterpret_cast<const unsigned char *const *>(ptrs);
+	switch (kb) {
+	case 1: hipLaunchKernelGGL(k_gather_pointer<uint8_t>, grid_for(n), dim3(256), 0, s, p, off1, second_level, off2, index2, static_cast<uint8_t *>(keys), n); break;
+	case 2: hipLaunchKernelGGL(k_gather_pointer<uint16_t>, grid_for(n), dim3(256), 0, s, p, off1, second_level, off2, index2, static_cast<uint16_t *>(keys), n); break;
+	case 4: hipLaunchKernelGGL(k_gather_pointer<uint32_t>, grid_for(n), dim3(256), 0, s, p, off1, second_level, off2, index2, static_cast<uint32_t *>(keys), n); break;
+	case 8: hipLaunchKernelGGL(k_gather_pointer<uint64_t>, grid_for(n), dim3(256), 0, s, p, off1, second_level, off2, index2, static_cast<uint64_t *>(keys), n); break;
+	default: return hipErrorInvalidValue;
+	}
+	return hipGetLastError();
+}
+
+hipError_t permute_pointers(uint64_t n, const void *const *ptrs, const uint32_t *perm, void **out, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_permute_ptr, grid_for(n), dim3(256), 0, s, ptrs, perm, out, n);
+	return hipGetLastError();
+}
+
+hipError_t permute_u32(uint64_t n, const uint32_t *src, const uint32_t *perm, uint32_t *out, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_permute_u32, grid_for(n), dim3(256), 0, s, src, perm, out, n);
+	return hipGetLastError();
+}
+
+} // namespace rsbd8hip
+
+using namespace rsbd8hip;
+
+namespace {
+
+thread_local std::string g_last_error;
+std::atomic<int> g_tile_cfg{-1};
+
+int fail(int code, const char *what)
+{
+	g_last_error = what;
+	return code;
+}
+
+int fail_hip(hipError_t e, const char *what)
+{
+	g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+	return e == hipErrorOutOfMemory ? RSBD8_HIP_ENOMEM : (e == hipErrorInvalidValue ? RSBD8_HIP_EINVAL : RSBD8_HIP_EDEVICE);
+}
+
+int key_bytes_of(int keytype)
+{
+	switch (keytype) {
+	case RSBD8_HIP_U8: case RSBD8_HIP_I8: return 1;
+	case RSBD8_HIP_U16: case RSBD8_HIP_I16: return 2;
+	case RSBD8_HIP_U32: case RSBD8_HIP_I32: case RSBD8_HIP_F32: return 4;
+	case RSBD8_HIP_U64: case RSBD8_HIP_I64: case RSBD8_HIP_F64: return 8;
+	default: return 0;
+	}
+}
+
+inline size_t align_up(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
+
+struct IndirectWs {
+	void *keys_a, *keys_b;
+	uint32_t *perm_a, *perm_b;
+};
+
+IndirectWs carve_indirect(void *ws, size_t count, int kb)
+{
+	char *p = static_cast<char *>(ws) + align_up(sort_workspace_bytes(count));
+	IndirectWs r;
+	r.keys_a = p; p += align_up(count * kb);
+	r.keys_b = p; p += align_up(count * kb);
+	r.perm_a = reinterpret_cast<uint32_t *>(p); p += align_up(count * 4);
+	r.perm_b = reinterpret_cast<uint32_t *>(p);
+	return r;
+}
+
+bool valid_flags(unsigned flags) { return flags < 32u; }
+
+bool is_device_memory(const void *p)
+{
+	hipPointerAttribute_t attr;
+	if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+		(void)hipGetLastError();
+		return false;
+	}
+	return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+int check_common(int keytype, unsigned flags, size_t count, size_t ws_bytes, int layout)
+{
+	if (!key_bytes_of(keytype)) return fail(RSBD8_HIP_EINVAL, "unknown key type");
+	if (!valid_flags(flags)) return fail(RSBD8_HIP_EINVAL, "unknown flag bits");
+	if (count > 0xFFFFFFFFull) return fail(RSBD8_HIP_EINVAL, "count above 2^32-1 is not supported");
+	if (ws_bytes < rsbd8_hip_workspace_size(keytype, count, layout)) return fail(RSBD8_HIP_EINVAL, "workspace too small");
+	return RSBD8_HIP_OK;
+}
+
+bool needs_buffer(int kb, unsigned flags)
+{
+	if (kb != 1) return true;
+	const unsigned m = flags & 7u;
+	return m == (RSBD8_HIP_ABS | RSBD8_HIP_SIGN) || m == (RSBD8_HIP_SIGN | RSBD8_HIP_FLTP) ||
+		m == (RSBD8_HIP_ABS | RSBD8_HIP_SIGN | RSBD8_HIP_FLTP);
+}
+
+} // namespace
+
+extern "C" {
+
+int rsbd8_hip_decode(int keytype, unsigned mode, unsigned direction, int indirect, unsigned *flags)
+{
+	if (!key_bytes_of(keytype) || !flags) return fail(RSBD8_HIP_EINVAL, "bad key type or flags pointer");
+	if (!(mode <= 7u || (mode >= 64u && mode <= 66u))) return fail(RSBD8_HIP_EINVAL, "not a sortingmode value");
+	if (direction > 3u) return fail(RSBD8_HIP_EINVAL, "not a sortingdirection value");
+	const bool issigned = keytype >= RSBD8_HIP_I8;   // std::is_signed_v: true for float and double
+	const bool isfloat = keytype >= RSBD8_HIP_F32;   // std::is_floating_point_v
+	bool abs, sign, fltp;
+	if (mode >= 64u) {
+		abs = mode >= 65u && issigned;    // H:59779-59780
+		sign = mode <= 65u && issigned;   // H:59809-59811
+		fltp = isfloat;                   // H:59840-59842
+	} else {
+		abs = mode & 1u;                  // H:59804
+		sign = (mode >> 1) & 1u;          // H:59835
+		fltp = (mode >> 2) & 1u;          // H:59865
+	}
+	unsigned f = (abs ? RSBD8_HIP_ABS : 0u) | (sign ? RSBD8_HIP_SIGN : 0u) | (fltp ? RSBD8_HIP_FLTP : 0u);
+	if (direction & 1u) f |= RSBD8_HIP_DESC;  // H:59867-59870
+	// tie reversal: always with indirection (H:59872-59875), otherwise only where distinct bit
+	// patterns can share a key (H:59877-59904)
+	if ((direction & 2u) && (indirect || (abs && sign) || (!abs && sign && fltp))) f |= RSBD8_HIP_REV;
+	*flags = f;
+	return RSBD8_HIP_OK;
+}
+
+size_t rsbd8_hip_workspace_size(int keytype, size_t count, int layout)
+{
+	const int kb = key_bytes_of(keytype);
+	size_t s = sort_workspace_bytes(count);
+	if (layout == RSBD8_HIP_LAYOUT_INDIRECT) s = align_up(s) + 2 * align_up(count * (kb ? kb : 8)) + 2 * align_up(count * 4);
+	return s;
+}
+
+int rsbd8_hip_workspace_init(void *workspace, size_t bytes, rsbd8_hip_stream stream)
+{
+	if (!workspace) return fail(RSBD8_HIP_EINVAL, "null workspace");
+	const hipError_t e = workspace_init(workspace, bytes, static_cast<hipStream_t>(stream));
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "workspace_init");
+}
+
+int rsbd8_hip_set_tile_config(int cfg)
+{
+	if (cfg < -1 || cfg >= kNumTileCfgs) return fail(RSBD8_HIP_EINVAL, "tile config out of range");
+	return g_tile_cfg.exchange(cfg);
+}
+
+const char *rsbd8_hip_last_error(void) { return g_last_error.c_str(); }
+
+int rsbd8_hip_profile_begin(void)
+{
+	profile_begin();
+	return RSBD8_HIP_OK;
+}
+
+int rsbd8_hip_profile_end(double *ms, unsigned *launches, int nkinds)
+{
+	if (!ms || !launches || nkinds < 0) return fail(RSBD8_HIP_EINVAL, "null output");
+	const hipError_t e = profile_end(ms, launches, nkinds);
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "profile_end");
+}
+
+int rsbd8_hip_radixsortcopynoalloc(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
+	const void *input, void *output, void *buffer, rsbd8_hip_stream stream)
+{
+	int r = check_common(keytype, flags, count, workspace_bytes, RSBD8_HIP_LAYOUT_KEYS);
+	if (r) return r;
+	const int kb = key_bytes_of(keytype);
+	if (!count) return RSBD8_HIP_OK;
+	if (!input || !output || !workspace) return fail(RSBD8_HIP_EINVAL, "null array");
+	if (input == output || (buffer && (buffer == input || buffer == output))) return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	if (!buffer && needs_buffer(kb, flags) && kb != 1) return fail(RSBD8_HIP_EINVAL, "buffer required");
+	SortJob job;
+	job.key_bytes = kb; job.flags = flags; job.n = count;
+	job.in = input; job.out = output; job.buf = buffer;
+	job.cfg = g_tile_cfg.load();
+	const hipError_t e = sort_launch(job, workspace, workspace_bytes, static_cast<hipStream_t>(stream));
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "radixsortcopynoalloc");
+}
+
+int rsbd8_hip_radixsortnoalloc(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
+	void *input, void *buffer, int movetobuffer, rsbd8_hip_stream stream)
+{
+	int r = check_common(keytype, flags, count, workspace_bytes, RSBD8_HIP_LAYOUT_KEYS);
+	if (r) return r;
+	const int kb = key_bytes_of(keytype);
+	if (!count) return RSBD8_HIP_OK;
+	if (!input || !workspace) return fail(RSBD8_HIP_EINVAL, "null array");
+	if (buffer == input) return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	if (!buffer && (needs_buffer(kb, flags) || movetobuffer)) return fail(RSBD8_HIP_EINVAL, "buffer required");
+	SortJob job;
+	job.key_bytes = kb; job.flags = flags; job.n = count;
+	job.in = input;
+	job.out = movetobuffer ? buffer : input;
+	job.buf = movetobuffer ? input : buffer;
+	job.cfg = g_tile_cfg.load();
+	const hipError_t e = sort_launch(job, workspace, workspace_bytes, static_cast<hipStream_t>(stream));
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "radixsortnoalloc");
+}
+
+int rsbd8_hip_sort_pairs(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
+	const void *keys_in, void *keys_out, void *keys_buffer, const uint32_t *payload_in, uint32_t *payload_out,
+	uint32_t *payload_buffer, rsbd8_hip_stream stream)
+{
+	int r = check_common(keytype, flags, count, workspace_bytes, RSBD8_HIP_LAYOUT_PAIRS);
+	if (r) return r;
+	if (!count) return RSBD8_HIP_OK;
+	if (!keys_in || !keys_out || !keys_buffer || !payload_out || !payload_buffer || !workspace)
+		return fail(RSBD8_HIP_EINVAL, "null array");
+	if (keys_in == keys_out || keys_in == keys_buffer || keys_out == keys_buffer || payload_out == payload_buffer ||
+		(payload_in && (payload_in == payload_out || payload_in == payload_buffer)))
+		return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	SortJob job;
+	job.key_bytes = key_bytes_of(keytype); job.flags = flags; job.n = count;
+	job.in = keys_in; job.out = keys_out; job.buf = keys_buffer;
+	job.payload = true; job.pin = payload_in; job.pout = payload_out; job.pbuf = payload_buffer;
+	job.cfg = g_tile_cfg.load();
+	const hipError_t e = sort_launch(job, workspace, workspace_bytes, static_cast<hipStream_t>(stream));
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "sort_pairs");
+}
+
+int rsbd8_hip_sort_indices(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
+	const void *base, size_t stride, size_t key_offset, const uint32_t *idx_in, uint32_t *idx_out, rsbd8_hip_stream stream)
+{
+	int r = check_common(keytype, flags, count, workspace_bytes, RSBD8_HIP_LAYOUT_INDIRECT);
+	if (r) return r;
+	if (!count) return RSBD8_HIP_OK;
+	if (!base || !idx_in || !idx_out || !workspace) return fail(RSBD8_HIP_EINVAL, "null array");
+	if (idx_in == idx_out) return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	const int kb = key_bytes_of(keytype);
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	IndirectWs w = carve_indirect(workspace, count, kb);
+	hipError_t e = gather_by_index(kb, count, base, stride, key_offset, idx_in, w.keys_a, s);
+	if (e != hipSuccess) return fail_hip(e, "gather_by_index");
+	SortJob job;
+	job.key_bytes = kb; job.flags = flags; job.n = count;
+	job.in = w.keys_a; job.out = w.keys_b; job.buf = w.keys_a;  // keys end wherever the parity puts them
+	job.keys_final_any = true;
+	job.payload = true; job.pin = idx_in; job.pout = idx_out; job.pbuf = w.perm_a;
+	job.cfg = g_tile_cfg.load();
+	e = sort_launch(job, workspace, workspace_bytes, s);
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "sort_indices");
+}
+
+static int sort_pointers(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
+	const void *const *input, void **output, ptrdiff_t ind1, int second, ptrdiff_t ind2, size_t index2, hipStream_t s)
+{
+	const int kb = key_bytes_of(keytype);
+	IndirectWs w = carve_indirect(workspace, count, kb);
+	hipError_t e = gather_by_pointer(kb, count, input, ind1, second, ind2, index2, w.keys_a, s);
+	if (e != hipSuccess) return fail_hip(e, "gather_by_pointer");
+	SortJob job;
+	job.key_bytes = kb; job.flags = flags; job.n = count;
+	job.in = w.keys_a; job.out = w.keys_b; job.buf = w.keys_a;
+	job.keys_final_any = true;
+	job.payload = true; job.pin = nullptr; job.pout = w.perm_b; job.pbuf = w.perm_a;
+	job.cfg = g_tile_cfg.load();
+	if ((e = sort_launch(job, workspace, workspace_bytes, s)) != hipSuccess) return fail_hip(e, "sort_pointers");
+	if ((e = permute_pointers(count, input, w.perm_b, output, s)) != hipSuccess) return fail_hip(e, "permute_pointers");
+	return RSBD8_HIP_OK;
+}
+
+int rsbd8_hip_radixsortcopynoalloc_indirect(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
+	const void *const *input, void **output, ptrdiff_t indirection1, int second_level, ptrdiff_t indirection2, size_t index2,
+	rsbd8_hip_stream stream)
+{
+	int r = check_common(keytype, flags, count, workspace_bytes, RSBD8_HIP_LAYOUT_INDIRECT);
+	if (r) return r;
+	if (!count) return RSBD8_HIP_OK;
+	if (!input || !output || !workspace) return fail(RSBD8_HIP_EINVAL, "null array");
+	if (static_cast<const void *>(input) == static_cast<const void *>(output)) return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	return sort_pointers(keytype, flags, count, workspace, workspace_bytes, input, output, indirection1, second_level,
+		indirection2, index2, static_cast<hipStream_t>(stream));
+}
+
+int rsbd8_hip_radixsortnoalloc_indirect(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
+	void **input, void **buffer, int movetobuffer, ptrdiff_t indirection1, int second_level, ptrdiff_t indirection2,
+	size_t index2, rsbd8_hip_stream stream)
+{
+	int r = check_common(keytype, flags, count, workspace_bytes, RSBD8_HIP_LAYOUT_INDIRECT);
+	if (r) return r;
+	if (!count) return RSBD8_HIP_OK;
+	if (!input || !buffer || !workspace) return fail(RSBD8_HIP_EINVAL, "null array");
+	if (input == buffer) return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	hipStream_t s = static_cast<hipStream_t>(stream);
+	// permute into buffer, then copy back unless the result is wanted in buffer
+	r = sort_pointers(keytype, flags, count, workspace, workspace_bytes, input, buffer, indirection1, second_level,
+		indirection2, index2, s);
+	if (r || movetobuffer) return r;
+	const hipError_t e = hipMemcpyAsync(input, buffer, count * sizeof(void *), hipMemcpyDeviceToDevice, s);
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "copy back");
+}
+
+static int sort_allocating(int keytype, unsigned flags, size_t count, const void *input, void *output, bool inplace,
+	hipStream_t s)
+{
+	const int kb = key_bytes_of(keytype);
+	const size_t bytes = count * kb;
+	const size_t wsb = rsbd8_hip_workspace_size(keytype, count, RSBD8_HIP_LAYOUT_KEYS);
+	const bool dev_in = is_device_memory(input), dev_out = inplace ? dev_in : is_device_memory(output);
+	char *mem = nullptr;
+	// one allocation: workspace, buffer, and staging arrays for host-side input/output
+	const size_t need = align_up(wsb) + align_up(bytes) + (dev_in ? 0 : align_up(bytes)) + (dev_out ? 0 : align_up(bytes));
+	hipError_t e = hipMallocAsync(reinterpret_cast<void **>(&mem), need, s);
+	if (e != hipSuccess) return fail_hip(e, "allocation");
+	char *ws = mem, *buf = mem + align_up(wsb), *p = buf + align_up(bytes);
+	const void *din = input;
+	void *dout = output;
+	if (!dev_in) { din = p; p += align_up(bytes); }
+	if (!dev_out) dout = inplace ? const_cast<void *>(din) : p;
+	int r = RSBD8_HIP_OK;
+	if ((e = workspace_init(ws, wsb, s)) != hipSuccess) { r = fail_hip(e, "workspace_init"); goto out; }
+	if (!dev_in && (e = hipMemcpyAsync(const_cast<void *>(din), input, bytes, hipMemcpyHostToDevice, s)) != hipSuccess) {
+		r = fail_hip(e, "upload");
+		goto out;
+	}
+	{
+		SortJob job;
+		job.key_bytes = kb; job.flags = flags; job.n = count;
+		job.in = din; job.out = dout; job.buf = buf;
+		job.cfg = g_tile_cfg.load();
+		if ((e = sort_launch(job, ws, wsb, s)) != hipSuccess) { r = fail_hip(e, "sort"); goto out; }
+	}
+	if (!dev_out && (e = hipMemcpyAsync(output, dout, bytes, hipMemcpyDeviceToHost, s)) != hipSuccess) {
+		r = fail_hip(e, "download");
+		goto out;
+	}
+out:
+	(void)hipFreeAsync(mem, s);
+	if ((e = hipStreamSynchronize(s)) != hipSuccess && r == RSBD8_HIP_OK) r = fail_hip(e, "synchronize");
+	return r;
+}
+
+int rsbd8_hip_radixsortcopy(int keytype, unsigned flags, size_t count, const void *input, void *output, rsbd8_hip_stream stream)
+{
+	if (!key_bytes_of(keytype) || !valid_flags(flags)) return fail(RSBD8_HIP_EINVAL, "bad key type or flags");
+	if (count > 0xFFFFFFFFull) return fail(RSBD8_HIP_EINVAL, "count above 2^32-1 is not supported");
+	if (!input || !output) return fail(RSBD8_HIP_EINVAL, "null array");
+	if (input == output) return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	if (count == 0) return RSBD8_HIP_OK;
+	if (count == 1) {  // the reference copies the single element (H:60525)
+		const hipError_t e = hipMemcpyAsync(output, input, key_bytes_of(keytype), hipMemcpyDefault, static_cast<hipStream_t>(stream));
+		if (e != hipSuccess) return fail_hip(e, "copy");
+		const hipError_t e2 = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+		return e2 == hipSuccess ? RSBD8_HIP_OK : fail_hip(e2, "synchronize");
+	}
+	return sort_allocating(keytype, flags, count, input, output, false, static_cast<hipStream_t>(stream));
+}
+
+int rsbd8_hip_radixsort(int keytype, unsigned flags, size_t count, void *input, rsbd8_hip_stream stream)
+{
+	if (!key_bytes_of(keytype) || !valid_flags(flags)) return fail(RSBD8_HIP_EINVAL, "bad key type or flags");
+	if (count > 0xFFFFFFFFull) return fail(RSBD8_HIP_EINVAL, "count above 2^32-1 is not supported");
+	if (!input) return fail(RSBD8_HIP_EINVAL, "null array");
+	if (count <= 1) return RSBD8_HIP_OK;  // already sorted (H:60465)
+	return sort_allocating(keytype, flags, count, input, input, true, static_cast<hipStream_t>(stream));
+}
+
+} // extern "C"

@@ -1,0 +1,126 @@
+// rsbd8_device.hpp -- device-side building blocks of the gfx950 rsbd8 radix sort.
+//
+// The sort is an 8-bit-digit LSD radix sort over a "sort key" derived from the raw key bits.
+// Every mode of the reference (Radixsortbidi8.hpp = H, filterinput H:10000-10027, the virtual
+// top-bit flips of the offsets transform H:11642-11801) reduces to one branch-free transform
+//     k = (((((x & A) << sh) | (s & B)) ^ (smask & C)) + (s & D)) ^ E
+// with s the sign bit of x and smask = -s, with descending order folded into E (complement).
+// Arrays always hold the original bits; the transform is recomputed on every load.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rsbd8hip {
+
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 256;
+constexpr int kMaxPasses = 8;
+
+// flag bits of the C-ABI (include/rsbd8_hip.h)
+constexpr unsigned kAbs = 1u, kSign = 2u, kFltp = 4u, kDesc = 8u, kRev = 16u;
+
+struct KeyXform {
+	uint64_t A, B, C, D, E;
+	uint32_t sh;
+	uint32_t pad;
+};
+
+// host+device: transform parameters for a W-bit key with the decoded flags
+inline __host__ __device__ KeyXform make_xform(unsigned width_bits, unsigned flags)
+{
+	const uint64_t ones = width_bits >= 64 ? ~0ull : ((1ull << width_bits) - 1ull);
+	const uint64_t msb = 1ull << (width_bits - 1);
+	KeyXform t{ones, 0, 0, 0, 0, 0, 0};
+	switch (flags & 7u) {
+	case 0u: break;                                              // forceunsigned
+	case kSign: t.E = msb; break;                                // forcesigned
+	case kAbs | kSign: t.C = ones; t.D = 1; break;               // forceabssigned: -x for x < 0
+	case kAbs: t.sh = 1; t.C = ones; break;                      // forcetieredabssigned
+	case kSign | kFltp: t.A = ones & ~msb; t.C = ones; t.D = 1; t.E = msb; break; // forcefloatingp
+	case kAbs | kSign | kFltp: t.A = ones & ~msb; break;         // forceabsfloatingp
+	case kFltp: t.A = ones & ~msb; t.C = ones; break;            // specialfloatingp
+	default: t.sh = 1; t.B = 1; t.E = 1; break;                  // forcetieredabsfloatingp
+	}
+	if (flags & kDesc) t.E ^= ones;
+	return t;
+}
+
+template <typename K> struct KeyTraits;
+template <> struct KeyTraits<uint8_t> { using W = uint32_t; static constexpr int bits = 8; };
+template <> struct KeyTraits<uint16_t> { using W = uint32_t; static constexpr int bits = 16; };
+template <> struct KeyTraits<uint32_t> { using W = uint32_t; static constexpr int bits = 32; };
+template <> struct KeyTraits<uint64_t> { using W = uint64_t; static constexpr int bits = 64; };
+
+// the transform in the key's working width; bits above the key width are never looked at
+template <typename K>
+__device__ __forceinline__ typename KeyTraits<K>::W sort_key(K raw, const KeyXform &t)
+{
+	using W = typename KeyTraits<K>::W;
+	const W x = static_cast<W>(raw);
+	const W s = x >> (KeyTraits<K>::bits - 1);
+	const W smask = W(0) - s;
+	W k = (x & static_cast<W>(t.A)) << t.sh;
+	k |= s & static_cast<W>(t.B);
+	k ^= smask & static_cast<W>(t.C);
+	k += s & static_cast<W>(t.D);
+	return k ^ static_cast<W>(t.E);
+}
+
+template <typename K>
+__device__ __forceinline__ uint32_t digit_of(K raw, const KeyXform &t, uint32_t shift)
+{
+	return static_cast<uint32_t>(sort_key<K>(raw, t) >> shift) & 0xFFu;
+}
+
+// lanes of the wave holding the same 8-bit digit (wave64 ballot match)
+__device__ __forceinline__ uint64_t match_digit(uint32_t d)
+{
+	uint64_t m = ~0ull;
+#pragma unroll
+	for (int b = 0; b < 8; ++b) {
+		const bool bit = (d >> b) & 1u;
+		const uint64_t v = __ballot(bit);
+		m &= bit ? v : ~v;
+	}
+	return m;
+}
+
+// number of set bits of m in lanes below this one
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m)
+{
+	return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+		__builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// inclusive wave64 prefix sum
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t v)
+{
+	const uint32_t lane = lane_id();
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+		const uint32_t u = __shfl_up(v, o, 64);
+		if (lane >= static_cast<uint32_t>(o)) v += u;
+	}
+	return v;
+}
+
+// look-back word: value in bits 0..31, status in bits 32..33, pass tag in bits 34..41
+constexpr uint64_t kLbAggregate = 1ull << 32;
+constexpr uint64_t kLbInclusive = 2ull << 32;
+__device__ __forceinline__ uint64_t lb_word(uint32_t value, uint64_t status, uint32_t tag)
+{
+	return static_cast<uint64_t>(value) | status | (static_cast<uint64_t>(tag) << 34);
+}
+
+__device__ __forceinline__ uint64_t atomic_load_agent(const uint64_t *p)
+{
+	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_store_agent(uint64_t *p, uint64_t v)
+{
+	__hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+} // namespace rsbd8hip

@@ -1,0 +1,82 @@
+// rsbd8_engine.hpp -- host-side interface of the gfx950 sort engine (internal to the library).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "rsbd8_device.hpp"
+
+namespace rsbd8hip {
+
+// Tile configurations of the onesweep scatter kernel: threads per block x keys per thread.
+struct TileCfg {
+	int threads;
+	int kpt;
+};
+constexpr int kNumTileCfgs = 5;
+constexpr TileCfg kTileCfgs[kNumTileCfgs] = {{256, 8}, {256, 16}, {512, 8}, {512, 16}, {256, 24}};
+constexpr int kMinTile = 256 * 8;
+
+// One planned pass as the device sees it (written by the plan kernel, read by the scatter kernel).
+struct PassPlan {
+	const void *src;
+	void *dst;
+	const uint32_t *psrc;
+	uint32_t *pdst;
+	uint32_t active;
+	uint32_t reverse;  // read the source back to front (isrevorder on the first read)
+	uint32_t iota;     // payload is the physical source index (no payload array to read)
+	uint32_t pad;
+};
+
+// Device control block at the start of every workspace.
+struct Control {
+	uint32_t tile_counter[kMaxPasses];
+	uint32_t epoch;        // advanced by every sort; tags the look-back words of each pass
+	uint32_t num_active;
+	uint32_t active_mask;
+	uint32_t pad[5];
+	PassPlan pass[kMaxPasses];
+	PassPlan pre;          // copy before the first pass (in-place sorts that would overlap)
+	PassPlan fin;          // copy when every digit is uniform (no pass executes)
+};
+
+constexpr size_t kCtrlBytes = 4096;
+constexpr size_t kHistBytes = kMaxPasses * kRadix * sizeof(uint32_t);
+constexpr size_t kLookbackOffset = kCtrlBytes + 2 * kHistBytes;
+static_assert(sizeof(Control) <= kCtrlBytes, "control block too large");
+
+struct SortJob {
+	int key_bytes = 4;             // 1, 2, 4 or 8
+	unsigned flags = 0;            // kAbs | kSign | kFltp | kDesc | kRev (already decoded)
+	uint64_t n = 0;
+	const void *in = nullptr;      // source keys (may equal out or buf for in-place sorts)
+	void *out = nullptr;           // final keys
+	void *buf = nullptr;           // the other ping-pong array (may be null when never needed)
+	bool payload = false;          // carry a u32 payload along
+	const uint32_t *pin = nullptr; // source payload; null means "source index"
+	uint32_t *pout = nullptr;
+	uint32_t *pbuf = nullptr;
+	int cfg = -1;                  // tile configuration index, -1 picks the default
+	bool fill8 = true;             // 8-bit keys-only bijective modes may use the count fill
+	bool keys_final_any = false;   // only the payload is wanted: keys may end in out or buf
+};
+
+size_t sort_workspace_bytes(uint64_t n);
+hipError_t workspace_init(void *ws, size_t bytes, hipStream_t stream);
+hipError_t sort_launch(const SortJob &job, void *ws, size_t ws_bytes, hipStream_t stream);
+int default_cfg(int key_bytes, uint64_t n, bool payload);
+
+// Gather keys through records or pointers into a dense key array (indirection layouts).
+hipError_t gather_by_index(int key_bytes, uint64_t n, const void *base, uint64_t stride, uint64_t offset,
+	const uint32_t *idx, void *keys_out, hipStream_t stream);
+hipError_t gather_by_pointer(int key_bytes, uint64_t n, const void *const *ptrs, int64_t off1, int second_level,
+	int64_t off2, uint64_t index2, void *keys_out, hipStream_t stream);
+hipError_t permute_pointers(uint64_t n, const void *const *ptrs, const uint32_t *perm, void **out, hipStream_t stream);
+hipError_t permute_u32(uint64_t n, const uint32_t *src, const uint32_t *perm, uint32_t *out, hipStream_t stream);
+
+int device_cu_count();
+void profile_begin();
+hipError_t profile_end(double *ms, unsigned *launches, int nkinds);
+
+} // namespace rsbd8hip

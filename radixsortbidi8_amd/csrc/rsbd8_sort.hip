@@ -1,0 +1,577 @@
+// rsbd8_sort.hip -- gfx950 kernels of the stable 8-bit LSD radix sort.
+//
+// Replaces the reference's counting pass (Radixsortbidi8.hpp = H, H:32107-32473), its offsets
+// transform with pass skipping (H:11626-12946) and its bidirectional scatter passes
+// (H:31649-31982) with:
+//   k_hist      one HBM read of the keys building every digit's 256-bin histogram at once in
+//               LDS (wave-uniform digits are added with one atomic per wave);
+//   k_plan      one block: per-digit exclusive scans (bucket starts), uniform-digit detection
+//               (the pass-skip flag of H:11652) and the ping-pong buffer plan, all on device;
+//   k_onesweep  one launch per digit: persistent blocks take tiles in order, rank digits with
+//               wave64 ballot matching, publish per-tile digit counts and resolve the global
+//               offsets with a decoupled look-back, stage the tile in LDS in digit order and
+//               write coalesced runs; inactive (uniform-digit) passes exit at once;
+//   k_copy      copies when no pass runs or an in-place sort needs a staging copy;
+//   k_fill8     the 8-bit count fill of H:48769-48957 for bijective modes;
+//   k_finish    resets the tile counters and advances the look-back epoch.
+// The transform of every mode is fused into each load (rsbd8_device.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "rsbd8_engine.hpp"
+
+namespace rsbd8hip {
+
+namespace {
+
+inline Control *ctrl_of(void *ws) { return reinterpret_cast<Control *>(ws); }
+inline uint32_t *hist_of(void *ws) { return reinterpret_cast<uint32_t *>(static_cast<char *>(ws) + kCtrlBytes); }
+inline uint32_t *bucket_of(void *ws) { return reinterpret_cast<uint32_t *>(static_cast<char *>(ws) + kCtrlBytes + kHistBytes); }
+inline uint64_t *lookback_of(void *ws) { return reinterpret_cast<uint64_t *>(static_cast<char *>(ws) + kLookbackOffset); }
+
+__device__ __forceinline__ uint32_t pass_tag(uint32_t epoch, int pass)
+{
+	return ((epoch * kMaxPasses + static_cast<uint32_t>(pass)) % 0x3FFFFFFFu) + 1u;
+}
+
+// ---------------------------------------------------------------------------------------------
+// histogram of every digit in one read
+
+template <typename K, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_hist(const K *__restrict__ in, uint64_t n, KeyXform xf,
+	uint32_t *__restrict__ hist)
+{
+	constexpr int P = sizeof(K);
+	__shared__ uint32_t s_h[P * kRadix];
+	for (int i = threadIdx.x; i < P * kRadix; i += THREADS) s_h[i] = 0;
+	__syncthreads();
+	const uint64_t stride = static_cast<uint64_t>(gridDim.x) * THREADS;
+	for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * THREADS + threadIdx.x; i < n; i += stride) {
+		const auto k = sort_key<K>(in[i], xf);
+		const uint64_t active = __ballot(1);
+#pragma unroll
+		for (int p = 0; p < P; ++p) {
+			const uint32_t d = static_cast<uint32_t>(k >> (8 * p)) & 0xFFu;
+			const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+			if (__ballot(d == d0) == active) {
+				// the whole wave shares this digit: one atomic instead of a 64-way conflict
+				if (lanes_below(active) == 0) atomicAdd(&s_h[p * kRadix + d0], static_cast<uint32_t>(__popcll(active)));
+			} else {
+				atomicAdd(&s_h[p * kRadix + d], 1u);
+			}
+		}
+	}
+	__syncthreads();
+	for (int i = threadIdx.x; i < P * kRadix; i += THREADS)
+		if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+// ---------------------------------------------------------------------------------------------
+// plan: bucket starts, pass skipping and buffer assignment
+
+struct PlanArgs {
+	uint64_t n;
+	int passes;
+	const void *src;
+	void *fin;
+	void *other;
+	const uint32_t *psrc;  // null: payload is the source index
+	uint32_t *pfin;
+	uint32_t *pother;
+	int payload;
+	int reverse;
+	int keys_final_any;
+};
+
+__global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_t *hist, uint32_t *bucket)
+{
+	__shared__ uint32_t s_wave[4];
+	__shared__ uint32_t s_uniform[kMaxPasses];
+	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+	for (int p = 0; p < a.passes; ++p) {
+		const uint32_t c = hist[p * kRadix + t];
+		hist[p * kRadix + t] = 0;  // leave the histogram zeroed for the next sort
+		const uint32_t incl = wave_inclusive_sum(c);
+		if (lane == 63) s_wave[w] = incl;
+		const bool full = static_cast<uint64_t>(c) == a.n;
+		__syncthreads();
+		uint32_t before = 0;
+		for (uint32_t v = 0; v < w; ++v) before += s_wave[v];
+		bucket[p * kRadix + t] = before + incl - c;
+		if (t == 0) s_uniform[p] = 0;
+		__syncthreads();
+		if (full) s_uniform[p] = 1;
+		__syncthreads();
+	}
+	if (t != 0) return;
+	int A = 0;
+	uint32_t mask = 0;
+	for (int p = 0; p < a.passes; ++p)
+		if (!s_uniform[p]) { ++A; mask |= 1u << p; }
+	ctrl->num_active = static_cast<uint32_t>(A);
+	ctrl->active_mask = mask;
+	void *X[2] = {a.fin, a.other};
+	uint32_t *PX[2] = {a.pfin, a.pother};
+	PassPlan none{};
+	ctrl->pre = none;
+	ctrl->fin = none;
+	for (int p = 0; p < kMaxPasses; ++p) ctrl->pass[p] = none;
+	if (A == 0) {
+		PassPlan &f = ctrl->fin;
+		f.src = a.src; f.dst = a.fin; f.psrc = a.psrc; f.pdst = a.pfin;
+		f.iota = a.payload && !a.psrc;
+		f.reverse = a.reverse;
+		f.active = (a.src != a.fin) || a.reverse || (a.payload && (f.iota || a.psrc != a.pfin));
+		return;
+	}
+	const void *src = a.src;
+	const uint32_t *psrc = a.psrc;
+	int rev = a.reverse, iota = a.payload && !a.psrc;
+	const int first = (A - 1) & 1;
+	if (a.keys_final_any && src == X[first]) {
+		// the caller only wants the payload: let the keys end in the other array
+		void *t = X[0]; X[0] = X[1]; X[1] = t;
+	}
+	if (src == X[first] || (a.payload && !iota && psrc == PX[first])) {
+		// an in-place sort would read and write the same array in its first pass: stage the
+		// source in the other array first (the copy also applies the tie reversal)
+		PassPlan &c = ctrl->pre;
+		c.active = 1; c.src = src; c.dst = X[first ^ 1]; c.psrc = psrc; c.pdst = PX[first ^ 1];
+		c.reverse = rev; c.iota = iota;
+		src = X[first ^ 1]; psrc = PX[first ^ 1]; rev = 0; iota = 0;
+	}
+	int j = 0;
+	for (int p = 0; p < a.passes; ++p) {
+		if (!(mask >> p & 1u)) continue;
+		PassPlan &pp = ctrl->pass[p];
+		const int slot = (A - 1 - j) & 1;
+		pp.active = 1; pp.src = src; pp.dst = X[slot]; pp.psrc = psrc; pp.pdst = PX[slot];
+		pp.reverse = rev; pp.iota = iota;
+		src = X[slot]; psrc = PX[slot]; rev = 0; iota = 0;
+		++j;
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// copy (staging copy before the first pass, or the whole result when no pass runs)
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_copy(const Control *ctrl, int which, uint64_t n, int payload)
+{
+	const PassPlan c = which ? ctrl->fin : ctrl->pre;
+	if (!c.active) return;
+	const K *src = static_cast<const K *>(c.src);
+	K *dst = static_cast<K *>(c.dst);
+	const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+	if (c.reverse && c.src == c.dst) {
+		// in-place reversal: swap mirrored pairs
+		for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n / 2; i += stride) {
+			const K a = dst[i], b = dst[n - 1 - i];
+			dst[i] = b; dst[n - 1 - i] = a;
+			if (payload) {
+				if (c.iota) { c.pdst[i] = static_cast<uint32_t>(n - 1 - i); c.pdst[n - 1 - i] = static_cast<uint32_t>(i); }
+				else { const uint32_t pa = c.pdst[i], pb = c.pdst[n - 1 - i]; c.pdst[i] = pb; c.pdst[n - 1 - i] = pa; }
+			}
+		}
+		if (payload && c.iota && (n & 1) && blockIdx.x == 0 && threadIdx.x == 0) c.pdst[n / 2] = static_cast<uint32_t>(n / 2);
+		return;
+	}
+	for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+		const uint64_t s = c.reverse ? n - 1 - i : i;
+		if (c.src != c.dst) dst[i] = src[s];
+		if (payload) c.pdst[i] = c.iota ? static_cast<uint32_t>(s) : c.psrc[s];
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// onesweep scatter pass
+
+template <typename K, bool PAY, int THREADS, int KPT>
+__global__ __launch_bounds__(THREADS) void k_onesweep(Control *ctrl, const uint32_t *__restrict__ bucket,
+	uint64_t *__restrict__ lookback, uint64_t n, uint32_t num_tiles, KeyXform xf, int pass)
+{
+	constexpr int TILE = THREADS * KPT;
+	constexpr int NW = THREADS / 64;
+	static_assert(THREADS >= 256 && THREADS % 64 == 0, "one thread per digit is needed");
+	const PassPlan pp = ctrl->pass[pass];
+	if (!pp.active) return;
+	const K *__restrict__ src = static_cast<const K *>(pp.src);
+	K *__restrict__ dst = static_cast<K *>(pp.dst);
+	const uint32_t tag = pass_tag(ctrl->epoch, pass);
+	const uint32_t shift = 8u * static_cast<uint32_t>(pass);
+
+	__shared__ uint32_t s_whist[NW * kRadix];
+	__shared__ uint32_t s_digoff[kRadix];
+	__shared__ uint32_t s_glob[kRadix];
+	__shared__ uint32_t s_wsum[4];
+	__shared__ uint32_t s_tile;
+	__shared__ uint8_t s_dig[PAY ? TILE : 1];
+	constexpr int kStage = (PAY && sizeof(K) < 4) ? 4 : static_cast<int>(sizeof(K));
+	__shared__ __align__(16) unsigned char s_raw[TILE * kStage];
+	K *s_keys = reinterpret_cast<K *>(s_raw);
+
+	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+	const uint32_t *pbucket = bucket + pass * kRadix;
+
+	for (;;) {
+		if (t == 0) s_tile = atomicAdd(&ctrl->tile_counter[pass], 1u);
+		for (int i = t; i < NW * kRadix; i += THREADS) s_whist[i] = 0;
+		__syncthreads();
+		const uint32_t tile = s_tile;
+		if (tile >= num_tiles) break;
+		const uint64_t base = static_cast<uint64_t>(tile) * TILE;
+		const uint32_t valid = (n - base) < static_cast<uint64_t>(TILE) ? static_cast<uint32_t>(n - base) : static_cast<uint32_t>(TILE);
+
+		K keys[KPT];
+		uint32_t dig[KPT];
+		uint32_t rank[KPT];
+		uint32_t pay[PAY ? KPT : 1];
+#pragma unroll
+		for (int j = 0; j < KPT; ++j) {
+			const uint32_t l = w * 64u * KPT + j * 64u + lane;
+			const uint64_t g = base + l;
+			if (l < valid) {
+				const uint64_t s = pp.reverse ? n - 1 - g : g;
+				keys[j] = src[s];
+				dig[j] = digit_of<K>(keys[j], xf, shift);
+				if constexpr (PAY) pay[j] = pp.iota ? static_cast<uint32_t>(s) : pp.psrc[s];
+			} else {
+				keys[j] = K(0);
+				dig[j] = 0xFFu;  // past the end: ranked last, never written
+				if constexpr (PAY) pay[j] = 0;
+			}
+		}
+		// rank within the wave in logical order (j-major, lane-minor)
+#pragma unroll
+		for (int j = 0; j < KPT; ++j) {
+			const uint64_t peers = match_digit(dig[j]);
+			const uint32_t below = lanes_below(peers);
+			uint32_t *slot = &s_whist[w * kRadix + dig[j]];
+			const uint32_t old = *slot;
+			if (below == 0) *slot = old + static_cast<uint32_t>(__popcll(peers));
+			rank[j] = old + below;
+		}
+		__syncthreads();
+		uint32_t total = 0, pub = 0;
+		if (t < kRadix) {
+			for (int v = 0; v < NW; ++v) {
+				const uint32_t c = s_whist[v * kRadix + t];
+				s_whist[v * kRadix + t] = total;
+				total += c;
+			}
+			pub = total - ((t == kRadix - 1) ? static_cast<uint32_t>(TILE) - valid : 0u);
+			atomic_store_agent(&lookback[static_cast<uint64_t>(tile) * kRadix + t],
+				lb_word(pub, tile == 0 ? kLbInclusive : kLbAggregate, tag));
+			const uint32_t incl = wave_inclusive_sum(total);
+			if (lane == 63) s_wsum[w] = incl;
+			s_digoff[t] = incl - total;
+		}
+		__syncthreads();
+		if (t < kRadix) {
+			uint32_t before = 0;
+			for (uint32_t v = 0; v < w; ++v) before += s_wsum[v];
+			const uint32_t digoff = s_digoff[t] + before;
+			s_digoff[t] = digoff;
+			// decoupled look-back over the preceding tiles of this digit
+			uint32_t excl = 0;
+			if (tile > 0) {
+				int64_t prev = static_cast<int64_t>(tile) - 1;
+				for (;;) {
+					const uint64_t v = atomic_load_agent(&lookback[static_cast<uint64_t>(prev) * kRadix + t]);
+					if ((v >> 34) != tag || !(v & (3ull << 32))) {
+						__builtin_amdgcn_s_sleep(1);
+						continue;
+					}
+					excl += static_cast<uint32_t>(v);
+					if (v & kLbInclusive) break;
+					--prev;
+				}
+				atomic_store_agent(&lookback[static_cast<uint64_t>(tile) * kRadix + t], lb_word(excl + pub, kLbInclusive, tag));
+			}
+			s_glob[t] = pbucket[t] + excl - digoff;
+		}
+		__syncthreads();
+		// stage the tile in LDS in digit order
+#pragma unroll
+		for (int j = 0; j < KPT; ++j) {
+			const uint32_t pos = s_digoff[dig[j]] + s_whist[w * kRadix + dig[j]] + rank[j];
+			s_keys[pos] = keys[j];
+		}
+		__syncthreads();
+		for (uint32_t i = t; i < valid; i += THREADS) {
+			const K k = s_keys[i];
+			const uint32_t d = digit_of<K>(k, xf, shift);
+			if constexpr (PAY) s_dig[i] = static_cast<uint8_t>(d);
+			dst[s_glob[d] + i] = k;
+		}
+		if constexpr (PAY) {
+			__syncthreads();
+			uint32_t *s_pay = reinterpret_cast<uint32_t *>(s_raw);
+#pragma unroll
+			for (int j = 0; j < KPT; ++j) {
+				const uint32_t pos = s_digoff[dig[j]] + s_whist[w * kRadix + dig[j]] + rank[j];
+				s_pay[pos] = pay[j];
+			}
+			__syncthreads();
+			for (uint32_t i = t; i < valid; i += THREADS) pp.pdst[s_glob[s_dig[i]] + i] = s_pay[i];
+		}
+		__syncthreads();
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// 8-bit count fill and finish
+
+__global__ __launch_bounds__(256) void k_fill8(const uint32_t *__restrict__ bucket, KeyXform xf,
+	uint8_t *__restrict__ out, uint64_t n)
+{
+	__shared__ uint32_t s_b[kRadix];
+	__shared__ uint8_t s_inv[kRadix];
+	s_b[threadIdx.x] = bucket[threadIdx.x];
+	// the bijective 8-bit transforms are invertible: bucket f(x) holds only the pattern x
+	s_inv[sort_key<uint8_t>(static_cast<uint8_t>(threadIdx.x), xf) & 0xFFu] = static_cast<uint8_t>(threadIdx.x);
+	__syncthreads();
+	const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+	for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+		// last bucket whose start is <= i
+		uint32_t lo = 0;
+#pragma unroll
+		for (uint32_t step = 128; step; step >>= 1)
+			if (s_b[lo + step] <= i) lo += step;
+		out[i] = s_inv[lo];
+	}
+}
+
+__global__ void k_finish(Control *ctrl)
+{
+	if (threadIdx.x < kMaxPasses) ctrl->tile_counter[threadIdx.x] = 0;
+	if (threadIdx.x == 0) ctrl->epoch = ctrl->epoch + 1u;
+}
+
+__global__ void k_zero(uint64_t *p, uint64_t words)
+{
+	const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+	for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += stride) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------------------------
+// host side
+
+// Optional per-launch HIP-event timing on the launch stream (used by the benchmark).
+struct ProfRec {
+	int kind;  // 0 histogram, 1 scatter pass, 2 plan/copy/finish
+	int pass;
+	const void *ws;
+	hipEvent_t a, b;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfRec> g_prof;
+
+struct ProfScope {
+	bool on;
+	ProfRec rec;
+	hipStream_t stream;
+	ProfScope(int kind, int pass, const void *ws, hipStream_t s) : on(false), rec{kind, pass, ws, nullptr, nullptr}, stream(s)
+	{
+		std::lock_guard<std::mutex> lock(g_prof_mu);
+		if (!g_prof_on) return;
+		if (hipEventCreate(&rec.a) != hipSuccess || hipEventCreate(&rec.b) != hipSuccess) return;
+		on = hipEventRecord(rec.a, s) == hipSuccess;
+	}
+	~ProfScope()
+	{
+		if (!on) return;
+		std::lock_guard<std::mutex> lock(g_prof_mu);
+		if (hipEventRecord(rec.b, stream) == hipSuccess) g_prof.push_back(rec);
+	}
+};
+
+struct DevInfo {
+	int cus = 256;
+	int occ[2][4][kNumTileCfgs] = {};  // [payload][key size log2][cfg] resident blocks per CU
+};
+
+DevInfo &devinfo()
+{
+	static DevInfo info;
+	static std::once_flag once;
+	std::call_once(once, [] {
+		int dev = 0;
+		if (hipGetDevice(&dev) == hipSuccess) {
+			hipDeviceProp_t prop;
+			if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) info.cus = prop.multiProcessorCount;
+		}
+	});
+	return info;
+}
+
+template <typename K, bool PAY, int THREADS, int KPT>
+hipError_t launch_pass(Control *ctrl, const uint32_t *bucket, uint64_t *lookback, uint64_t n, const KeyXform &xf,
+	int pass, hipStream_t stream)
+{
+	constexpr int TILE = THREADS * KPT;
+	const uint32_t tiles = static_cast<uint32_t>((n + TILE - 1) / TILE);
+	static int occ = -1;
+	if (occ < 0) {
+		int o = 0;
+		if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_onesweep<K, PAY, THREADS, KPT>, THREADS, 0) != hipSuccess || o < 1) o = 1;
+		occ = o;
+	}
+	const uint32_t grid = std::min<uint32_t>(tiles, static_cast<uint32_t>(occ * devinfo().cus));
+	hipLaunchKernelGGL((k_onesweep<K, PAY, THREADS, KPT>), dim3(grid), dim3(THREADS), 0, stream, ctrl, bucket, lookback, n,
+		tiles, xf, pass);
+	return hipGetLastError();
+}
+
+template <typename K, bool PAY>
+hipError_t launch_pass_cfg(int cfg, Control *ctrl, const uint32_t *bucket, uint64_t *lookback, uint64_t n,
+	const KeyXform &xf, int pass, hipStream_t stream)
+{
+	switch (cfg) {
+	case 0: return launch_pass<K, PAY, 256, 8>(ctrl, bucket, lookback, n, xf, pass, stream);
+	case 1: return launch_pass<K, PAY, 256, 16>(ctrl, bucket, lookback, n, xf, pass, stream);
+	case 2: return launch_pass<K, PAY, 512, 8>(ctrl, bucket, lookback, n, xf, pass, stream);
+	case 3: return launch_pass<K, PAY, 512, 16>(ctrl, bucket, lookback, n, xf, pass, stream);
+	default: return launch_pass<K, PAY, 256, 24>(ctrl, bucket, lookback, n, xf, pass, stream);
+	}
+}
+
+template <typename K>
+hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
+{
+	constexpr int P = sizeof(K);
+	const uint64_t n = job.n;
+	const KeyXform xf = make_xform(8 * sizeof(K), job.flags);
+	Control *ctrl = ctrl_of(ws);
+	uint32_t *hist = hist_of(ws), *bucket = bucket_of(ws);
+	uint64_t *lookback = lookback_of(ws);
+	const int cus = devinfo().cus;
+	hipError_t e;
+
+	const uint32_t hgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 256 * 16 - 1) / (256 * 16), 4u * cus)));
+	{
+		ProfScope ps(0, 0, ws, stream);
+		hipLaunchKernelGGL((k_hist<K, 256>), dim3(hgrid), dim3(256), 0, stream, static_cast<const K *>(job.in), n, xf, hist);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+	}
+
+	PlanArgs pa;
+	pa.n = n; pa.passes = P; pa.src = job.in; pa.fin = job.out; pa.other = job.buf;
+	pa.psrc = job.payload ? job.pin : nullptr;
+	pa.pfin = job.payload ? job.pout : nullptr;
+	pa.pother = job.payload ? job.pbuf : nullptr;
+	pa.payload = job.payload;
+	pa.reverse = (job.flags & kRev) != 0;
+	pa.keys_final_any = job.keys_final_any;
+
+	const bool fill = (P == 1) && !job.payload && job.fill8 &&
+		(job.flags & 7u) != (kAbs | kSign) && (job.flags & 7u) != (kSign | kFltp) && (job.flags & 7u) != (kAbs | kSign | kFltp);
+	hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, stream, pa, ctrl, hist, bucket);
+	if ((e = hipGetLastError()) != hipSuccess) return e;
+
+	const uint32_t cgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 8u * cus)));
+	if (fill) {
+		// bijective 8-bit modes: the sorted output follows from the counts alone
+		hipLaunchKernelGGL(k_fill8, dim3(cgrid), dim3(256), 0, stream, bucket, xf, static_cast<uint8_t *>(job.out), n);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+	} else {
+		hipLaunchKernelGGL((k_copy<K>), dim3(cgrid), dim3(256), 0, stream, ctrl, 0, n, job.payload ? 1 : 0);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+		const int cfg = job.cfg >= 0 ? job.cfg : default_cfg(P, n, job.payload);
+		for (int p = 0; p < P; ++p) {
+			ProfScope ps(1, p, ws, stream);
+			e = job.payload ? launch_pass_cfg<K, true>(cfg, ctrl, bucket, lookback, n, xf, p, stream)
+			                : launch_pass_cfg<K, false>(cfg, ctrl, bucket, lookback, n, xf, p, stream);
+			if (e != hipSuccess) return e;
+		}
+		hipLaunchKernelGGL((k_copy<K>), dim3(cgrid), dim3(256), 0, stream, ctrl, 1, n, job.payload ? 1 : 0);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+	}
+	hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, stream, ctrl);
+	return hipGetLastError();
+}
+
+} // namespace
+
+int device_cu_count() { return devinfo().cus; }
+
+void profile_begin()
+{
+	std::lock_guard<std::mutex> lock(g_prof_mu);
+	for (auto &r : g_prof) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+	g_prof.clear();
+	g_prof_on = true;
+}
+
+// Sums the recorded launch times per kind: 0 histogram, 1 active scatter passes, 2 scatter
+// launches of skipped passes. A pass counts as active by the plan its workspace holds now (the
+// plan of the last sort run on that workspace).
+hipError_t profile_end(double *ms, unsigned *launches, int nkinds)
+{
+	std::vector<ProfRec> recs;
+	{
+		std::lock_guard<std::mutex> lock(g_prof_mu);
+		g_prof_on = false;
+		recs.swap(g_prof);
+	}
+	for (int k = 0; k < nkinds; ++k) { ms[k] = 0; launches[k] = 0; }
+	hipError_t err = hipSuccess;
+	for (auto &r : recs) {
+		hipError_t e = hipEventSynchronize(r.b);
+		float t = 0;
+		if (e == hipSuccess) e = hipEventElapsedTime(&t, r.a, r.b);
+		int kind = r.kind;
+		if (e == hipSuccess && kind == 1) {
+			uint32_t mask = 0;
+			e = hipMemcpy(&mask, &static_cast<const Control *>(r.ws)->active_mask, sizeof(mask), hipMemcpyDeviceToHost);
+			if (!(mask >> r.pass & 1u)) kind = 2;
+		}
+		if (e != hipSuccess) err = e;
+		else if (kind < nkinds) { ms[kind] += t; ++launches[kind]; }
+		(void)hipEventDestroy(r.a);
+		(void)hipEventDestroy(r.b);
+	}
+	return err;
+}
+
+int default_cfg(int key_bytes, uint64_t n, bool payload)
+{
+	(void)payload;
+	if (n <= (1u << 16)) return 0;
+	if (key_bytes >= 8) return 1;
+	return 3;
+}
+
+size_t sort_workspace_bytes(uint64_t n)
+{
+	const uint64_t tiles = (n + kMinTile - 1) / kMinTile + 1;
+	return kLookbackOffset + static_cast<size_t>(tiles) * kRadix * sizeof(uint64_t);
+}
+
+hipError_t workspace_init(void *ws, size_t bytes, hipStream_t stream)
+{
+	const uint64_t words = bytes / 8;
+	const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((words + 255) / 256, 4096)));
+	hipLaunchKernelGGL(k_zero, dim3(grid), dim3(256), 0, stream, static_cast<uint64_t *>(ws), words);
+	return hipGetLastError();
+}
+
+hipError_t sort_launch(const SortJob &job, void *ws, size_t ws_bytes, hipStream_t stream)
+{
+	if (job.n == 0) return hipSuccess;
+	if (job.n > 0xFFFFFFFFull) return hipErrorInvalidValue;
+	if (ws_bytes < sort_workspace_bytes(job.n)) return hipErrorInvalidValue;
+	switch (job.key_bytes) {
+	case 1: return run_sort<uint8_t>(job, ws, stream);
+	case 2: return run_sort<uint16_t>(job, ws, stream);
+	case 4: return run_sort<uint32_t>(job, ws, stream);
+	case 8: return run_sort<uint64_t>(job, ws, stream);
+	default: return hipErrorInvalidValue;
+	}
+}
+
+} // namespace rsbd8hip
