@@ -118,6 +118,28 @@ int rsbd8_hip_sort_pairs(int keytype, unsigned flags, size_t count, void *worksp
 	const void *keys_in, void *keys_out, void *keys_buffer, const uint32_t *payload_in, uint32_t *payload_out,
 	uint32_t *payload_buffer, rsbd8_hip_stream stream);
 
+/* Segmented (batched) sort: nseg independent arrays in one CSR batch, each sorted stably with the
+ * same flags (the reference runs one radixsortcopy per array, H:60468-60527; no counterpart for the
+ * batch itself). A plan is made once from the host copy of the nseg+1 offsets: segments up to the
+ * LDS capacity of small_cfg (0: 1024, 1: 4096, 2: 8192 keys; -1 automatic) are sorted in LDS by one
+ * persistent kernel, longer ones by per-digit onesweep passes over all their tiles with the tile
+ * configuration tile_cfg (-1 automatic). The plan owns its device tables and workspace. run issues
+ * the kernels on a stream; capture records them into a hipGraph kept in the plan, and
+ * launch_graph replays that graph with one launch (on the arrays given to capture). buffer holds
+ * total elements; input, output and buffer must not alias. dev_offsets is the device copy of the
+ * offsets. info receives {short segments, long segments, tiles, tile keys, LDS capacity, total
+ * keys, plan bytes}. */
+typedef struct rsbd8_hip_segplan rsbd8_hip_segplan;
+int rsbd8_hip_segmented_plan_create(int keytype, unsigned flags, size_t nseg, const uint64_t *host_offsets, int tile_cfg,
+	int small_cfg, rsbd8_hip_segplan **plan);
+int rsbd8_hip_segmented_run(rsbd8_hip_segplan *plan, const void *input, void *output, void *buffer,
+	const uint64_t *dev_offsets, rsbd8_hip_stream stream);
+int rsbd8_hip_segmented_capture(rsbd8_hip_segplan *plan, const void *input, void *output, void *buffer,
+	const uint64_t *dev_offsets, rsbd8_hip_stream stream);
+int rsbd8_hip_segmented_launch_graph(rsbd8_hip_segplan *plan, rsbd8_hip_stream stream);
+int rsbd8_hip_segmented_plan_info(const rsbd8_hip_segplan *plan, uint64_t info[7]);
+void rsbd8_hip_segmented_plan_destroy(rsbd8_hip_segplan *plan);
+
 /* Process-wide tile configuration of the scatter passes: 0..4 selects one of
  * {256x8, 256x16, 512x8, 512x16, 256x24} threads x keys per thread, -1 restores the automatic
  * choice. Returns the previous setting. (No reference counterpart: the reference's static

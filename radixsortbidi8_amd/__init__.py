@@ -88,6 +88,12 @@ def _load():
         "rsbd8_hip_sort_pairs": ([ci, cu, sz, vp, sz, vp, vp, vp, vp, vp, vp, vp], ci),
         "rsbd8_hip_set_tile_config": ([ci], ci),
         "rsbd8_hip_last_error": ([], ctypes.c_char_p),
+        "rsbd8_hip_segmented_plan_create": ([ci, cu, sz, vp, ci, ci, ctypes.POINTER(vp)], ci),
+        "rsbd8_hip_segmented_run": ([vp, vp, vp, vp, vp, vp], ci),
+        "rsbd8_hip_segmented_capture": ([vp, vp, vp, vp, vp, vp], ci),
+        "rsbd8_hip_segmented_launch_graph": ([vp, vp], ci),
+        "rsbd8_hip_segmented_plan_info": ([vp, ctypes.POINTER(ctypes.c_uint64)], ci),
+        "rsbd8_hip_segmented_plan_destroy": ([vp], None),
         "rsbd8_hip_profile_begin": ([], ci),
         "rsbd8_hip_profile_end": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(cu), ci], ci),
     }
@@ -317,3 +323,61 @@ def sort_pairs(keys_in, keys_out, keys_buffer, payload_in, payload_out, payload_
     _check(_load().rsbd8_hip_sort_pairs(kt, f, keys_in.numel(), workspace.ptr(), workspace.nbytes, _ptr(keys_in),
                                          _ptr(keys_out), _ptr(keys_buffer), _ptr(payload_in), _ptr(payload_out),
                                          _ptr(payload_buffer), _stream_ptr(stream)))
+
+
+# ---- segmented batches --------------------------------------------------------------------
+
+SMALL_CAPS = (1024, 4096, 8192)
+
+
+class SegmentedPlan:
+    """A batch of independent arrays (CSR offsets) sorted with one set of flags. Short segments are
+    sorted in LDS by one persistent kernel, long ones by batched onesweep passes; ``capture`` records
+    the whole sequence into a hipGraph that ``launch`` replays with one launch."""
+
+    def __init__(self, key_dtype, offsets: np.ndarray, direction=sortingdirection.ascfwdorder,
+                 mode=sortingmode.native, tile_cfg: int = -1, small_cfg: int = -1):
+        torch = _torch()
+        self.keytype = keytype_of(key_dtype)
+        self.flags = decode(self.keytype, mode, direction)
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        self.nseg = off.size - 1
+        self.total = int(off[-1] - off[0]) if self.nseg > 0 else 0
+        self.dev_offsets = torch.from_numpy(off.view(np.int64)).cuda()
+        h = ctypes.c_void_p()
+        _check(_load().rsbd8_hip_segmented_plan_create(self.keytype, self.flags, self.nseg,
+                                                       off.ctypes.data_as(ctypes.c_void_p), int(tile_cfg),
+                                                       int(small_cfg), ctypes.byref(h)))
+        self._h = h
+        self._keep = None
+
+    def info(self):
+        v = (ctypes.c_uint64 * 7)()
+        _check(_load().rsbd8_hip_segmented_plan_info(self._h, v))
+        return dict(zip(("short", "long", "tiles", "tile_keys", "lds_capacity", "total", "plan_bytes"), list(v)))
+
+    def run(self, keys_in, keys_out, buffer, stream=None):
+        _check(_load().rsbd8_hip_segmented_run(self._h, _ptr(keys_in), _ptr(keys_out), _ptr(buffer),
+                                               _ptr(self.dev_offsets), _stream_ptr(stream)))
+
+    def capture(self, keys_in, keys_out, buffer, stream=None):
+        torch = _torch()
+        s = stream if stream is not None else torch.cuda.Stream()
+        self._keep = (keys_in, keys_out, buffer)
+        torch.cuda.synchronize()
+        _check(_load().rsbd8_hip_segmented_capture(self._h, _ptr(keys_in), _ptr(keys_out), _ptr(buffer),
+                                                   _ptr(self.dev_offsets), _stream_ptr(s)))
+
+    def launch(self, stream=None):
+        _check(_load().rsbd8_hip_segmented_launch_graph(self._h, _stream_ptr(stream)))
+
+    def close(self):
+        if self._h:
+            _load().rsbd8_hip_segmented_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

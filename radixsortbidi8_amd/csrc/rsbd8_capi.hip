@@ -5,10 +5,12 @@
 #include <atomic>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 
 #include "../../include/rsbd8_hip.h"
 #include "rsbd8_engine.hpp"
+#include "rsbd8_segmented.hpp"
 
 namespace rsbd8hip {
 
@@ -458,6 +460,66 @@ int rsbd8_hip_radixsort(int keytype, unsigned flags, size_t count, void *input, 
 	if (!input) return fail(RSBD8_HIP_EINVAL, "null array");
 	if (count <= 1) return RSBD8_HIP_OK;  // already sorted (H:60465)
 	return sort_allocating(keytype, flags, count, input, input, true, static_cast<hipStream_t>(stream));
+}
+
+struct rsbd8_hip_segplan {
+	rsbd8hip::SegPlan *impl;
+};
+
+int rsbd8_hip_segmented_plan_create(int keytype, unsigned flags, size_t nseg, const uint64_t *host_offsets, int tile_cfg,
+	int small_cfg, rsbd8_hip_segplan **plan)
+{
+	if (!plan || (!host_offsets && nseg)) return fail(RSBD8_HIP_EINVAL, "null argument");
+	*plan = nullptr;
+	const int kb = key_bytes_of(keytype);
+	if (!kb || !valid_flags(flags)) return fail(RSBD8_HIP_EINVAL, "bad key type or flags");
+	if (tile_cfg < 0) tile_cfg = kb >= 8 ? 1 : 3;
+	if (small_cfg < 0) small_cfg = kb >= 8 ? 1 : 2;
+	rsbd8hip::SegPlan *impl = nullptr;
+	const hipError_t e = segplan_create(kb, flags, nseg, host_offsets, tile_cfg, small_cfg, &impl);
+	if (e != hipSuccess) return fail_hip(e, "segmented_plan_create");
+	*plan = new (std::nothrow) rsbd8_hip_segplan{impl};
+	if (!*plan) { segplan_destroy(impl); return fail(RSBD8_HIP_ENOMEM, "plan allocation"); }
+	return RSBD8_HIP_OK;
+}
+
+int rsbd8_hip_segmented_run(rsbd8_hip_segplan *plan, const void *input, void *output, void *buffer, const uint64_t *dev_offsets,
+	rsbd8_hip_stream stream)
+{
+	if (!plan) return fail(RSBD8_HIP_EINVAL, "null plan");
+	if (input == output || (buffer && (buffer == input || buffer == output))) return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	const hipError_t e = segplan_run(plan->impl, input, output, buffer, dev_offsets, static_cast<hipStream_t>(stream));
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "segmented_run");
+}
+
+int rsbd8_hip_segmented_capture(rsbd8_hip_segplan *plan, const void *input, void *output, void *buffer,
+	const uint64_t *dev_offsets, rsbd8_hip_stream stream)
+{
+	if (!plan) return fail(RSBD8_HIP_EINVAL, "null plan");
+	if (input == output || (buffer && (buffer == input || buffer == output))) return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	const hipError_t e = segplan_capture(plan->impl, input, output, buffer, dev_offsets, static_cast<hipStream_t>(stream));
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "segmented_capture");
+}
+
+int rsbd8_hip_segmented_launch_graph(rsbd8_hip_segplan *plan, rsbd8_hip_stream stream)
+{
+	if (!plan) return fail(RSBD8_HIP_EINVAL, "null plan");
+	const hipError_t e = segplan_launch(plan->impl, static_cast<hipStream_t>(stream));
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "segmented_launch_graph");
+}
+
+int rsbd8_hip_segmented_plan_info(const rsbd8_hip_segplan *plan, uint64_t info[7])
+{
+	if (!plan || !info) return fail(RSBD8_HIP_EINVAL, "null argument");
+	segplan_info(plan->impl, info);
+	return RSBD8_HIP_OK;
+}
+
+void rsbd8_hip_segmented_plan_destroy(rsbd8_hip_segplan *plan)
+{
+	if (!plan) return;
+	segplan_destroy(plan->impl);
+	delete plan;
 }
 
 } // extern "C"

@@ -124,3 +124,54 @@ __device__ __forceinline__ void atomic_store_agent(uint64_t *p, uint64_t v)
 }
 
 } // namespace rsbd8hip
+
+namespace rsbd8hip {
+
+// Block-wide stable rank of THREADS*KPT digits held in registers in logical order
+// l = wave*64*KPT + j*64 + lane (wave-major, then slot j, then lane). Returns each key's
+// position in the digit-sorted block and leaves, for threads t < 256, the digit's block total in
+// *total and its exclusive digit offset in s_digoff[t]. The three __syncthreads() inside must be
+// reached by every thread of the block. s_whist holds (THREADS/64)*256 words, s_wsum 4 words.
+template <int THREADS, int KPT>
+__device__ __forceinline__ void block_rank(const uint32_t (&dig)[KPT], uint32_t (&pos)[KPT], uint32_t *s_whist,
+	uint32_t *s_digoff, uint32_t *s_wsum, uint32_t *total)
+{
+	constexpr int NW = THREADS / 64;
+	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+	for (int i = t; i < NW * 256; i += THREADS) s_whist[i] = 0;
+	__syncthreads();
+	uint32_t rank[KPT];
+#pragma unroll
+	for (int j = 0; j < KPT; ++j) {
+		const uint64_t peers = match_digit(dig[j]);
+		const uint32_t below = lanes_below(peers);
+		uint32_t *slot = &s_whist[w * 256 + dig[j]];
+		const uint32_t old = *slot;
+		if (below == 0) *slot = old + static_cast<uint32_t>(__popcll(peers));
+		rank[j] = old + below;
+	}
+	__syncthreads();
+	uint32_t tot = 0;
+	if (t < 256) {
+		for (int v = 0; v < NW; ++v) {
+			const uint32_t c = s_whist[v * 256 + t];
+			s_whist[v * 256 + t] = tot;
+			tot += c;
+		}
+		const uint32_t incl = wave_inclusive_sum(tot);
+		if (lane == 63) s_wsum[w] = incl;
+		s_digoff[t] = incl - tot;
+	}
+	__syncthreads();
+	if (t < 256) {
+		uint32_t before = 0;
+		for (uint32_t v = 0; v < w; ++v) before += s_wsum[v];
+		s_digoff[t] += before;
+		*total = tot;
+	}
+	__syncthreads();
+#pragma unroll
+	for (int j = 0; j < KPT; ++j) pos[j] = s_digoff[dig[j]] + s_whist[w * 256 + dig[j]] + rank[j];
+}
+
+} // namespace rsbd8hip
