@@ -114,11 +114,17 @@ def main():
 
     k = 8
     scatter_ms, scatter_launches = prof["scatter"]
+    count_ms, count_launches = prof["countscan"]
     passes = round(scatter_launches / args.steps) if args.steps else 0
+    counted = round(count_launches / args.steps) if args.steps else 0
     avg_pass_ms = scatter_ms / max(1, scatter_launches)
     bytes_per_pass = n * 2 * k
     achieved = bytes_per_pass / (avg_pass_ms * 1e-3) / 1e9
-    sort_bytes = n * k + passes * n * 2 * k + (2 * n * k if passes == 0 else 0)
+    # the plan as executed: one histogram read, one count read per pass whose tile counts were not
+    # produced by the histogram read, and a read + write per executed scatter pass
+    count_reads = max(0, counted - 1)
+    sort_bytes = n * k + count_reads * n * k + passes * n * 2 * k + (2 * n * k if passes == 0 else 0)
+    ref_bytes = n * k + passes * n * 2 * k + (2 * n * k if passes == 0 else 0)
     sort_gbs = sort_bytes / (ms_per_step * 1e-3) / 1e9
     traffic = traffic_from_profile(args.n_log2)
     line = {
@@ -137,12 +143,13 @@ def main():
         "config": {
             "workload": "2^28 uniform u64 keys (seeded), stable ascending sort, keys only, device-resident copy in->out",
             "n": n, "key": "u64", "direction": "ascfwdorder", "mode": "native",
-            "plan": f"8-bit digits, {passes} executed onesweep passes",
+            "plan": f"8-bit digits, {passes} executed passes, " + (
+                "count + chained scan + scatter per pass" if counted else "onesweep scatter per pass"),
             "tile_config": r.TILE_CONFIGS[args.tile_config] if args.tile_config >= 0 else "auto",
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_onesweep (one scatter pass)",
+            "kernel": ("k_scatter" if counted else "k_onesweep") + " (one scatter pass: read + write of every key)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -155,7 +162,10 @@ def main():
             "algorithmic_bytes": sort_bytes,
             "achieved_gbs": round(sort_gbs, 1),
             "frac": round(sort_gbs / HBM_PEAK_GBS, 4),
+            "reference_plan_bytes": ref_bytes,
+            "reference_plan_frac": round(ref_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "hist_ms": round(prof["hist"][0] / max(1, prof["hist"][1]), 4),
+            "count_scan_ms_per_pass": round(count_ms / max(1, count_launches), 4),
         },
         "verified": sorted_ok and perm_ok,
         "learned_vs_fixed": None,

@@ -146,11 +146,18 @@ void rsbd8_hip_segmented_plan_destroy(rsbd8_hip_segplan *plan);
  * thread thresholds H:12948-13152 are its closest analogue.) */
 int rsbd8_hip_set_tile_config(int cfg);
 
+/* Process-wide pass algorithm: 1 onesweep (each scatter pass resolves its tile offsets with a
+ * decoupled look-back over the preceding tiles), 2 count + scan + scatter (per-tile digit counts,
+ * a decoupled look-back chained scan of the count matrix, then a scatter with no inter-block
+ * dependency), 0 automatic. Returns the previous setting. */
+int rsbd8_hip_set_algorithm(int algo);
+
 /* Per-launch HIP-event timing of the sorts issued between begin and end, on their own streams
  * (no reference counterpart; the reference's benchmark times whole calls with rdtsc,
  * seqptests/WindowsProject1/WindowsProject1.cpp:912-937). end synchronizes and returns, per kind,
- * the summed milliseconds and launch counts: 0 histogram, 1 executed scatter passes, 2 scatter
- * launches of skipped (uniform-digit) passes. */
+ * the summed milliseconds and launch counts: 0 histogram, 1 scatter launches of executed passes,
+ * 2 scatter launches of skipped (uniform-digit) passes, 3 count + scan launches of executed passes,
+ * 4 count + scan launches of skipped passes. */
 int rsbd8_hip_profile_begin(void);
 int rsbd8_hip_profile_end(double *ms, unsigned *launches, int nkinds);
 

@@ -87,6 +87,7 @@ def _load():
         "rsbd8_hip_sort_indices": ([ci, cu, sz, vp, sz, vp, sz, sz, vp, vp, vp], ci),
         "rsbd8_hip_sort_pairs": ([ci, cu, sz, vp, sz, vp, vp, vp, vp, vp, vp, vp], ci),
         "rsbd8_hip_set_tile_config": ([ci], ci),
+        "rsbd8_hip_set_algorithm": ([ci], ci),
         "rsbd8_hip_last_error": ([], ctypes.c_char_p),
         "rsbd8_hip_segmented_plan_create": ([ci, cu, sz, vp, ci, ci, ctypes.POINTER(vp)], ci),
         "rsbd8_hip_segmented_run": ([vp, vp, vp, vp, vp, vp], ci),
@@ -185,6 +186,17 @@ def set_tile_config(cfg: int) -> int:
     return prev
 
 
+ALGO_AUTO, ALGO_ONESWEEP, ALGO_COUNTSCAN = 0, 1, 2
+
+
+def set_algorithm(algo: int) -> int:
+    """Force the pass algorithm (ALGO_ONESWEEP or ALGO_COUNTSCAN) or ALGO_AUTO; returns the previous."""
+    prev = _load().rsbd8_hip_set_algorithm(int(algo))
+    if prev < 0:
+        _check(prev)
+    return prev
+
+
 def profile_begin():
     """Start recording HIP-event timings of every sort launch (on its own stream)."""
     _check(_load().rsbd8_hip_profile_begin())
@@ -192,11 +204,13 @@ def profile_begin():
 
 def profile_end():
     """Stop recording; returns {kind: (total_ms, launches)} for kinds 'hist', 'scatter'
-    (executed passes) and 'scatter_skipped'."""
-    ms = (ctypes.c_double * 3)()
-    cnt = (ctypes.c_uint * 3)()
-    _check(_load().rsbd8_hip_profile_end(ms, cnt, 3))
-    return {k: (ms[i], cnt[i]) for i, k in enumerate(("hist", "scatter", "scatter_skipped"))}
+    (executed passes), 'scatter_skipped', 'countscan' (count + scan of executed passes) and
+    'countscan_skipped'."""
+    kinds = ("hist", "scatter", "scatter_skipped", "countscan", "countscan_skipped")
+    ms = (ctypes.c_double * len(kinds))()
+    cnt = (ctypes.c_uint * len(kinds))()
+    _check(_load().rsbd8_hip_profile_end(ms, cnt, len(kinds)))
+    return {k: (ms[i], cnt[i]) for i, k in enumerate(kinds)}
 
 
 # ---- workspace (the pzeroedindices equivalent, allocatearray H:59921-60065) -----------------

@@ -129,6 +129,7 @@ namespace {
 
 thread_local std::string g_last_error;
 std::atomic<int> g_tile_cfg{-1};
+std::atomic<int> g_algo{0};
 
 int fail(int code, const char *what)
 {
@@ -251,6 +252,12 @@ int rsbd8_hip_set_tile_config(int cfg)
 	return g_tile_cfg.exchange(cfg);
 }
 
+int rsbd8_hip_set_algorithm(int algo)
+{
+	if (algo < 0 || algo > 2) return fail(RSBD8_HIP_EINVAL, "algorithm out of range");
+	return g_algo.exchange(algo);
+}
+
 const char *rsbd8_hip_last_error(void) { return g_last_error.c_str(); }
 
 int rsbd8_hip_profile_begin(void)
@@ -280,6 +287,7 @@ int rsbd8_hip_radixsortcopynoalloc(int keytype, unsigned flags, size_t count, vo
 	job.key_bytes = kb; job.flags = flags; job.n = count;
 	job.in = input; job.out = output; job.buf = buffer;
 	job.cfg = g_tile_cfg.load();
+	job.algo = g_algo.load();
 	const hipError_t e = sort_launch(job, workspace, workspace_bytes, static_cast<hipStream_t>(stream));
 	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "radixsortcopynoalloc");
 }
@@ -300,6 +308,7 @@ int rsbd8_hip_radixsortnoalloc(int keytype, unsigned flags, size_t count, void *
 	job.out = movetobuffer ? buffer : input;
 	job.buf = movetobuffer ? input : buffer;
 	job.cfg = g_tile_cfg.load();
+	job.algo = g_algo.load();
 	const hipError_t e = sort_launch(job, workspace, workspace_bytes, static_cast<hipStream_t>(stream));
 	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "radixsortnoalloc");
 }
@@ -321,6 +330,7 @@ int rsbd8_hip_sort_pairs(int keytype, unsigned flags, size_t count, void *worksp
 	job.in = keys_in; job.out = keys_out; job.buf = keys_buffer;
 	job.payload = true; job.pin = payload_in; job.pout = payload_out; job.pbuf = payload_buffer;
 	job.cfg = g_tile_cfg.load();
+	job.algo = g_algo.load();
 	const hipError_t e = sort_launch(job, workspace, workspace_bytes, static_cast<hipStream_t>(stream));
 	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "sort_pairs");
 }
@@ -344,6 +354,7 @@ int rsbd8_hip_sort_indices(int keytype, unsigned flags, size_t count, void *work
 	job.keys_final_any = true;
 	job.payload = true; job.pin = idx_in; job.pout = idx_out; job.pbuf = w.perm_a;
 	job.cfg = g_tile_cfg.load();
+	job.algo = g_algo.load();
 	e = sort_launch(job, workspace, workspace_bytes, s);
 	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "sort_indices");
 }
@@ -361,6 +372,7 @@ static int sort_pointers(int keytype, unsigned flags, size_t count, void *worksp
 	job.keys_final_any = true;
 	job.payload = true; job.pin = nullptr; job.pout = w.perm_b; job.pbuf = w.perm_a;
 	job.cfg = g_tile_cfg.load();
+	job.algo = g_algo.load();
 	if ((e = sort_launch(job, workspace, workspace_bytes, s)) != hipSuccess) return fail_hip(e, "sort_pointers");
 	if ((e = permute_pointers(count, input, w.perm_b, output, s)) != hipSuccess) return fail_hip(e, "permute_pointers");
 	return RSBD8_HIP_OK;
@@ -425,6 +437,7 @@ static int sort_allocating(int keytype, unsigned flags, size_t count, const void
 		job.key_bytes = kb; job.flags = flags; job.n = count;
 		job.in = din; job.out = dout; job.buf = buf;
 		job.cfg = g_tile_cfg.load();
+		job.algo = g_algo.load();
 		if ((e = sort_launch(job, ws, wsb, s)) != hipSuccess) { r = fail_hip(e, "sort"); goto out; }
 	}
 	if (!dev_out && (e = hipMemcpyAsync(output, dout, bytes, hipMemcpyDeviceToHost, s)) != hipSuccess) {

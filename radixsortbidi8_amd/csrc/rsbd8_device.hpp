@@ -175,3 +175,38 @@ __device__ __forceinline__ void block_rank(const uint32_t (&dig)[KPT], uint32_t 
 }
 
 } // namespace rsbd8hip
+
+namespace rsbd8hip {
+
+// Decoupled look-back for one digit: sums the published counts of the tiles before `tile` back to
+// the nearest inclusive prefix, reading W predecessors per round so that W memory latencies overlap.
+// `words` points at the look-back word of this digit for tile 0 (stride 256 words per tile). Every
+// tile at or after `first` (the first tile of the sorted range) publishes before it looks back, and
+// tile `first` publishes an inclusive prefix, so the walk always ends.
+template <int W>
+__device__ __forceinline__ uint32_t lookback_window(const uint64_t *words, int64_t tile, int64_t first, uint32_t tag)
+{
+	uint32_t excl = 0;
+	int64_t prev = tile - 1;
+	for (;;) {
+		uint64_t v[W];
+#pragma unroll
+		for (int i = 0; i < W; ++i) v[i] = (prev - i >= first) ? atomic_load_agent(words + (prev - i) * 256) : 0ull;
+		int used = 0;
+		bool done = false;
+#pragma unroll
+		for (int i = 0; i < W; ++i) {
+			if (done || used != i) break;
+			const uint64_t x = v[i];
+			if ((x >> 34) != tag || !(x & (3ull << 32))) break;  // not published yet: read again from here
+			excl += static_cast<uint32_t>(x);
+			++used;
+			if (x & kLbInclusive) done = true;
+		}
+		if (done) return excl;
+		prev -= used;
+		if (used == 0) __builtin_amdgcn_s_sleep(1);
+	}
+}
+
+} // namespace rsbd8hip

@@ -32,10 +32,12 @@ struct PassPlan {
 // Device control block at the start of every workspace.
 struct Control {
 	uint32_t tile_counter[kMaxPasses];
+	uint32_t scan_counter[kMaxPasses];
 	uint32_t epoch;        // advanced by every sort; tags the look-back words of each pass
 	uint32_t num_active;
 	uint32_t active_mask;
-	uint32_t pad[5];
+	uint32_t counts_ready; // bit p: the tile counts of pass p were produced by the histogram read
+	uint32_t pad[4];
 	PassPlan pass[kMaxPasses];
 	PassPlan pre;          // copy before the first pass (in-place sorts that would overlap)
 	PassPlan fin;          // copy when every digit is uniform (no pass executes)
@@ -44,6 +46,7 @@ struct Control {
 constexpr size_t kCtrlBytes = 4096;
 constexpr size_t kHistBytes = kMaxPasses * kRadix * sizeof(uint32_t);
 constexpr size_t kLookbackOffset = kCtrlBytes + 2 * kHistBytes;
+constexpr int kScanChunk = 64;  // tiles per block of the chained scan over the tile counts
 static_assert(sizeof(Control) <= kCtrlBytes, "control block too large");
 
 struct SortJob {
@@ -60,12 +63,16 @@ struct SortJob {
 	int cfg = -1;                  // tile configuration index, -1 picks the default
 	bool fill8 = true;             // 8-bit keys-only bijective modes may use the count fill
 	bool keys_final_any = false;   // only the payload is wanted: keys may end in out or buf
+	int algo = 0;                  // 0 automatic, 1 onesweep (look-back in the scatter), 2 count + chained scan + scatter
 };
+
+constexpr int kAlgoAuto = 0, kAlgoOnesweep = 1, kAlgoCountScan = 2;
 
 size_t sort_workspace_bytes(uint64_t n);
 hipError_t workspace_init(void *ws, size_t bytes, hipStream_t stream);
 hipError_t sort_launch(const SortJob &job, void *ws, size_t ws_bytes, hipStream_t stream);
 int default_cfg(int key_bytes, uint64_t n, bool payload);
+int default_algo(int key_bytes, uint64_t n);
 
 // Gather keys through records or pointers into a dense key array (indirection layouts).
 hipError_t gather_by_index(int key_bytes, uint64_t n, const void *base, uint64_t stride, uint64_t offset,

@@ -238,17 +238,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_pass(const K *__restrict__ in, 
 				atomic_store_agent(&lb[t], lb_word(pub, kLbInclusive, tag));
 			} else {
 				atomic_store_agent(&lb[t], lb_word(pub, kLbAggregate, tag));
-				int64_t prev = static_cast<int64_t>(g) - 1;
-				for (;;) {
-					const uint64_t v = atomic_load_agent(&lookback[static_cast<uint64_t>(prev) * 256 + t]);
-					if ((v >> 34) != tag || !(v & (3ull << 32))) {
-						__builtin_amdgcn_s_sleep(1);
-						continue;
-					}
-					excl += static_cast<uint32_t>(v);
-					if (v & kLbInclusive) break;
-					--prev;
-				}
+				excl = lookback_window<16>(lookback + t, static_cast<int64_t>(g), static_cast<int64_t>(g) - tl.y, tag);
 				atomic_store_agent(&lb[t], lb_word(excl + pub, kLbInclusive, tag));
 			}
 			s_glob[t] = bucket[(static_cast<uint64_t>(tl.x) * P + pass) * 256 + t] + excl - s_digoff[t];

@@ -31,6 +31,11 @@ inline Control *ctrl_of(void *ws) { return reinterpret_cast<Control *>(ws); }
 inline uint32_t *hist_of(void *ws) { return reinterpret_cast<uint32_t *>(static_cast<char *>(ws) + kCtrlBytes); }
 inline uint32_t *bucket_of(void *ws) { return reinterpret_cast<uint32_t *>(static_cast<char *>(ws) + kCtrlBytes + kHistBytes); }
 inline uint64_t *lookback_of(void *ws) { return reinterpret_cast<uint64_t *>(static_cast<char *>(ws) + kLookbackOffset); }
+inline uint64_t max_tiles(uint64_t n) { return (n + kMinTile - 1) / kMinTile + 1; }
+inline uint32_t *counts_of(void *ws, uint64_t n)
+{
+	return reinterpret_cast<uint32_t *>(static_cast<char *>(ws) + kLookbackOffset + max_tiles(n) * kRadix * sizeof(uint64_t));
+}
 
 __device__ __forceinline__ uint32_t pass_tag(uint32_t epoch, int pass)
 {
@@ -40,7 +45,52 @@ __device__ __forceinline__ uint32_t pass_tag(uint32_t epoch, int pass)
 // ---------------------------------------------------------------------------------------------
 // histogram of every digit in one read
 
-template <typename K, int THREADS>
+// Calls f(key) for every key of the physical range [lo, lo + len) of p, with the keys of a block
+// spread over its threads. 16-byte vector loads (four in flight per thread) when the range is
+// 16-byte aligned; the order in which a thread sees its keys is unspecified (counting only).
+template <typename K, int THREADS, typename F>
+__device__ __forceinline__ void for_each_key(const K *__restrict__ p, uint64_t lo, uint32_t len, F &&f)
+{
+	constexpr uint32_t VEC = 16 / sizeof(K);
+	const K *q = p + lo;
+	uint32_t done = 0;
+	if ((reinterpret_cast<uintptr_t>(q) & 15u) == 0) {
+		const uint32_t nvec = len / VEC;
+		const uint4 *qv = reinterpret_cast<const uint4 *>(q);
+		for (uint32_t v = threadIdx.x; v < nvec; v += THREADS * 4) {
+			uint4 u[4];
+#pragma unroll
+			for (int jj = 0; jj < 4; ++jj) {
+				const uint32_t idx = v + jj * THREADS;
+				if (idx < nvec) u[jj] = qv[idx];
+			}
+#pragma unroll
+			for (int jj = 0; jj < 4; ++jj) {
+				if (v + jj * THREADS >= nvec) break;
+				K ks[VEC];
+				__builtin_memcpy(ks, &u[jj], 16);
+#pragma unroll
+				for (uint32_t e = 0; e < VEC; ++e) f(ks[e]);
+			}
+		}
+		done = nvec * VEC;
+	}
+	for (uint32_t i = done + threadIdx.x; i < len; i += THREADS) f(q[i]);
+}
+
+// adds 1 to s[d] for every lane, with one atomic for the whole wave when all lanes share d
+__device__ __forceinline__ void wave_count(uint32_t *s, uint32_t d)
+{
+	const uint64_t active = __ballot(1);
+	const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+	if (__ballot(d == d0) == active) {
+		if (lanes_below(active) == 0) atomicAdd(&s[d0], static_cast<uint32_t>(__popcll(active)));
+	} else {
+		atomicAdd(&s[d], 1u);
+	}
+}
+
+template <typename K, int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_hist(const K *__restrict__ in, uint64_t n, KeyXform xf,
 	uint32_t *__restrict__ hist)
 {
@@ -48,21 +98,16 @@ __global__ __launch_bounds__(THREADS) void k_hist(const K *__restrict__ in, uint
 	__shared__ uint32_t s_h[P * kRadix];
 	for (int i = threadIdx.x; i < P * kRadix; i += THREADS) s_h[i] = 0;
 	__syncthreads();
-	const uint64_t stride = static_cast<uint64_t>(gridDim.x) * THREADS;
-	for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * THREADS + threadIdx.x; i < n; i += stride) {
-		const auto k = sort_key<K>(in[i], xf);
-		const uint64_t active = __ballot(1);
+	// one contiguous, 64-key aligned range per block
+	const uint64_t chunk = ((n + gridDim.x - 1) / gridDim.x + 63) & ~63ull;
+	const uint64_t lo = static_cast<uint64_t>(blockIdx.x) * chunk;
+	if (lo < n) {
+		const uint32_t len = static_cast<uint32_t>((n - lo) < chunk ? n - lo : chunk);
+		for_each_key<K, THREADS>(in, lo, len, [&](K x) {
+			const auto k = sort_key<K>(x, xf);
 #pragma unroll
-		for (int p = 0; p < P; ++p) {
-			const uint32_t d = static_cast<uint32_t>(k >> (8 * p)) & 0xFFu;
-			const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-			if (__ballot(d == d0) == active) {
-				// the whole wave shares this digit: one atomic instead of a 64-way conflict
-				if (lanes_below(active) == 0) atomicAdd(&s_h[p * kRadix + d0], static_cast<uint32_t>(__popcll(active)));
-			} else {
-				atomicAdd(&s_h[p * kRadix + d], 1u);
-			}
-		}
+			for (int p = 0; p < P; ++p) wave_count(s_h + p * kRadix, static_cast<uint32_t>(k >> (8 * p)) & 0xFFu);
+		});
 	}
 	__syncthreads();
 	for (int i = threadIdx.x; i < P * kRadix; i += THREADS)
@@ -84,6 +129,7 @@ struct PlanArgs {
 	int payload;
 	int reverse;
 	int keys_final_any;
+	int counts0;
 };
 
 __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_t *hist, uint32_t *bucket)
@@ -113,6 +159,8 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 		if (!s_uniform[p]) { ++A; mask |= 1u << p; }
 	ctrl->num_active = static_cast<uint32_t>(A);
 	ctrl->active_mask = mask;
+	// the histogram read also counted digit 0 per tile (in the logical order of the first read)
+	ctrl->counts_ready = (a.counts0 && (mask & 1u)) ? 1u : 0u;
 	void *X[2] = {a.fin, a.other};
 	uint32_t *PX[2] = {a.pfin, a.pother};
 	PassPlan none{};
@@ -278,17 +326,7 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(Control *ctrl, const uint3
 			// decoupled look-back over the preceding tiles of this digit
 			uint32_t excl = 0;
 			if (tile > 0) {
-				int64_t prev = static_cast<int64_t>(tile) - 1;
-				for (;;) {
-					const uint64_t v = atomic_load_agent(&lookback[static_cast<uint64_t>(prev) * kRadix + t]);
-					if ((v >> 34) != tag || !(v & (3ull << 32))) {
-						__builtin_amdgcn_s_sleep(1);
-						continue;
-					}
-					excl += static_cast<uint32_t>(v);
-					if (v & kLbInclusive) break;
-					--prev;
-				}
+				excl = lookback_window<16>(lookback + t, tile, 0, tag);
 				atomic_store_agent(&lookback[static_cast<uint64_t>(tile) * kRadix + t], lb_word(excl + pub, kLbInclusive, tag));
 			}
 			s_glob[t] = pbucket[t] + excl - digoff;
@@ -322,6 +360,210 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(Control *ctrl, const uint3
 	}
 }
 
+
+// ---------------------------------------------------------------------------------------------
+// count + chained scan + scatter (no inter-block dependency inside the scatter)
+
+
+// Tile-based read of every key: the histograms of all digits (as k_hist) and, when counts is not
+// null, the per-tile counts of digit 0 in the logical order of the first pass (tile t holds the
+// logical positions [t*tile_keys, (t+1)*tile_keys), read back to front under isrevorder).
+template <typename K, int THREADS, int ITEMS>
+__global__ __launch_bounds__(THREADS) void k_hist_tiles(const K *__restrict__ in, uint64_t n, KeyXform xf,
+	uint32_t *__restrict__ hist, uint32_t *__restrict__ counts, uint32_t tile_keys, uint32_t num_tiles, int reverse)
+{
+	constexpr int P = sizeof(K);
+	__shared__ uint32_t s_h[P * kRadix];
+	__shared__ uint32_t s_t[kRadix];
+	for (int i = threadIdx.x; i < P * kRadix; i += THREADS) s_h[i] = 0;
+	for (uint32_t tile = blockIdx.x; tile < num_tiles; tile += gridDim.x) {
+		for (int i = threadIdx.x; i < kRadix; i += THREADS) s_t[i] = 0;
+		__syncthreads();
+		const uint64_t base = static_cast<uint64_t>(tile) * tile_keys;
+		const uint32_t len = (n - base) < tile_keys ? static_cast<uint32_t>(n - base) : tile_keys;
+		// the tile's keys as a physical range (the logical range read back to front under isrevorder)
+		const uint64_t lo = reverse ? n - base - len : base;
+		for_each_key<K, THREADS>(in, lo, len, [&](K x) {
+			const auto k = sort_key<K>(x, xf);
+#pragma unroll
+			for (int p = 0; p < P; ++p) wave_count(s_h + p * kRadix, static_cast<uint32_t>(k >> (8 * p)) & 0xFFu);
+			if (counts) wave_count(s_t, static_cast<uint32_t>(k) & 0xFFu);
+		});
+		__syncthreads();
+		if (counts)
+			for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kRadix + i] = s_t[i];
+		__syncthreads();
+	}
+	__syncthreads();
+	for (int i = threadIdx.x; i < P * kRadix; i += THREADS)
+		if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+// per-tile counts of one digit (passes after the first, or when the histogram read did not count)
+template <typename K, int THREADS, int ITEMS>
+__global__ __launch_bounds__(THREADS) void k_count(const Control *ctrl, int pass, KeyXform xf, uint64_t n,
+	uint32_t tile_keys, uint32_t num_tiles, uint32_t *__restrict__ counts)
+{
+	const PassPlan pp = ctrl->pass[pass];
+	if (!pp.active || (ctrl->counts_ready >> pass & 1u)) return;
+	const K *__restrict__ src = static_cast<const K *>(pp.src);
+	const uint32_t shift = 8u * static_cast<uint32_t>(pass);
+	__shared__ uint32_t s_t[kRadix];
+	for (uint32_t tile = blockIdx.x; tile < num_tiles; tile += gridDim.x) {
+		for (int i = threadIdx.x; i < kRadix; i += THREADS) s_t[i] = 0;
+		__syncthreads();
+		const uint64_t base = static_cast<uint64_t>(tile) * tile_keys;
+		const uint32_t len = (n - base) < tile_keys ? static_cast<uint32_t>(n - base) : tile_keys;
+		const uint64_t lo = pp.reverse ? n - base - len : base;
+		for_each_key<K, THREADS>(src, lo, len, [&](K x) { wave_count(s_t, digit_of<K>(x, xf, shift)); });
+		__syncthreads();
+		for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kRadix + i] = s_t[i];
+		__syncthreads();
+	}
+}
+
+// Chained scan with decoupled look-back over the tile-count matrix: every block takes the next
+// chunk of kScanChunk tiles, scans it per digit, publishes the chunk total, resolves the prefix
+// of the preceding chunks and turns the counts into global output offsets in place.
+__global__ __launch_bounds__(256) void k_scan(Control *ctrl, int pass, const uint32_t *__restrict__ bucket,
+	uint32_t *__restrict__ counts, uint64_t *__restrict__ lookback, uint32_t num_tiles)
+{
+	const PassPlan pp = ctrl->pass[pass];
+	if (!pp.active) return;
+	const uint32_t nchunks = (num_tiles + kScanChunk - 1) / kScanChunk;
+	const uint32_t tag = pass_tag(ctrl->epoch, pass);
+	const uint32_t d = threadIdx.x;
+	const uint32_t start = bucket[pass * kRadix + d];
+	__shared__ uint32_t s_chunk;
+	for (;;) {
+		if (d == 0) s_chunk = atomicAdd(&ctrl->scan_counter[pass], 1u);
+		__syncthreads();
+		const uint32_t chunk = s_chunk;
+		__syncthreads();
+		if (chunk >= nchunks) break;
+		const uint32_t t0 = chunk * kScanChunk;
+		const uint32_t t1 = min(t0 + kScanChunk, num_tiles);
+		uint32_t run = 0;
+		for (uint32_t t = t0; t < t1; t += 8) {
+			uint32_t c[8];
+#pragma unroll
+			for (int j = 0; j < 8; ++j) c[j] = (t + j < t1) ? counts[static_cast<uint64_t>(t + j) * kRadix + d] : 0u;
+#pragma unroll
+			for (int j = 0; j < 8; ++j) {
+				if (t + j < t1) counts[static_cast<uint64_t>(t + j) * kRadix + d] = run;
+				run += c[j];
+			}
+		}
+		uint32_t excl = 0;
+		uint64_t *lb = lookback + static_cast<uint64_t>(chunk) * kRadix + d;
+		if (chunk == 0) {
+			atomic_store_agent(lb, lb_word(run, kLbInclusive, tag));
+		} else {
+			atomic_store_agent(lb, lb_word(run, kLbAggregate, tag));
+			excl = lookback_window<8>(lookback + d, chunk, 0, tag);
+			atomic_store_agent(lb, lb_word(excl + run, kLbInclusive, tag));
+		}
+		const uint32_t add = start + excl;
+		for (uint32_t t = t0; t < t1; ++t) counts[static_cast<uint64_t>(t) * kRadix + d] += add;
+	}
+}
+
+// Scatter of one digit with known tile offsets. Blocks are dealt to the XCDs round robin, so
+// block b works on the tile range of XCD b % 8: neighbouring tiles, whose digit runs are
+// neighbours in the output, are written through the same L2.
+template <typename K, bool PAY, int THREADS, int KPT>
+__global__ __launch_bounds__(THREADS) void k_scatter(const Control *ctrl, int pass, KeyXform xf, uint64_t n,
+	uint32_t num_tiles, const uint32_t *__restrict__ offsets)
+{
+	constexpr int TILE = THREADS * KPT;
+	constexpr int NW = THREADS / 64;
+	const PassPlan pp = ctrl->pass[pass];
+	if (!pp.active) return;
+	const K *__restrict__ src = static_cast<const K *>(pp.src);
+	K *__restrict__ dst = static_cast<K *>(pp.dst);
+	const uint32_t shift = 8u * static_cast<uint32_t>(pass);
+	__shared__ uint32_t s_whist[NW * kRadix];
+	__shared__ uint32_t s_digoff[kRadix];
+	__shared__ uint32_t s_glob[kRadix];
+	__shared__ uint32_t s_wsum[4];
+	__shared__ uint8_t s_dig[PAY ? TILE : 1];
+	constexpr int kStage = (PAY && sizeof(K) < 4) ? 4 : static_cast<int>(sizeof(K));
+	__shared__ __align__(16) unsigned char s_raw[TILE * kStage];
+	K *s_keys = reinterpret_cast<K *>(s_raw);
+	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+	const uint32_t per = (num_tiles + 7) / 8;
+	const uint32_t x = blockIdx.x & 7u, step = gridDim.x >> 3;
+	K keys[KPT];
+	uint32_t pay[PAY ? KPT : 1];
+	// loads of a tile are issued before the stores of the previous one, so that waiting for them
+	// (the memory counter is in order and counts stores too) never waits for those stores
+	auto load_tile = [&](uint32_t tl) {
+		const uint64_t base = static_cast<uint64_t>(tl) * TILE;
+		const uint32_t valid = (n - base) < static_cast<uint64_t>(TILE) ? static_cast<uint32_t>(n - base) : static_cast<uint32_t>(TILE);
+#pragma unroll
+		for (int j = 0; j < KPT; ++j) {
+			const uint32_t l = w * 64u * KPT + j * 64u + lane;
+			const uint64_t g = base + l;
+			if (l < valid) {
+				const uint64_t sidx = pp.reverse ? n - 1 - g : g;
+				keys[j] = src[sidx];
+				if constexpr (PAY) pay[j] = pp.iota ? static_cast<uint32_t>(sidx) : pp.psrc[sidx];
+			} else {
+				keys[j] = K(0);
+				if constexpr (PAY) pay[j] = 0;
+			}
+		}
+	};
+	uint32_t k = blockIdx.x >> 3;
+	uint32_t tile = x * per + k;
+	if (k < per && tile < num_tiles) load_tile(tile);
+	while (k < per && tile < num_tiles) {
+		const uint64_t base = static_cast<uint64_t>(tile) * TILE;
+		const uint32_t valid = (n - base) < static_cast<uint64_t>(TILE) ? static_cast<uint32_t>(n - base) : static_cast<uint32_t>(TILE);
+		uint32_t dig[KPT], pos[KPT], total;
+#pragma unroll
+		for (int j = 0; j < KPT; ++j) {
+			const uint32_t l = w * 64u * KPT + j * 64u + lane;
+			dig[j] = l < valid ? digit_of<K>(keys[j], xf, shift) : 0xFFu;
+		}
+		block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total);
+		if (t < kRadix) s_glob[t] = offsets[static_cast<uint64_t>(tile) * kRadix + t] - s_digoff[t];
+#pragma unroll
+		for (int j = 0; j < KPT; ++j) s_keys[pos[j]] = keys[j];
+		uint32_t ppos[PAY ? KPT : 1];
+		if constexpr (PAY) {
+#pragma unroll
+			for (int j = 0; j < KPT; ++j) ppos[j] = pos[j];
+		}
+		uint32_t pcur[PAY ? KPT : 1];
+		if constexpr (PAY) {
+#pragma unroll
+			for (int j = 0; j < KPT; ++j) pcur[j] = pay[j];
+		}
+		// prefetch the next tile of this block
+		const uint32_t k2 = k + step, tile2 = x * per + k2;
+		if (k2 < per && tile2 < num_tiles) load_tile(tile2);
+		__syncthreads();
+		for (uint32_t i = t; i < valid; i += THREADS) {
+			const K kk = s_keys[i];
+			const uint32_t dd = digit_of<K>(kk, xf, shift);
+			if constexpr (PAY) s_dig[i] = static_cast<uint8_t>(dd);
+			dst[s_glob[dd] + i] = kk;
+		}
+		if constexpr (PAY) {
+			__syncthreads();
+			uint32_t *s_pay = reinterpret_cast<uint32_t *>(s_raw);
+#pragma unroll
+			for (int j = 0; j < KPT; ++j) s_pay[ppos[j]] = pcur[j];
+			__syncthreads();
+			for (uint32_t i = t; i < valid; i += THREADS) pp.pdst[s_glob[s_dig[i]] + i] = s_pay[i];
+		}
+		__syncthreads();
+		k = k2;
+		tile = tile2;
+	}
+}
+
 // ---------------------------------------------------------------------------------------------
 // 8-bit count fill and finish
 
@@ -347,7 +589,10 @@ __global__ __launch_bounds__(256) void k_fill8(const uint32_t *__restrict__ buck
 
 __global__ void k_finish(Control *ctrl)
 {
-	if (threadIdx.x < kMaxPasses) ctrl->tile_counter[threadIdx.x] = 0;
+	if (threadIdx.x < kMaxPasses) {
+		ctrl->tile_counter[threadIdx.x] = 0;
+		ctrl->scan_counter[threadIdx.x] = 0;
+	}
 	if (threadIdx.x == 0) ctrl->epoch = ctrl->epoch + 1u;
 }
 
@@ -362,7 +607,7 @@ __global__ void k_zero(uint64_t *p, uint64_t words)
 
 // Optional per-launch HIP-event timing on the launch stream (used by the benchmark).
 struct ProfRec {
-	int kind;  // 0 histogram, 1 scatter pass, 2 plan/copy/finish
+	int kind;  // 0 histogram, 1 scatter pass, 3 count + scan of a pass
 	int pass;
 	const void *ws;
 	hipEvent_t a, b;
@@ -440,6 +685,36 @@ hipError_t launch_pass_cfg(int cfg, Control *ctrl, const uint32_t *bucket, uint6
 	}
 }
 
+template <typename K, bool PAY, int THREADS, int KPT>
+hipError_t launch_scatter(Control *ctrl, int pass, const KeyXform &xf, uint64_t n, uint32_t tiles, const uint32_t *offsets,
+	hipStream_t stream)
+{
+	static int occ = -1;
+	if (occ < 0) {
+		int o = 0;
+		if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_scatter<K, PAY, THREADS, KPT>, THREADS, 0) != hipSuccess || o < 1) o = 1;
+		occ = o;
+	}
+	const uint32_t per = (tiles + 7) / 8;
+	const uint32_t want = static_cast<uint32_t>(occ * devinfo().cus * 2 + 7) / 8;
+	const uint32_t grid = 8u * std::max<uint32_t>(1u, std::min<uint32_t>(per, want));
+	hipLaunchKernelGGL((k_scatter<K, PAY, THREADS, KPT>), dim3(grid), dim3(THREADS), 0, stream, ctrl, pass, xf, n, tiles, offsets);
+	return hipGetLastError();
+}
+
+template <typename K, bool PAY>
+hipError_t launch_scatter_cfg(int cfg, Control *ctrl, int pass, const KeyXform &xf, uint64_t n, uint32_t tiles,
+	const uint32_t *offsets, hipStream_t stream)
+{
+	switch (cfg) {
+	case 0: return launch_scatter<K, PAY, 256, 8>(ctrl, pass, xf, n, tiles, offsets, stream);
+	case 1: return launch_scatter<K, PAY, 256, 16>(ctrl, pass, xf, n, tiles, offsets, stream);
+	case 2: return launch_scatter<K, PAY, 512, 8>(ctrl, pass, xf, n, tiles, offsets, stream);
+	case 3: return launch_scatter<K, PAY, 512, 16>(ctrl, pass, xf, n, tiles, offsets, stream);
+	default: return launch_scatter<K, PAY, 256, 24>(ctrl, pass, xf, n, tiles, offsets, stream);
+	}
+}
+
 template <typename K>
 hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 {
@@ -452,10 +727,25 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	const int cus = devinfo().cus;
 	hipError_t e;
 
-	const uint32_t hgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 256 * 16 - 1) / (256 * 16), 4u * cus)));
+	constexpr int HITEMS = 16 / sizeof(K) < 4 ? 4 : 16 / sizeof(K) * 2;
+	const int cfg = job.cfg >= 0 ? job.cfg : default_cfg(P, n, job.payload);
+	const int algo = job.algo ? job.algo : default_algo(P, n);
+	const uint32_t tile_keys = static_cast<uint32_t>(kTileCfgs[cfg].threads * kTileCfgs[cfg].kpt);
+	const uint32_t tiles = static_cast<uint32_t>((n + tile_keys - 1) / tile_keys);
+	uint32_t *counts = counts_of(ws, n);
+	const bool fill = (P == 1) && !job.payload && job.fill8 &&
+		(job.flags & 7u) != (kAbs | kSign) && (job.flags & 7u) != (kSign | kFltp) && (job.flags & 7u) != (kAbs | kSign | kFltp);
+	const bool counts0 = algo == kAlgoCountScan && !fill;
 	{
 		ProfScope ps(0, 0, ws, stream);
-		hipLaunchKernelGGL((k_hist<K, 256>), dim3(hgrid), dim3(256), 0, stream, static_cast<const K *>(job.in), n, xf, hist);
+		if (counts0) {
+			const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, 8u * cus));
+			hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS>), dim3(grid), dim3(256), 0, stream, static_cast<const K *>(job.in), n,
+				xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0);
+		} else {
+			const uint32_t hgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 256 * HITEMS - 1) / (256 * HITEMS), 8u * cus)));
+			hipLaunchKernelGGL((k_hist<K, 256, HITEMS>), dim3(hgrid), dim3(256), 0, stream, static_cast<const K *>(job.in), n, xf, hist);
+		}
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 	}
 
@@ -467,9 +757,8 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	pa.payload = job.payload;
 	pa.reverse = (job.flags & kRev) != 0;
 	pa.keys_final_any = job.keys_final_any;
+	pa.counts0 = counts0 ? 1 : 0;
 
-	const bool fill = (P == 1) && !job.payload && job.fill8 &&
-		(job.flags & 7u) != (kAbs | kSign) && (job.flags & 7u) != (kSign | kFltp) && (job.flags & 7u) != (kAbs | kSign | kFltp);
 	hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, stream, pa, ctrl, hist, bucket);
 	if ((e = hipGetLastError()) != hipSuccess) return e;
 
@@ -481,11 +770,27 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	} else {
 		hipLaunchKernelGGL((k_copy<K>), dim3(cgrid), dim3(256), 0, stream, ctrl, 0, n, job.payload ? 1 : 0);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
-		const int cfg = job.cfg >= 0 ? job.cfg : default_cfg(P, n, job.payload);
 		for (int p = 0; p < P; ++p) {
+			if (algo == kAlgoOnesweep) {
+				ProfScope ps(1, p, ws, stream);
+				e = job.payload ? launch_pass_cfg<K, true>(cfg, ctrl, bucket, lookback, n, xf, p, stream)
+				                : launch_pass_cfg<K, false>(cfg, ctrl, bucket, lookback, n, xf, p, stream);
+				if (e != hipSuccess) return e;
+				continue;
+			}
+			{
+				ProfScope ps(3, p, ws, stream);
+				const uint32_t cgrid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, 8u * cus));
+				hipLaunchKernelGGL((k_count<K, 256, HITEMS>), dim3(cgrid), dim3(256), 0, stream, ctrl, p, xf, n, tile_keys, tiles, counts);
+				if ((e = hipGetLastError()) != hipSuccess) return e;
+				const uint32_t nchunks = (tiles + kScanChunk - 1) / kScanChunk;
+				const uint32_t sgrid = std::max<uint32_t>(1u, std::min<uint32_t>(nchunks, 4u * cus));
+				hipLaunchKernelGGL(k_scan, dim3(sgrid), dim3(256), 0, stream, ctrl, p, bucket, counts, lookback, tiles);
+				if ((e = hipGetLastError()) != hipSuccess) return e;
+			}
 			ProfScope ps(1, p, ws, stream);
-			e = job.payload ? launch_pass_cfg<K, true>(cfg, ctrl, bucket, lookback, n, xf, p, stream)
-			                : launch_pass_cfg<K, false>(cfg, ctrl, bucket, lookback, n, xf, p, stream);
+			e = job.payload ? launch_scatter_cfg<K, true>(cfg, ctrl, p, xf, n, tiles, counts, stream)
+			                : launch_scatter_cfg<K, false>(cfg, ctrl, p, xf, n, tiles, counts, stream);
 			if (e != hipSuccess) return e;
 		}
 		hipLaunchKernelGGL((k_copy<K>), dim3(cgrid), dim3(256), 0, stream, ctrl, 1, n, job.payload ? 1 : 0);
@@ -507,7 +812,8 @@ void profile_begin()
 	g_prof_on = true;
 }
 
-// Sums the recorded launch times per kind: 0 histogram, 1 active scatter passes, 2 scatter
+// Sums the recorded launch times per kind: 0 histogram, 1 scatter launches of executed passes,
+// 2 scatter launches of skipped passes, 3 count+scan launches of executed passes, 4 count+scan
 // launches of skipped passes. A pass counts as active by the plan its workspace holds now (the
 // plan of the last sort run on that workspace).
 hipError_t profile_end(double *ms, unsigned *launches, int nkinds)
@@ -525,10 +831,10 @@ hipError_t profile_end(double *ms, unsigned *launches, int nkinds)
 		float t = 0;
 		if (e == hipSuccess) e = hipEventElapsedTime(&t, r.a, r.b);
 		int kind = r.kind;
-		if (e == hipSuccess && kind == 1) {
+		if (e == hipSuccess && (kind == 1 || kind == 3)) {
 			uint32_t mask = 0;
 			e = hipMemcpy(&mask, &static_cast<const Control *>(r.ws)->active_mask, sizeof(mask), hipMemcpyDeviceToHost);
-			if (!(mask >> r.pass & 1u)) kind = 2;
+			if (!(mask >> r.pass & 1u)) kind += 1;
 		}
 		if (e != hipSuccess) err = e;
 		else if (kind < nkinds) { ms[kind] += t; ++launches[kind]; }
@@ -538,18 +844,26 @@ hipError_t profile_end(double *ms, unsigned *launches, int nkinds)
 	return err;
 }
 
+// measured on MI355X (2^20..2^28 uniform keys): the count + scan path wins from about 2^22 keys,
+// the onesweep path below that (fewer launches)
+int default_algo(int key_bytes, uint64_t n)
+{
+	(void)key_bytes;
+	return n >= (1u << 22) ? kAlgoCountScan : kAlgoOnesweep;
+}
+
 int default_cfg(int key_bytes, uint64_t n, bool payload)
 {
 	(void)payload;
 	if (n <= (1u << 16)) return 0;
-	if (key_bytes >= 8) return 1;
-	return 3;
+	if (default_algo(key_bytes, n) == kAlgoCountScan) return key_bytes >= 8 ? 4 : 1;
+	return key_bytes >= 8 ? 4 : 2;
 }
 
 size_t sort_workspace_bytes(uint64_t n)
 {
-	const uint64_t tiles = (n + kMinTile - 1) / kMinTile + 1;
-	return kLookbackOffset + static_cast<size_t>(tiles) * kRadix * sizeof(uint64_t);
+	// look-back words (one per tile and digit) followed by the tile-count matrix
+	return kLookbackOffset + static_cast<size_t>(max_tiles(n)) * kRadix * (sizeof(uint64_t) + sizeof(uint32_t));
 }
 
 hipError_t workspace_init(void *ws, size_t bytes, hipStream_t stream)

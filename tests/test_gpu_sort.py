@@ -91,6 +91,14 @@ def combos(nbytes):
     return out + [(KEYTYPES[nbytes][0], m) for m in range(8)]
 
 
+@pytest.fixture(params=[1, 2], ids=["onesweep", "countscan"])
+def algo(request):
+    """Runs a test under each pass algorithm (onesweep look-back, count + chained scan + scatter)."""
+    prev = r.set_algorithm(request.param)
+    yield request.param
+    r.set_algorithm(prev)
+
+
 # ---- known answers ---------------------------------------------------------------------------
 
 @pytest.mark.parametrize("case", KAT["cases"], ids=lambda c: c["name"])
@@ -129,7 +137,7 @@ def test_known_answers_on_gpu(case):
 
 @pytest.mark.parametrize("nbytes", [1, 2, 4, 8])
 @pytest.mark.parametrize("kind", ["uniform", "dups", "specials", "lowbyte", "twobytes", "equal", "sorted"])
-def test_copy_all_modes_directions(nbytes, kind):
+def test_copy_all_modes_directions(nbytes, kind, algo):
     rng = np.random.default_rng(nbytes * 131 + len(kind))
     for n in (2, 3, 1000, 70001):
         bits = gen(nbytes, n, kind, rng)
@@ -151,7 +159,7 @@ def test_copy_all_modes_directions(nbytes, kind):
 
 @pytest.mark.parametrize("nbytes", [1, 2, 4, 8])
 @pytest.mark.parametrize("kind", ["uniform", "lowbyte", "twobytes", "equal"])
-def test_inplace_movetobuffer(nbytes, kind):
+def test_inplace_movetobuffer(nbytes, kind, algo):
     """radixsortnoalloc in both output placements; odd and even active-pass counts exercise the
     staging copy the in-place plan needs."""
     rng = np.random.default_rng(7 * nbytes + len(kind))
@@ -191,7 +199,7 @@ def test_allocating_forms_host_and_device():
 
 
 @pytest.mark.parametrize("cfg", [0, 1, 2, 3, 4])
-def test_every_tile_config(cfg):
+def test_every_tile_config(cfg, algo):
     rng = np.random.default_rng(cfg)
     prev = r.set_tile_config(cfg)
     try:
@@ -212,7 +220,7 @@ def test_every_tile_config(cfg):
 
 
 @pytest.mark.parametrize("nbytes", [1, 2, 4, 8])
-def test_pairs_payload(nbytes):
+def test_pairs_payload(nbytes, algo):
     rng = np.random.default_rng(40 + nbytes)
     n = 99991
     for kind in ("dups", "specials", "equal"):
@@ -236,7 +244,7 @@ def test_pairs_payload(nbytes):
                 assert np.array_equal(host(po), perm)
 
 
-def test_indices_records():
+def test_indices_records(algo):
     """Records addressed by indices (the indirect config's shape): 32-byte records, key at +8."""
     rng = np.random.default_rng(21)
     n, stride = 200003, 32
@@ -263,7 +271,7 @@ def test_indices_records():
     assert np.array_equal(host(o2), exp)
 
 
-def test_pointer_indirection_levels():
+def test_pointer_indirection_levels(algo):
     """First-level (type+offset) and second-level (pointer member plus index) indirection, copy and
     in-place forms."""
     rng = np.random.default_rng(99)
@@ -308,7 +316,7 @@ def test_zero_and_one_element():
     assert r.radixsortcopy(a, o)
 
 
-def test_workspace_reuse_many_calls():
+def test_workspace_reuse_many_calls(algo):
     """The look-back epoch advances per sort: many sorts of different sizes on one workspace."""
     rng = np.random.default_rng(5)
     ws = r.Workspace(r.U64, 1 << 18)
