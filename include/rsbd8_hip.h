@@ -152,6 +152,57 @@ int rsbd8_hip_set_tile_config(int cfg);
  * dependency), 0 automatic. Returns the previous setting. */
 int rsbd8_hip_set_algorithm(int algo);
 
+/* A sort plan chosen per call (no reference counterpart; the reference picks its plan from
+ * compile-time thresholds, H:12948-13152): algo 0 automatic, 1 onesweep, 2 count + scan + scatter,
+ * 3 a single block sorting everything in LDS (keys only, count <= 8192; otherwise automatic);
+ * tile_cfg -1 automatic or 0..4 as for rsbd8_hip_set_tile_config; presort_check first tests
+ * whether the input is already in order or strictly reversed (then the sort is one copy). */
+typedef struct rsbd8_hip_plan {
+	int algo;
+	int tile_cfg;
+	int presort_check;
+	int reserved;
+} rsbd8_hip_plan;
+
+/* rsbd8_hip_radixsortcopynoalloc with an explicit plan. */
+int rsbd8_hip_radixsortcopynoalloc_plan(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
+	const void *input, void *output, void *buffer, const rsbd8_hip_plan *plan, rsbd8_hip_stream stream);
+
+/* The candidate plans, indexed in the order of the selector network's outputs (0 <= index <
+ * rsbd8_hip_plan_count()): onesweep then count + scan, each over the tile configurations, each
+ * without and with the presort check; the last is the single-block LDS sort (count <= 8192). */
+int rsbd8_hip_plan_count(void);
+int rsbd8_hip_plan_at(int index, rsbd8_hip_plan *plan);
+
+/* Learned plan selection (no reference counterpart; the reference's analogue is its compile-time
+ * thresholds, Radixsortbidi8.hpp:12948-13152, and count-based branches, H:56797-57052). One block
+ * samples the keys, evaluates the selector network (params: 24 -> hidden -> hidden ->
+ * rsbd8_hip_plan_count() outputs, hidden <= 64, as trained by the mlp_* calls) and picks the plan with
+ * the lowest predicted time among those valid for count. select_plan blocks until the choice is
+ * known (it synchronizes the stream; the first call on a thread allocates one page of pinned host
+ * memory, kept until the process ends). radixsortcopynoalloc_learned then sorts with the chosen plan
+ * exactly as rsbd8_hip_radixsortcopynoalloc_plan does; plan_index (may be NULL) receives the choice. */
+int rsbd8_hip_select_plan(int keytype, unsigned flags, size_t count, const void *input, const float *params, int hidden,
+	int *plan_index, rsbd8_hip_stream stream);
+int rsbd8_hip_radixsortcopynoalloc_learned(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
+	const void *input, void *output, void *buffer, const float *params, int hidden, int *plan_index, rsbd8_hip_stream stream);
+
+/* The plan selector's inputs and network (no reference counterpart). rsbd8_hip_features writes 24
+ * floats describing a key array from a sample of its keys (size, width, flags, per-digit byte
+ * entropy, neighbour order statistics); the mlp_* calls run a 3-layer ReLU network
+ * features -> hidden -> hidden -> outputs (hidden <= 64) on device arrays: forward (acts, 2*hidden
+ * floats per sample, may be NULL), backward of the masked mean squared error (per_sample holds
+ * batch * param_count floats, grad param_count floats) and an Adam step. */
+int rsbd8_hip_features(int keytype, unsigned flags, size_t count, const void *input, float *features, rsbd8_hip_stream stream);
+int rsbd8_hip_mlp_param_count(int features, int hidden, int outputs);
+int rsbd8_hip_mlp_forward(int features, int hidden, int outputs, const float *params, const float *x, int batch,
+	float *acts, float *y, rsbd8_hip_stream stream);
+int rsbd8_hip_mlp_backward(int features, int hidden, int outputs, const float *params, const float *x, int batch,
+	const float *acts, const float *y, const float *target, const float *mask, float mask_sum, float *per_sample,
+	float *grad, rsbd8_hip_stream stream);
+int rsbd8_hip_mlp_adam(int count, float *params, const float *grad, float *m, float *v, float lr, float beta1, float beta2,
+	float eps, int step, float decay, rsbd8_hip_stream stream);
+
 /* Per-launch HIP-event timing of the sorts issued between begin and end, on their own streams
  * (no reference counterpart; the reference's benchmark times whole calls with rdtsc,
  * seqptests/WindowsProject1/WindowsProject1.cpp:912-937). end synchronizes and returns, per kind,

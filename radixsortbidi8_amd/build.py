@@ -9,7 +9,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "librsbd8_hip.so")
-SOURCES = ["rsbd8_sort.hip", "rsbd8_segmented.hip", "rsbd8_capi.hip"]
+SOURCES = ["rsbd8_sort.hip", "rsbd8_segmented.hip", "rsbd8_plan.hip", "rsbd8_capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -11,6 +11,7 @@
 #include "../../include/rsbd8_hip.h"
 #include "rsbd8_engine.hpp"
 #include "rsbd8_segmented.hpp"
+#include "rsbd8_plan.hpp"
 
 namespace rsbd8hip {
 
@@ -533,6 +534,155 @@ void rsbd8_hip_segmented_plan_destroy(rsbd8_hip_segplan *plan)
 	if (!plan) return;
 	segplan_destroy(plan->impl);
 	delete plan;
+}
+
+namespace {
+
+// The candidate plans in the order of the selector network's outputs: (onesweep, count + scan) x
+// tile configuration x (no presort check, presort check), then the single-block LDS sort.
+constexpr int kNumPlans = 2 * kNumTileCfgs * 2 + 1;
+
+rsbd8_hip_plan plan_at(int i)
+{
+	if (i == kNumPlans - 1) return rsbd8_hip_plan{kAlgoLds, -1, 0, 0};
+	return rsbd8_hip_plan{1 + i / (2 * kNumTileCfgs), (i / 2) % kNumTileCfgs, i & 1, 0};
+}
+
+// plans that can run on count keys, as a bit mask over plan indices
+uint32_t valid_plans(size_t count)
+{
+	uint32_t m = (1u << (kNumPlans - 1)) - 1u;
+	if (count <= kLdsMax) m |= 1u << (kNumPlans - 1);
+	return m;
+}
+
+int sort_with_plan(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes, const void *input,
+	void *output, void *buffer, const rsbd8_hip_plan &plan, hipStream_t stream, const char *what)
+{
+	int r = check_common(keytype, flags, count, workspace_bytes, RSBD8_HIP_LAYOUT_KEYS);
+	if (r) return r;
+	const int kb = key_bytes_of(keytype);
+	if (!count) return RSBD8_HIP_OK;
+	if (!input || !output || !workspace) return fail(RSBD8_HIP_EINVAL, "null argument");
+	if (input == output || (buffer && (buffer == input || buffer == output))) return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	if (!buffer && needs_buffer(kb, flags) && kb != 1) return fail(RSBD8_HIP_EINVAL, "buffer required");
+	if (plan.algo < 0 || plan.algo > 3 || plan.tile_cfg < -1 || plan.tile_cfg >= kNumTileCfgs)
+		return fail(RSBD8_HIP_EINVAL, "plan out of range");
+	SortJob job;
+	job.key_bytes = kb; job.flags = flags; job.n = count;
+	job.in = input; job.out = output; job.buf = buffer;
+	job.cfg = plan.tile_cfg;
+	job.algo = plan.algo;
+	job.presort_check = plan.presort_check != 0;
+	const hipError_t e = sort_launch(job, workspace, workspace_bytes, stream);
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, what);
+}
+
+// One page of pinned host memory per calling thread, allocated on its first selection and kept
+// for the life of the process: the selection kernel writes its answer there.
+int *selection_slot()
+{
+	thread_local int *slot = nullptr;
+	if (!slot) {
+		void *p = nullptr;
+		if (hipHostMalloc(&p, sizeof(int), hipHostMallocDefault) != hipSuccess) return nullptr;
+		slot = static_cast<int *>(p);
+	}
+	return slot;
+}
+
+} // namespace
+
+int rsbd8_hip_radixsortcopynoalloc_plan(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
+	const void *input, void *output, void *buffer, const rsbd8_hip_plan *plan, rsbd8_hip_stream stream)
+{
+	if (!plan) return fail(RSBD8_HIP_EINVAL, "null plan");
+	return sort_with_plan(keytype, flags, count, workspace, workspace_bytes, input, output, buffer, *plan,
+		static_cast<hipStream_t>(stream), "radixsortcopynoalloc_plan");
+}
+
+int rsbd8_hip_plan_count(void) { return kNumPlans; }
+
+int rsbd8_hip_plan_at(int index, rsbd8_hip_plan *plan)
+{
+	if (index < 0 || index >= kNumPlans || !plan) return fail(RSBD8_HIP_EINVAL, "plan index out of range");
+	*plan = plan_at(index);
+	return RSBD8_HIP_OK;
+}
+
+int rsbd8_hip_select_plan(int keytype, unsigned flags, size_t count, const void *input, const float *params, int hidden,
+	int *plan_index, rsbd8_hip_stream stream)
+{
+	const int kb = key_bytes_of(keytype);
+	if (!kb || !valid_flags(flags) || !params || !plan_index || (count && !input) || hidden <= 0 || hidden > 64)
+		return fail(RSBD8_HIP_EINVAL, "bad argument");
+	int *slot = selection_slot();
+	if (!slot) return fail(RSBD8_HIP_ENOMEM, "pinned selection slot");
+	*slot = -1;
+	const hipStream_t s = static_cast<hipStream_t>(stream);
+	hipError_t e = select_launch(kb, flags, count, input, hidden, kNumPlans, params, valid_plans(count), slot, s);
+	if (e == hipSuccess) e = hipStreamSynchronize(s);
+	if (e != hipSuccess) return fail_hip(e, "select_plan");
+	const int idx = __atomic_load_n(slot, __ATOMIC_ACQUIRE);
+	if (idx < 0 || idx >= kNumPlans) return fail(RSBD8_HIP_EINVAL, "selection produced no plan");
+	*plan_index = idx;
+	return RSBD8_HIP_OK;
+}
+
+int rsbd8_hip_radixsortcopynoalloc_learned(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
+	const void *input, void *output, void *buffer, const float *params, int hidden, int *plan_index, rsbd8_hip_stream stream)
+{
+	int idx = 0;
+	if (count > 1) {
+		const int r = rsbd8_hip_select_plan(keytype, flags, count, input, params, hidden, &idx, stream);
+		if (r) return r;
+	}
+	if (plan_index) *plan_index = idx;
+	return sort_with_plan(keytype, flags, count, workspace, workspace_bytes, input, output, buffer, plan_at(idx),
+		static_cast<hipStream_t>(stream), "radixsortcopynoalloc_learned");
+}
+
+int rsbd8_hip_features(int keytype, unsigned flags, size_t count, const void *input, float *features, rsbd8_hip_stream stream)
+{
+	const int kb = key_bytes_of(keytype);
+	if (!kb || !valid_flags(flags) || !features || (count && !input)) return fail(RSBD8_HIP_EINVAL, "bad argument");
+	const hipError_t e = features_launch(kb, flags, count, input, features, static_cast<hipStream_t>(stream));
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "features");
+}
+
+int rsbd8_hip_mlp_param_count(int features, int hidden, int outputs)
+{
+	if (features <= 0 || hidden <= 0 || outputs <= 0) return fail(RSBD8_HIP_EINVAL, "bad dimensions");
+	return mlp_param_count(features, hidden, outputs);
+}
+
+int rsbd8_hip_mlp_forward(int features, int hidden, int outputs, const float *params, const float *x, int batch,
+	float *acts, float *y, rsbd8_hip_stream stream)
+{
+	if (features <= 0 || hidden <= 0 || hidden > 64 || outputs <= 0 || !params || !x || !y || batch < 0)
+		return fail(RSBD8_HIP_EINVAL, "bad argument");
+	const hipError_t e = mlp_forward(features, hidden, outputs, params, x, batch, acts, y, static_cast<hipStream_t>(stream));
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "mlp_forward");
+}
+
+int rsbd8_hip_mlp_backward(int features, int hidden, int outputs, const float *params, const float *x, int batch,
+	const float *acts, const float *y, const float *target, const float *mask, float mask_sum, float *per_sample,
+	float *grad, rsbd8_hip_stream stream)
+{
+	if (features <= 0 || hidden <= 0 || hidden > 64 || outputs <= 0 || !params || !x || !acts || !y || !target || !mask ||
+		!per_sample || !grad || batch < 0)
+		return fail(RSBD8_HIP_EINVAL, "bad argument");
+	const hipError_t e = mlp_backward(features, hidden, outputs, params, x, batch, acts, y, target, mask, mask_sum, per_sample,
+		grad, static_cast<hipStream_t>(stream));
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "mlp_backward");
+}
+
+int rsbd8_hip_mlp_adam(int count, float *params, const float *grad, float *m, float *v, float lr, float beta1, float beta2,
+	float eps, int step, float decay, rsbd8_hip_stream stream)
+{
+	if (count <= 0 || !params || !grad || !m || !v || step < 1) return fail(RSBD8_HIP_EINVAL, "bad argument");
+	const hipError_t e = mlp_adam(count, params, grad, m, v, lr, beta1, beta2, eps, step, decay, static_cast<hipStream_t>(stream));
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "mlp_adam");
 }
 
 } // extern "C"

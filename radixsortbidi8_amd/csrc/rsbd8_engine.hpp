@@ -37,7 +37,8 @@ struct Control {
 	uint32_t num_active;
 	uint32_t active_mask;
 	uint32_t counts_ready; // bit p: the tile counts of pass p were produced by the histogram read
-	uint32_t pad[4];
+	uint32_t check;        // presort check: bit 0 ran, bit 1 some k[j] < k[j+1], bit 2 equal, bit 3 greater
+	uint32_t pad[3];
 	PassPlan pass[kMaxPasses];
 	PassPlan pre;          // copy before the first pass (in-place sorts that would overlap)
 	PassPlan fin;          // copy when every digit is uniform (no pass executes)
@@ -63,10 +64,13 @@ struct SortJob {
 	int cfg = -1;                  // tile configuration index, -1 picks the default
 	bool fill8 = true;             // 8-bit keys-only bijective modes may use the count fill
 	bool keys_final_any = false;   // only the payload is wanted: keys may end in out or buf
-	int algo = 0;                  // 0 automatic, 1 onesweep (look-back in the scatter), 2 count + chained scan + scatter
+	int algo = 0;                  // 0 automatic, 1 onesweep (look-back in the scatter), 2 count + chained scan + scatter,
+	                               // 3 one block sorts everything in LDS (keys only, n <= kLdsMax)
+	bool presort_check = false;    // first test whether the input is already in order (or strictly reversed)
 };
 
-constexpr int kAlgoAuto = 0, kAlgoOnesweep = 1, kAlgoCountScan = 2;
+constexpr int kAlgoAuto = 0, kAlgoOnesweep = 1, kAlgoCountScan = 2, kAlgoLds = 3;
+constexpr uint32_t kLdsMax = 512 * 16;
 
 size_t sort_workspace_bytes(uint64_t n);
 hipError_t workspace_init(void *ws, size_t bytes, hipStream_t stream);

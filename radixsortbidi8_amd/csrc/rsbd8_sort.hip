@@ -42,6 +42,16 @@ __device__ __forceinline__ uint32_t pass_tag(uint32_t epoch, int pass)
 	return ((epoch * kMaxPasses + static_cast<uint32_t>(pass)) % 0x3FFFFFFFu) + 1u;
 }
 
+// 0: a real sort is needed; 1: the output is the input as it is; 2: the output is the input
+// reversed (physical order). For isrevorder the logical input is the physical one read backwards.
+__device__ __forceinline__ uint32_t presort_result(uint32_t check, int reverse)
+{
+	if (!(check & 1u)) return 0;
+	const bool lt = check & 2u, eq = check & 4u, gt = check & 8u;
+	if (!reverse) return !gt ? 1u : ((!lt && !eq) ? 2u : 0u);
+	return !lt ? 2u : ((!gt && !eq) ? 1u : 0u);
+}
+
 // ---------------------------------------------------------------------------------------------
 // histogram of every digit in one read
 
@@ -92,8 +102,9 @@ __device__ __forceinline__ void wave_count(uint32_t *s, uint32_t d)
 
 template <typename K, int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_hist(const K *__restrict__ in, uint64_t n, KeyXform xf,
-	uint32_t *__restrict__ hist)
+	uint32_t *__restrict__ hist, const Control *ctrl, int reverse)
 {
+	if (presort_result(ctrl->check, reverse)) return;  // the input is already in order
 	constexpr int P = sizeof(K);
 	__shared__ uint32_t s_h[P * kRadix];
 	for (int i = threadIdx.x; i < P * kRadix; i += THREADS) s_h[i] = 0;
@@ -112,6 +123,75 @@ __global__ __launch_bounds__(THREADS) void k_hist(const K *__restrict__ in, uint
 	__syncthreads();
 	for (int i = threadIdx.x; i < P * kRadix; i += THREADS)
 		if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+
+// ---------------------------------------------------------------------------------------------
+// presort check: is the input already in order, or strictly in reverse order?
+
+template <typename K, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_check(const K *__restrict__ in, uint64_t n, KeyXform xf, Control *ctrl)
+{
+	if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&ctrl->check, 1u);
+	if (n < 2) return;
+	const uint64_t pairs = n - 1;
+	const uint64_t chunk = (pairs + gridDim.x - 1) / gridDim.x;
+	const uint64_t lo = static_cast<uint64_t>(blockIdx.x) * chunk;
+	const uint64_t hi = min(pairs, lo + chunk);
+	uint32_t mine = 0;
+	uint64_t it = 0;
+	for (uint64_t j = lo + threadIdx.x; j < hi; j += THREADS, ++it) {
+		if ((it & 63u) == 0) {
+			// stop once both an ascent and a descent were seen anywhere: no shortcut is possible
+			const uint32_t seen = __hip_atomic_load(&ctrl->check, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) | mine;
+			if ((seen & 10u) == 10u) break;
+		}
+		const auto a = sort_key<K>(in[j], xf), b = sort_key<K>(in[j + 1], xf);
+		mine |= a < b ? 2u : (a == b ? 4u : 8u);
+		if ((it & 63u) == 63u && mine) { atomicOr(&ctrl->check, mine); }
+	}
+	if (mine) atomicOr(&ctrl->check, mine);
+}
+
+// One block sorts the whole array in LDS (8-bit LSD passes with the block ranking), keys only.
+template <typename K, int THREADS, int KPT>
+__global__ __launch_bounds__(THREADS) void k_lds_sort(const K *__restrict__ in, K *__restrict__ out, uint32_t n,
+	KeyXform xf, int reverse)
+{
+	constexpr int CAP = THREADS * KPT;
+	constexpr int NW = THREADS / 64;
+	constexpr int P = sizeof(K);
+	__shared__ uint32_t s_whist[NW * 256];
+	__shared__ uint32_t s_digoff[256];
+	__shared__ uint32_t s_wsum[4];
+	__shared__ K s_keys[CAP];
+	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+	K keys[KPT];
+#pragma unroll
+	for (int j = 0; j < KPT; ++j) {
+		const uint32_t l = w * 64u * KPT + j * 64u + lane;
+		keys[j] = l < n ? in[reverse ? n - 1 - l : l] : K(0);
+	}
+	for (int p = 0; p < P; ++p) {
+		uint32_t dig[KPT], pos[KPT], total;
+#pragma unroll
+		for (int j = 0; j < KPT; ++j) {
+			const uint32_t l = w * 64u * KPT + j * 64u + lane;
+			dig[j] = l < n ? digit_of<K>(keys[j], xf, 8u * p) : 0xFFu;
+		}
+		block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total);
+#pragma unroll
+		for (int j = 0; j < KPT; ++j) s_keys[pos[j]] = keys[j];
+		__syncthreads();
+#pragma unroll
+		for (int j = 0; j < KPT; ++j) keys[j] = s_keys[w * 64u * KPT + j * 64u + lane];
+		__syncthreads();
+	}
+#pragma unroll
+	for (int j = 0; j < KPT; ++j) {
+		const uint32_t l = w * 64u * KPT + j * 64u + lane;
+		if (l < n) out[l] = keys[j];
+	}
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -153,9 +233,10 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 		__syncthreads();
 	}
 	if (t != 0) return;
+	const uint32_t pres = presort_result(ctrl->check, a.reverse);
 	int A = 0;
 	uint32_t mask = 0;
-	for (int p = 0; p < a.passes; ++p)
+	for (int p = 0; p < a.passes && !pres; ++p)
 		if (!s_uniform[p]) { ++A; mask |= 1u << p; }
 	ctrl->num_active = static_cast<uint32_t>(A);
 	ctrl->active_mask = mask;
@@ -171,8 +252,9 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 		PassPlan &f = ctrl->fin;
 		f.src = a.src; f.dst = a.fin; f.psrc = a.psrc; f.pdst = a.pfin;
 		f.iota = a.payload && !a.psrc;
-		f.reverse = a.reverse;
-		f.active = (a.src != a.fin) || a.reverse || (a.payload && (f.iota || a.psrc != a.pfin));
+		// already in order: the result is the input (reversed when it was strictly descending)
+		f.reverse = pres ? (pres == 2u) : a.reverse;
+		f.active = (a.src != a.fin) || f.reverse || (a.payload && (f.iota || a.psrc != a.pfin));
 		return;
 	}
 	const void *src = a.src;
@@ -370,8 +452,10 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(Control *ctrl, const uint3
 // logical positions [t*tile_keys, (t+1)*tile_keys), read back to front under isrevorder).
 template <typename K, int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_hist_tiles(const K *__restrict__ in, uint64_t n, KeyXform xf,
-	uint32_t *__restrict__ hist, uint32_t *__restrict__ counts, uint32_t tile_keys, uint32_t num_tiles, int reverse)
+	uint32_t *__restrict__ hist, uint32_t *__restrict__ counts, uint32_t tile_keys, uint32_t num_tiles, int reverse,
+	const Control *ctrl)
 {
+	if (presort_result(ctrl->check, reverse)) return;  // the input is already in order
 	constexpr int P = sizeof(K);
 	__shared__ uint32_t s_h[P * kRadix];
 	__shared__ uint32_t s_t[kRadix];
@@ -593,7 +677,10 @@ __global__ void k_finish(Control *ctrl)
 		ctrl->tile_counter[threadIdx.x] = 0;
 		ctrl->scan_counter[threadIdx.x] = 0;
 	}
-	if (threadIdx.x == 0) ctrl->epoch = ctrl->epoch + 1u;
+	if (threadIdx.x == 0) {
+		ctrl->epoch = ctrl->epoch + 1u;
+		ctrl->check = 0;
+	}
 }
 
 __global__ void k_zero(uint64_t *p, uint64_t words)
@@ -729,22 +816,44 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 
 	constexpr int HITEMS = 16 / sizeof(K) < 4 ? 4 : 16 / sizeof(K) * 2;
 	const int cfg = job.cfg >= 0 ? job.cfg : default_cfg(P, n, job.payload);
-	const int algo = job.algo ? job.algo : default_algo(P, n);
+	int algo = job.algo ? job.algo : default_algo(P, n);
+	if (algo == kAlgoLds) {
+		if (!job.payload && n <= kLdsMax && !job.keys_final_any) {
+			// one block holds the whole array: every load happens before any store, so in place is fine
+			const int rev = (job.flags & kRev) ? 1 : 0;
+			const uint32_t nn = static_cast<uint32_t>(n);
+			ProfScope ps(1, 0, ws, stream);
+			if (n <= 1024)
+				hipLaunchKernelGGL((k_lds_sort<K, 256, 4>), dim3(1), dim3(256), 0, stream, static_cast<const K *>(job.in), static_cast<K *>(job.out), nn, xf, rev);
+			else if (n <= 4096)
+				hipLaunchKernelGGL((k_lds_sort<K, 256, 16>), dim3(1), dim3(256), 0, stream, static_cast<const K *>(job.in), static_cast<K *>(job.out), nn, xf, rev);
+			else
+				hipLaunchKernelGGL((k_lds_sort<K, 512, 16>), dim3(1), dim3(512), 0, stream, static_cast<const K *>(job.in), static_cast<K *>(job.out), nn, xf, rev);
+			return hipGetLastError();
+		}
+		algo = default_algo(P, n);
+	}
 	const uint32_t tile_keys = static_cast<uint32_t>(kTileCfgs[cfg].threads * kTileCfgs[cfg].kpt);
 	const uint32_t tiles = static_cast<uint32_t>((n + tile_keys - 1) / tile_keys);
 	uint32_t *counts = counts_of(ws, n);
 	const bool fill = (P == 1) && !job.payload && job.fill8 &&
 		(job.flags & 7u) != (kAbs | kSign) && (job.flags & 7u) != (kSign | kFltp) && (job.flags & 7u) != (kAbs | kSign | kFltp);
 	const bool counts0 = algo == kAlgoCountScan && !fill;
+	if (job.presort_check && !fill) {
+		const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 4095) / 4096, 4u * cus)));
+		hipLaunchKernelGGL((k_check<K, 256>), dim3(grid), dim3(256), 0, stream, static_cast<const K *>(job.in), n, xf, ctrl);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+	}
 	{
 		ProfScope ps(0, 0, ws, stream);
 		if (counts0) {
 			const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, 8u * cus));
 			hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS>), dim3(grid), dim3(256), 0, stream, static_cast<const K *>(job.in), n,
-				xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0);
+				xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0, ctrl);
 		} else {
 			const uint32_t hgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 256 * HITEMS - 1) / (256 * HITEMS), 8u * cus)));
-			hipLaunchKernelGGL((k_hist<K, 256, HITEMS>), dim3(hgrid), dim3(256), 0, stream, static_cast<const K *>(job.in), n, xf, hist);
+			hipLaunchKernelGGL((k_hist<K, 256, HITEMS>), dim3(hgrid), dim3(256), 0, stream, static_cast<const K *>(job.in), n, xf, hist,
+				ctrl, (job.flags & kRev) ? 1 : 0);
 		}
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 	}

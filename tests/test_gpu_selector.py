@@ -1,0 +1,132 @@
+"""GPU checks of the plan selector.
+
+* every plan (algorithm x tile configuration x presort check, and the single-block LDS sort) gives
+  the oracle's output bit for bit, on random, sorted, reverse-sorted and nearly sorted inputs;
+* the feature kernel against features computed on the host from the same sampled positions;
+* the network kernels: forward against the float64 reference, gradients against the float64
+  backpropagation (itself pinned by finite differences in test_selector_ref.py), a training run
+  that lowers the loss;
+* a trained network beats the autotuned fixed plan on a held-out mix with presorted inputs.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+r = pytest.importorskip("radixsortbidi8_amd")
+from radixsortbidi8_amd import planner as pl  # noqa: E402
+
+
+@pytest.mark.parametrize("dt", ["u32", "u64", "f32", "u16", "u8", "f64"])
+def test_every_plan_bit_exact(dt):
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    arena = pl.Arena(1 << 17)
+    for n in (3, 1000, 5000, 100003):
+        for dist in ("uniform", "sorted", "reverse", "unique16", "nearly"):
+            keys = pl.make_keys(dt, n, "sorted" if dist == "nearly" else dist, gen)
+            if dist == "nearly":
+                keys = keys.clone()
+                sv = pl._signed_view(keys)
+                a, b = sv[n // 2].clone(), sv[n // 2 + 1].clone()
+                sv[n // 2], sv[n // 2 + 1] = b, a
+            host = keys.cpu().numpy()
+            for d in (0, 1, 2, 3):
+                flags = r.decode(r.keytype_of(keys.dtype), 64, d)
+                exp, _ = oracle.radixsort(host, flags)
+                out, buf = arena.views(keys)
+                for p in pl.PLANS:
+                    if not pl.plan_valid(p, n):
+                        continue
+                    out.zero_()
+                    pl.sort_with_plan(keys, out, buf, arena.ws, p, flags)
+                    torch.cuda.synchronize()
+                    got = out.cpu().numpy()
+                    assert np.array_equal(got.view(np.uint8), exp.view(np.uint8)), (dt, n, dist, d, p)
+
+
+def test_features_match_host():
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    keys = pl.make_keys("u32", 1 << 18, "zipf", gen)
+    flags = r.decode(r.U32)
+    f = pl.features(keys, flags).cpu().numpy()
+    h = keys.cpu().numpy()
+    n = h.size
+    samp = h[(np.arange(4096, dtype=np.int64) * n) // 4096]
+    for p in range(4):
+        c = np.bincount((samp >> (8 * p)) & 0xFF, minlength=256) / 4096.0
+        c = c[c > 0]
+        assert abs(f[10 + p] - (-(c * np.log2(c)).sum() / 8)) < 1e-4
+    idx = (np.arange(2048, dtype=np.int64) * (n - 1)) // 2048
+    a, b = h[idx], h[idx + 1]
+    assert abs(f[18] - np.mean(a < b)) < 1e-6 and abs(f[20] - np.mean(a > b)) < 1e-6
+    assert abs(f[0] - 18 / 32) < 1e-6 and f[3] == 1.0
+
+
+def test_network_kernels_match_reference():
+    rng = np.random.default_rng(2)
+    sel = pl.Selector(seed=4)
+    p = sel.params.cpu().numpy().astype(np.float64)
+    x = rng.random((37, pl.NUM_FEATURES)).astype(np.float32)
+    t = rng.normal(8, 2, (37, pl.NUM_PLANS)).astype(np.float32)
+    m = (rng.random((37, pl.NUM_PLANS)) < 0.8).astype(np.float32)
+    y = sel.forward(torch.from_numpy(x).cuda()).cpu().numpy()
+    _, _, yr = pl.ref_forward(p, x.astype(np.float64), sel.F, sel.H, sel.O)
+    assert np.allclose(y, yr, rtol=1e-4, atol=1e-4)
+    loss, g = sel.gradient(torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda(), torch.from_numpy(m).cuda())
+    lr, gr = pl.ref_loss_grad(p, x, t.astype(np.float64), m.astype(np.float64), sel.F, sel.H, sel.O)
+    assert abs(loss - lr) < 1e-3 * max(1.0, abs(lr))
+    g = g.cpu().numpy()
+    assert np.allclose(g, gr, rtol=2e-3, atol=2e-5 * np.abs(gr).max())
+
+
+def test_training_lowers_loss():
+    rng = np.random.default_rng(9)
+    x = torch.from_numpy(rng.random((64, pl.NUM_FEATURES)).astype(np.float32)).cuda()
+    w = rng.normal(0, 1, (pl.NUM_FEATURES, pl.NUM_PLANS))
+    t = torch.from_numpy((6 + x.cpu().numpy() @ w).astype(np.float32)).cuda()
+    m = torch.ones_like(t)
+    sel = pl.Selector(seed=1)
+    l0, _ = sel.gradient(x, t, m)
+    l1 = sel.train(x, t, m, steps=600, lr=3e-3)
+    assert l1 < 0.2 * l0
+
+
+def test_trained_selector_beats_fixed_plan():
+    """Train on a small mix that includes presorted arrays, then compare on other seeds with every
+    plan timed: the network's pick must cost less in total than the fixed table's (tuned on uniform
+    keys), and the library's fused selection must sort bit-identically to the fixed plan."""
+    def stream(count, seed):
+        rng = np.random.default_rng(seed)
+        out = []
+        for _ in range(count):
+            dt = ("u32", "u64", "f32")[int(rng.integers(0, 3))]
+            n = int(2.0 ** rng.uniform(17, 21))
+            dist = ("uniform", "sorted", "reverse", "zipf")[int(rng.integers(0, 4))]
+            out.append((dt, n, dist, int(rng.integers(0, 2 ** 62))))
+        return out
+    table = pl.autotune(log2_sizes=range(16, 23, 2), dtypes=("u32", "u64"), reps=2)
+    X, T, M = pl.collect(stream(48, 100), reps=2)
+    sel = pl.Selector(seed=2)
+    sel.train(torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda(), torch.from_numpy(M).cuda(), steps=1500)
+    held = stream(40, 200)
+    HX, HT, HM = pl.collect(held, reps=3)
+    off = pl.regret(held, HX, HT, HM, sel, table)
+    live = pl.evaluate(held, sel, table)
+    print(off, live)
+    assert live["mismatches"] == 0
+    assert off["learned_ms"] < 0.95 * off["fixed_ms"], off
+
+
+def test_library_selection_matches_host_choice():
+    """rsbd8_hip_select_plan (features, network and argmin in one block) picks the same plan as the
+    feature kernel + forward kernel + host argmin."""
+    gen = torch.Generator(device="cuda").manual_seed(21)
+    sel = pl.Selector(seed=7)
+    for dt, n, dist in (("u32", 100000, "uniform"), ("u64", 5000, "sorted"), ("f32", 3000, "zipf"),
+                        ("u16", 70000, "reverse"), ("u8", 20, "uniform")):
+        keys = pl.make_keys(dt, n, dist, gen)
+        flags = r.decode(r.keytype_of(keys.dtype))
+        f = pl.features(keys, flags)
+        assert sel.select(keys, flags) == sel.choose(f, n), (dt, n, dist)
