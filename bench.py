@@ -1,11 +1,13 @@
 #!/usr/bin/env python3
 """Headline benchmark: stable LSD radix sort of 2^28 uniform u64 keys on one MI355X.
 
-One step = one complete sort (rsbd8::radixsortcopynoalloc semantics: histogram of every digit,
-plan, every executed scatter pass) of the device-resident input into the output array.
-Prints one JSON line: sorted GKeys/s (value), the HBM roofline of the dominant kernel (the
-onesweep scatter pass, timed per launch with HIP events on its stream), the whole-sort roofline
-fraction, and the multithreaded CPU baseline (the oracle's sliced LSD sort) timed on this host.
+One step = one complete sort (rsbd8::radixsortcopynoalloc semantics) of the device-resident input
+into the output array: the histogram of every digit, the device-side plan, and every executed
+partition, scatter and LDS sort launch of the plan it chose.
+Prints one JSON line: sorted GKeys/s (value), the HBM roofline of the dominant kernel (of the
+kernels that read and write every key, the one with the most time; timed per launch with HIP
+events on its stream), the whole-sort roofline fraction with a per-phase breakdown, and the
+multithreaded CPU baseline (the oracle's sliced LSD sort) timed on this host.
 """
 import argparse
 import json
@@ -44,10 +46,11 @@ def cpu_baseline(sample_log2: int, budget_s: float):
                       f"reference's 11-bit digit plan sliced over {threads} threads, median of {len(times)}"}
 
 
-def traffic_from_profile(n_log2: int):
-    """HBM bytes per scatter launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE), as
-    recorded in profiles/ by the counter runs; None when no such record exists."""
-    path = os.path.join(ROOT, "profiles", f"onesweep_u64_2p{n_log2}_traffic.json")
+def traffic_from_profile(n_log2: int, family: str):
+    """HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled by what the same counters read for a plain copy of known size), as recorded
+    in profiles/ by the counter runs; None when no such record exists."""
+    path = os.path.join(ROOT, "profiles", f"{family}_u64_2p{n_log2}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -65,6 +68,7 @@ def main():
     ap.add_argument("--cpu-budget-s", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tile-config", type=int, default=-1)
+    ap.add_argument("--algo", type=int, default=0, help="0 automatic, 1 onesweep, 2 count + scan, 4 MSD plan")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: every run here has exactly one MI355X; "
@@ -86,6 +90,7 @@ def main():
     ws = r.Workspace(r.U64, n)
     if args.tile_config >= 0:
         r.set_tile_config(args.tile_config)
+    r.set_algorithm(args.algo)
     stream = torch.cuda.current_stream()
 
     for _ in range(args.warmup):
@@ -113,20 +118,44 @@ def main():
           flush=True)
 
     k = 8
-    scatter_ms, scatter_launches = prof["scatter"]
-    count_ms, count_launches = prof["countscan"]
-    passes = round(scatter_launches / args.steps) if args.steps else 0
-    counted = round(count_launches / args.steps) if args.steps else 0
-    avg_pass_ms = scatter_ms / max(1, scatter_launches)
-    bytes_per_pass = n * 2 * k
-    achieved = bytes_per_pass / (avg_pass_ms * 1e-3) / 1e9
-    # the plan as executed: one histogram read, one count read per pass whose tile counts were not
-    # produced by the histogram read, and a read + write per executed scatter pass
-    count_reads = max(0, counted - 1)
-    sort_bytes = n * k + count_reads * n * k + passes * n * 2 * k + (2 * n * k if passes == 0 else 0)
-    ref_bytes = n * k + passes * n * 2 * k + (2 * n * k if passes == 0 else 0)
+    steps = max(1, args.steps)
+    # per kernel family: (total ms, launches) over the timed steps, and the algorithmic HBM bytes
+    # of one launch (keys read + keys written; the count/histogram reads read every key once)
+    fam = {
+        "lsd_scatter": (prof["scatter"], 2 * k * n),
+        "lsd_count_scan": (prof["countscan"], k * n),
+        "msd_scatter": (prof["msd_scatter"], 2 * k * n),
+        "msd_local_sort": (prof["msd_local"], 2 * k * n),
+        "msd_count_scan": (prof["msd_countscan"], 0),
+        "histogram": (prof["hist"], k * n),
+    }
+    msd = prof["msd_local"][1] > 0
+    if msd:
+        # hist read, top-digit scatter, second-digit count read, segmented scatter, LDS sort
+        sort_bytes = k * n + 2 * k * n + k * n + 2 * k * n + 2 * k * n
+        passes = 2
+        plan = ("two MSD partitions by the top two 8-bit digits (count + chained scan + scatter, the second "
+                "one segmented), then one LDS sort per top-16-bit bucket (an LDS pass on the third digit and "
+                "exact ranking inside its runs)")
+    else:
+        passes = round(prof["scatter"][1] / steps)
+        counted = round(prof["countscan"][1] / steps)
+        count_reads = max(0, counted - 1)
+        sort_bytes = k * n + count_reads * k * n + passes * 2 * k * n + (2 * k * n if passes == 0 else 0)
+        plan = f"8-bit LSD digits, {passes} executed passes, " + (
+            "count + chained scan + scatter per pass" if counted else "onesweep scatter per pass")
+    # the dominant kernel: of the kernels that read and write every key, the one with the most time
+    dom = max(("lsd_scatter", "msd_scatter", "msd_local_sort"), key=lambda f: fam[f][0][0])
+    (dom_ms, dom_launches), dom_bytes = fam[dom]
+    avg_ms = dom_ms / max(1, dom_launches)
+    achieved = dom_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    # the reference's own plan for this key type: one counting read, then six 11-bit-digit passes
+    # (H:1848-1896) that each read and write every key
+    ref_bytes = k * n + 6 * 2 * k * n
     sort_gbs = sort_bytes / (ms_per_step * 1e-3) / 1e9
-    traffic = traffic_from_profile(args.n_log2)
+    traffic = traffic_from_profile(args.n_log2, dom)
+    phases = {f: {"ms_per_step": round(v[0][0] / steps, 4), "launches_per_step": round(v[0][1] / steps, 2)}
+              for f, v in fam.items() if v[0][1]}
     line = {
         "metric": METRIC,
         "value": round(n / (ms_per_step * 1e-3) / 1e9, 4),
@@ -143,20 +172,21 @@ def main():
         "config": {
             "workload": "2^28 uniform u64 keys (seeded), stable ascending sort, keys only, device-resident copy in->out",
             "n": n, "key": "u64", "direction": "ascfwdorder", "mode": "native",
-            "plan": f"8-bit digits, {passes} executed passes, " + (
-                "count + chained scan + scatter per pass" if counted else "onesweep scatter per pass"),
+            "plan": plan,
             "tile_config": r.TILE_CONFIGS[args.tile_config] if args.tile_config >= 0 else "auto",
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": ("k_scatter" if counted else "k_onesweep") + " (one scatter pass: read + write of every key)",
+            "kernel": {"lsd_scatter": "k_scatter (one LSD scatter pass: read + write of every key)",
+                       "msd_scatter": "k_scatter (one MSD partition pass: read + write of every key)",
+                       "msd_local_sort": "k_local (the per-bucket LDS sort: read + write of every key)"}[dom],
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "algorithmic_bytes_per_launch": bytes_per_pass,
-            "avg_launch_ms": round(avg_pass_ms, 4),
+            "algorithmic_bytes_per_launch": dom_bytes,
+            "avg_launch_ms": round(avg_ms, 4),
         },
         "sort_roofline": {
             "algorithmic_bytes": sort_bytes,
@@ -164,8 +194,7 @@ def main():
             "frac": round(sort_gbs / HBM_PEAK_GBS, 4),
             "reference_plan_bytes": ref_bytes,
             "reference_plan_frac": round(ref_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "hist_ms": round(prof["hist"][0] / max(1, prof["hist"][1]), 4),
-            "count_scan_ms_per_pass": round(count_ms / max(1, count_launches), 4),
+            "phases": phases,
         },
         "verified": sorted_ok and perm_ok,
         "learned_vs_fixed": None,

@@ -149,7 +149,10 @@ int rsbd8_hip_set_tile_config(int cfg);
 /* Process-wide pass algorithm: 1 onesweep (each scatter pass resolves its tile offsets with a
  * decoupled look-back over the preceding tiles), 2 count + scan + scatter (per-tile digit counts,
  * a decoupled look-back chained scan of the count matrix, then a scatter with no inter-block
- * dependency), 0 automatic. Returns the previous setting. */
+ * dependency), 4 two MSD partitions by the top two digits followed by one LDS sort per resulting
+ * bucket (keys-only sorts of 4- and 8-byte keys; others use 2, and the device itself falls back to
+ * 2 when the top digits are too skewed for the buckets to fit in LDS), 0 automatic. Returns the
+ * previous setting. */
 int rsbd8_hip_set_algorithm(int algo);
 
 /* A sort plan chosen per call (no reference counterpart; the reference picks its plan from

@@ -186,11 +186,13 @@ def set_tile_config(cfg: int) -> int:
     return prev
 
 
-ALGO_AUTO, ALGO_ONESWEEP, ALGO_COUNTSCAN = 0, 1, 2
+ALGO_AUTO, ALGO_ONESWEEP, ALGO_COUNTSCAN, ALGO_MSD2 = 0, 1, 2, 4
 
 
 def set_algorithm(algo: int) -> int:
-    """Force the pass algorithm (ALGO_ONESWEEP or ALGO_COUNTSCAN) or ALGO_AUTO; returns the previous."""
+    """Force the pass algorithm (ALGO_ONESWEEP, ALGO_COUNTSCAN, or ALGO_MSD2: two MSD partitions and
+    a per-bucket LDS sort for keys-only 4/8-byte sorts, others fall back to ALGO_COUNTSCAN) or
+    ALGO_AUTO; returns the previous."""
     prev = _load().rsbd8_hip_set_algorithm(int(algo))
     if prev < 0:
         _check(prev)
@@ -204,9 +206,11 @@ def profile_begin():
 
 def profile_end():
     """Stop recording; returns {kind: (total_ms, launches)} for kinds 'hist', 'scatter'
-    (executed passes), 'scatter_skipped', 'countscan' (count + scan of executed passes) and
-    'countscan_skipped'."""
-    kinds = ("hist", "scatter", "scatter_skipped", "countscan", "countscan_skipped")
+    (executed passes), 'scatter_skipped', 'countscan' (count + scan of executed passes),
+    'countscan_skipped', and for the two-level MSD plan when it ran 'msd_scatter' (its two
+    partition scatters), 'msd_local' (the per-bucket LDS sort) and 'msd_countscan'."""
+    kinds = ("hist", "scatter", "scatter_skipped", "countscan", "countscan_skipped", "msd_scatter", "msd_local",
+             "msd_countscan")
     ms = (ctypes.c_double * len(kinds))()
     cnt = (ctypes.c_uint * len(kinds))()
     _check(_load().rsbd8_hip_profile_end(ms, cnt, len(kinds)))

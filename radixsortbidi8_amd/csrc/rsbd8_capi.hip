@@ -255,7 +255,7 @@ int rsbd8_hip_set_tile_config(int cfg)
 
 int rsbd8_hip_set_algorithm(int algo)
 {
-	if (algo < 0 || algo > 2) return fail(RSBD8_HIP_EINVAL, "algorithm out of range");
+	if (algo < 0 || algo > 4 || algo == 3) return fail(RSBD8_HIP_EINVAL, "algorithm out of range");
 	return g_algo.exchange(algo);
 }
 
@@ -566,7 +566,7 @@ int sort_with_plan(int keytype, unsigned flags, size_t count, void *workspace, s
 	if (!input || !output || !workspace) return fail(RSBD8_HIP_EINVAL, "null argument");
 	if (input == output || (buffer && (buffer == input || buffer == output))) return fail(RSBD8_HIP_EINVAL, "arrays alias");
 	if (!buffer && needs_buffer(kb, flags) && kb != 1) return fail(RSBD8_HIP_EINVAL, "buffer required");
-	if (plan.algo < 0 || plan.algo > 3 || plan.tile_cfg < -1 || plan.tile_cfg >= kNumTileCfgs)
+	if (plan.algo < 0 || plan.algo > 4 || plan.tile_cfg < -1 || plan.tile_cfg >= kNumTileCfgs)
 		return fail(RSBD8_HIP_EINVAL, "plan out of range");
 	SortJob job;
 	job.key_bytes = kb; job.flags = flags; job.n = count;

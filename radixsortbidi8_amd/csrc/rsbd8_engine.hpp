@@ -42,12 +42,31 @@ struct Control {
 	PassPlan pass[kMaxPasses];
 	PassPlan pre;          // copy before the first pass (in-place sorts that would overlap)
 	PassPlan fin;          // copy when every digit is uniform (no pass executes)
+	// two-level MSD plan (keys only, 4- and 8-byte keys): hyb[0] partitions by the top digit,
+	// hyb[1] by the second digit inside each top-digit segment, hyb[2] sorts every resulting
+	// bucket in LDS. The LSD plan is kept in lsd_* and put back when a bucket turns out too large.
+	PassPlan hyb[3];
+	PassPlan lsd_pass[kMaxPasses];
+	PassPlan lsd_pre;
+	PassPlan lsd_fin;
+	uint32_t lsd_counts_ready;
+	uint32_t lsd_active_mask;
+	uint32_t hyb_scan_counter;
+	uint32_t hyb_tiles2;   // tiles of the segmented second pass
+	uint32_t hyb_over;     // some bucket exceeds the LDS capacity: use the LSD plan
+	uint32_t hyb_queue;    // buckets queued for the LDS LSD passes (long runs of one third digit)
 };
 
 constexpr size_t kCtrlBytes = 4096;
 constexpr size_t kHistBytes = kMaxPasses * kRadix * sizeof(uint32_t);
 constexpr size_t kLookbackOffset = kCtrlBytes + 2 * kHistBytes;
 constexpr int kScanChunk = 64;  // tiles per block of the chained scan over the tile counts
+constexpr int kTagSlots = 16;   // look-back tag slots per sort: LSD passes 0..7, the MSD plan's scan 8
+constexpr uint32_t kHybBuckets = kRadix * kRadix;  // buckets after the two MSD partitions
+// tables of the two-level MSD plan after the tile counts: first tile of every top-digit segment
+// (257 words), segment starts and lengths (256 + 256), bucket starts and lengths (2 x 65536), and
+// the queue of buckets for the LDS LSD passes (65536)
+constexpr size_t kSegTableWords = 257 + 256 + 256 + 3 * kHybBuckets;
 static_assert(sizeof(Control) <= kCtrlBytes, "control block too large");
 
 struct SortJob {
@@ -65,18 +84,20 @@ struct SortJob {
 	bool fill8 = true;             // 8-bit keys-only bijective modes may use the count fill
 	bool keys_final_any = false;   // only the payload is wanted: keys may end in out or buf
 	int algo = 0;                  // 0 automatic, 1 onesweep (look-back in the scatter), 2 count + chained scan + scatter,
-	                               // 3 one block sorts everything in LDS (keys only, n <= kLdsMax)
+	                               // 3 one block sorts everything in LDS (keys only, n <= kLdsMax),
+	                               // 4 two MSD partitions + per-bucket LDS sort (keys only, 4/8-byte keys;
+	                               //   the device falls back to algorithm 2 for skewed keys)
 	bool presort_check = false;    // first test whether the input is already in order (or strictly reversed)
 };
 
-constexpr int kAlgoAuto = 0, kAlgoOnesweep = 1, kAlgoCountScan = 2, kAlgoLds = 3;
+constexpr int kAlgoAuto = 0, kAlgoOnesweep = 1, kAlgoCountScan = 2, kAlgoLds = 3, kAlgoHybrid = 4;
 constexpr uint32_t kLdsMax = 512 * 16;
 
 size_t sort_workspace_bytes(uint64_t n);
 hipError_t workspace_init(void *ws, size_t bytes, hipStream_t stream);
 hipError_t sort_launch(const SortJob &job, void *ws, size_t ws_bytes, hipStream_t stream);
 int default_cfg(int key_bytes, uint64_t n, bool payload);
-int default_algo(int key_bytes, uint64_t n);
+int default_algo(int key_bytes, uint64_t n, bool keys_only);
 
 // Gather keys through records or pointers into a dense key array (indirection layouts).
 hipError_t gather_by_index(int key_bytes, uint64_t n, const void *base, uint64_t stride, uint64_t offset,

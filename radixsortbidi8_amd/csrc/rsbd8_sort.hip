@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <vector>
 
@@ -32,14 +33,34 @@ inline uint32_t *hist_of(void *ws) { return reinterpret_cast<uint32_t *>(static_
 inline uint32_t *bucket_of(void *ws) { return reinterpret_cast<uint32_t *>(static_cast<char *>(ws) + kCtrlBytes + kHistBytes); }
 inline uint64_t *lookback_of(void *ws) { return reinterpret_cast<uint64_t *>(static_cast<char *>(ws) + kLookbackOffset); }
 inline uint64_t max_tiles(uint64_t n) { return (n + kMinTile - 1) / kMinTile + 1; }
+// rows of the tile-count matrix: the tiles of a pass, plus one partial tile per top-digit segment
+// when the tiles of the segmented MSD pass are aligned to the segments
+inline uint64_t count_rows(uint64_t n) { return max_tiles(n) + kRadix; }
 inline uint32_t *counts_of(void *ws, uint64_t n)
 {
 	return reinterpret_cast<uint32_t *>(static_cast<char *>(ws) + kLookbackOffset + max_tiles(n) * kRadix * sizeof(uint64_t));
 }
+inline uint32_t *segtab_of(void *ws, uint64_t n) { return counts_of(ws, n) + count_rows(n) * kRadix; }
 
-__device__ __forceinline__ uint32_t pass_tag(uint32_t epoch, int pass)
+__device__ __forceinline__ uint32_t pass_tag(uint32_t epoch, int slot)
 {
-	return ((epoch * kMaxPasses + static_cast<uint32_t>(pass)) % 0x3FFFFFFFu) + 1u;
+	return ((epoch * kTagSlots + static_cast<uint32_t>(slot)) % 0x3FFFFFFFu) + 1u;
+}
+
+// The MSD plan's tables (see kSegTableWords): tile_start[257], seg_start[256], seg_len[256],
+// bucket_start[65536], bucket_len[65536], queue[65536].
+struct SegTables {
+	uint32_t *tile_start;
+	uint32_t *seg_start;
+	uint32_t *seg_len;
+	uint32_t *bucket_start;
+	uint32_t *bucket_len;
+	uint32_t *queue;
+};
+__host__ __device__ inline SegTables seg_tables(uint32_t *base)
+{
+	return SegTables{base, base + 257, base + 257 + 256, base + 257 + 512, base + 257 + 512 + kHybBuckets,
+		base + 257 + 512 + 2 * kHybBuckets};
 }
 
 // 0: a real sort is needed; 1: the output is the input as it is; 2: the output is the input
@@ -210,29 +231,38 @@ struct PlanArgs {
 	int reverse;
 	int keys_final_any;
 	int counts0;
+	int hybrid;         // the two-level MSD plan may be chosen (keys only, 4/8-byte keys)
+	uint32_t hyb_cap;   // LDS capacity (keys) of the per-bucket sort
+	uint32_t tile2;     // tile size of the segmented second pass
+	uint32_t *segtab;   // the MSD plan's tables
 };
 
 __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_t *hist, uint32_t *bucket)
 {
 	__shared__ uint32_t s_wave[4];
 	__shared__ uint32_t s_uniform[kMaxPasses];
+	__shared__ uint32_t s_max[kMaxPasses];
+	__shared__ uint32_t s_top[kRadix];
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	for (int p = 0; p < a.passes; ++p) {
 		const uint32_t c = hist[p * kRadix + t];
 		hist[p * kRadix + t] = 0;  // leave the histogram zeroed for the next sort
+		if (p == a.passes - 1) s_top[t] = c;
 		const uint32_t incl = wave_inclusive_sum(c);
 		if (lane == 63) s_wave[w] = incl;
 		const bool full = static_cast<uint64_t>(c) == a.n;
+		if (t == 0) { s_uniform[p] = 0; s_max[p] = 0; }
 		__syncthreads();
 		uint32_t before = 0;
 		for (uint32_t v = 0; v < w; ++v) before += s_wave[v];
 		bucket[p * kRadix + t] = before + incl - c;
-		if (t == 0) s_uniform[p] = 0;
-		__syncthreads();
 		if (full) s_uniform[p] = 1;
+		atomicMax(&s_max[p], c);
 		__syncthreads();
 	}
 	if (t != 0) return;
+	PassPlan none{};
+	for (int h = 0; h < 3; ++h) ctrl->hyb[h] = none;
 	const uint32_t pres = presort_result(ctrl->check, a.reverse);
 	int A = 0;
 	uint32_t mask = 0;
@@ -244,7 +274,6 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 	ctrl->counts_ready = (a.counts0 && (mask & 1u)) ? 1u : 0u;
 	void *X[2] = {a.fin, a.other};
 	uint32_t *PX[2] = {a.pfin, a.pother};
-	PassPlan none{};
 	ctrl->pre = none;
 	ctrl->fin = none;
 	for (int p = 0; p < kMaxPasses; ++p) ctrl->pass[p] = none;
@@ -283,6 +312,40 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 		src = X[slot]; psrc = PX[slot]; rev = 0; iota = 0;
 		++j;
 	}
+	// The two-level MSD plan: both top digits must vary, and the largest bucket of the pair, as
+	// estimated from the two histograms, must fit the per-bucket LDS sort with room to spare (the
+	// exact bucket sizes are checked after the second count; larger ones fall back to this LSD plan).
+	const int P = a.passes;
+	if (!a.hybrid || P < 4 || !a.other || a.payload || s_uniform[P - 1] || s_uniform[P - 2]) return;
+	// (the same margin as the host's capacity choice: six standard deviations of a Poisson count)
+	const float est = static_cast<float>(static_cast<double>(s_max[P - 1]) * s_max[P - 2] / static_cast<double>(a.n));
+	if (est + 6.f * sqrtf(est) + 64.f > static_cast<float>(a.hyb_cap)) return;
+	void *X1 = a.other != a.src ? a.other : a.fin;
+	void *X2 = X1 == a.other ? a.fin : a.other;
+	for (int p = 0; p < kMaxPasses; ++p) { ctrl->lsd_pass[p] = ctrl->pass[p]; ctrl->pass[p] = none; }
+	ctrl->lsd_pre = ctrl->pre; ctrl->pre = none;
+	ctrl->lsd_fin = ctrl->fin; ctrl->fin = none;
+	ctrl->lsd_counts_ready = ctrl->counts_ready; ctrl->counts_ready = 0;
+	ctrl->lsd_active_mask = ctrl->active_mask; ctrl->active_mask = 0;
+	PassPlan h{};
+	h.active = 1;
+	h.src = a.src; h.dst = X1; h.reverse = a.reverse;
+	ctrl->hyb[0] = h;
+	h.src = X1; h.dst = X2; h.reverse = 0;
+	ctrl->hyb[1] = h;
+	h.src = X2; h.dst = a.fin;
+	ctrl->hyb[2] = h;
+	// tiles of the second pass never straddle a top-digit segment
+	const SegTables T = seg_tables(a.segtab);
+	uint32_t tiles = 0;
+	for (int sg = 0; sg < kRadix; ++sg) {
+		T.tile_start[sg] = tiles;
+		T.seg_start[sg] = bucket[(P - 1) * kRadix + sg];
+		T.seg_len[sg] = s_top[sg];
+		tiles += (s_top[sg] + a.tile2 - 1) / a.tile2;
+	}
+	T.tile_start[kRadix] = tiles;
+	ctrl->hyb_tiles2 = tiles;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -448,12 +511,13 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(Control *ctrl, const uint3
 
 
 // Tile-based read of every key: the histograms of all digits (as k_hist) and, when counts is not
-// null, the per-tile counts of digit 0 in the logical order of the first pass (tile t holds the
-// logical positions [t*tile_keys, (t+1)*tile_keys), read back to front under isrevorder).
+// null, the per-tile counts of the digit at count_shift in the logical order of the first pass
+// (tile t holds the logical positions [t*tile_keys, (t+1)*tile_keys), read back to front under
+// isrevorder).
 template <typename K, int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_hist_tiles(const K *__restrict__ in, uint64_t n, KeyXform xf,
 	uint32_t *__restrict__ hist, uint32_t *__restrict__ counts, uint32_t tile_keys, uint32_t num_tiles, int reverse,
-	const Control *ctrl)
+	const Control *ctrl, uint32_t count_shift)
 {
 	if (presort_result(ctrl->check, reverse)) return;  // the input is already in order
 	constexpr int P = sizeof(K);
@@ -471,7 +535,7 @@ __global__ __launch_bounds__(THREADS) void k_hist_tiles(const K *__restrict__ in
 			const auto k = sort_key<K>(x, xf);
 #pragma unroll
 			for (int p = 0; p < P; ++p) wave_count(s_h + p * kRadix, static_cast<uint32_t>(k >> (8 * p)) & 0xFFu);
-			if (counts) wave_count(s_t, static_cast<uint32_t>(k) & 0xFFu);
+			if (counts) wave_count(s_t, static_cast<uint32_t>(k >> count_shift) & 0xFFu);
 		});
 		__syncthreads();
 		if (counts)
@@ -509,18 +573,20 @@ __global__ __launch_bounds__(THREADS) void k_count(const Control *ctrl, int pass
 // Chained scan with decoupled look-back over the tile-count matrix: every block takes the next
 // chunk of kScanChunk tiles, scans it per digit, publishes the chunk total, resolves the prefix
 // of the preceding chunks and turns the counts into global output offsets in place.
-__global__ __launch_bounds__(256) void k_scan(Control *ctrl, int pass, const uint32_t *__restrict__ bucket,
-	uint32_t *__restrict__ counts, uint64_t *__restrict__ lookback, uint32_t num_tiles)
+// pp: the pass (inactive: exit), counter: its chunk ticket, start_row: the 256 bucket starts of its
+// digit, tag_slot: its look-back tag slot within the sort.
+__global__ __launch_bounds__(256) void k_scan(const Control *ctrl, const PassPlan *pp, uint32_t *counter,
+	const uint32_t *__restrict__ start_row, int tag_slot, uint32_t *__restrict__ counts, uint64_t *__restrict__ lookback,
+	uint32_t num_tiles)
 {
-	const PassPlan pp = ctrl->pass[pass];
-	if (!pp.active) return;
+	if (!pp->active) return;
 	const uint32_t nchunks = (num_tiles + kScanChunk - 1) / kScanChunk;
-	const uint32_t tag = pass_tag(ctrl->epoch, pass);
+	const uint32_t tag = pass_tag(ctrl->epoch, tag_slot);
 	const uint32_t d = threadIdx.x;
-	const uint32_t start = bucket[pass * kRadix + d];
+	const uint32_t start = start_row[d];
 	__shared__ uint32_t s_chunk;
 	for (;;) {
-		if (d == 0) s_chunk = atomicAdd(&ctrl->scan_counter[pass], 1u);
+		if (d == 0) s_chunk = atomicAdd(counter, 1u);
 		__syncthreads();
 		const uint32_t chunk = s_chunk;
 		__syncthreads();
@@ -555,21 +621,46 @@ __global__ __launch_bounds__(256) void k_scan(Control *ctrl, int pass, const uin
 // Scatter of one digit with known tile offsets. Blocks are dealt to the XCDs round robin, so
 // block b works on the tile range of XCD b % 8: neighbouring tiles, whose digit runs are
 // neighbours in the output, are written through the same L2.
-template <typename K, bool PAY, int THREADS, int KPT>
-__global__ __launch_bounds__(THREADS) void k_scatter(const Control *ctrl, int pass, KeyXform xf, uint64_t n,
-	uint32_t num_tiles, const uint32_t *__restrict__ offsets)
+// Tiles of the segmented MSD pass: segment s (a top-digit bucket) owns tiles
+// [tile_start[s], tile_start[s+1]), the last of them partial; no tile straddles two segments.
+struct SegMap {
+	const uint32_t *tile_start;  // 257 entries, tile_start[256] = number of tiles
+	const uint32_t *seg_start;
+	const uint32_t *seg_len;
+};
+
+// first key and key count of tile tl of a segment map (s_ts: tile_start staged in LDS)
+__device__ __forceinline__ void seg_tile(const uint32_t *s_ts, const SegMap &m, uint32_t TILE, uint32_t tl, uint64_t &base,
+	uint32_t &valid)
+{
+	// the last segment whose first tile is <= tl (empty segments share their successor's first tile)
+	uint32_t sg = 0;
+#pragma unroll
+	for (uint32_t step = 128; step; step >>= 1)
+		if (s_ts[sg + step] <= tl) sg += step;
+	const uint32_t k = tl - s_ts[sg];
+	base = static_cast<uint64_t>(m.seg_start[sg]) + static_cast<uint64_t>(k) * TILE;
+	const uint32_t left = m.seg_len[sg] - k * static_cast<uint32_t>(TILE);
+	valid = left < static_cast<uint32_t>(TILE) ? left : static_cast<uint32_t>(TILE);
+}
+
+// pp: the pass (inactive: exit); shift: its digit. With SEG, the tiles are those of the segment map
+// and num_tiles is an upper bound (the exact count is tile_start[256]).
+template <typename K, bool PAY, int THREADS, int KPT, bool SEG = false>
+__global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32_t shift, KeyXform xf, uint64_t n,
+	uint32_t num_tiles, const uint32_t *__restrict__ offsets, SegMap seg)
 {
 	constexpr int TILE = THREADS * KPT;
 	constexpr int NW = THREADS / 64;
-	const PassPlan pp = ctrl->pass[pass];
+	const PassPlan pp = *ppp;
 	if (!pp.active) return;
 	const K *__restrict__ src = static_cast<const K *>(pp.src);
 	K *__restrict__ dst = static_cast<K *>(pp.dst);
-	const uint32_t shift = 8u * static_cast<uint32_t>(pass);
 	__shared__ uint32_t s_whist[NW * kRadix];
 	__shared__ uint32_t s_digoff[kRadix];
 	__shared__ uint32_t s_glob[kRadix];
 	__shared__ uint32_t s_wsum[4];
+	__shared__ uint32_t s_ts[SEG ? kRadix + 1 : 1];
 	__shared__ uint8_t s_dig[PAY ? TILE : 1];
 	constexpr int kStage = (PAY && sizeof(K) < 4) ? 4 : static_cast<int>(sizeof(K));
 	__shared__ __align__(16) unsigned char s_raw[TILE * kStage];
@@ -577,13 +668,25 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const Control *ctrl, int pa
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	const uint32_t per = (num_tiles + 7) / 8;
 	const uint32_t x = blockIdx.x & 7u, step = gridDim.x >> 3;
+	uint32_t real_tiles = num_tiles;
+	if constexpr (SEG) {
+		for (int i = t; i <= kRadix; i += THREADS) s_ts[i] = seg.tile_start[i];
+		__syncthreads();
+		real_tiles = s_ts[kRadix];
+	}
+	auto geom = [&](uint32_t tl, uint64_t &base, uint32_t &valid) {
+		if constexpr (SEG) {
+			seg_tile(s_ts, seg, TILE, tl, base, valid);
+		} else {
+			base = static_cast<uint64_t>(tl) * TILE;
+			valid = (n - base) < static_cast<uint64_t>(TILE) ? static_cast<uint32_t>(n - base) : static_cast<uint32_t>(TILE);
+		}
+	};
 	K keys[KPT];
 	uint32_t pay[PAY ? KPT : 1];
 	// loads of a tile are issued before the stores of the previous one, so that waiting for them
 	// (the memory counter is in order and counts stores too) never waits for those stores
-	auto load_tile = [&](uint32_t tl) {
-		const uint64_t base = static_cast<uint64_t>(tl) * TILE;
-		const uint32_t valid = (n - base) < static_cast<uint64_t>(TILE) ? static_cast<uint32_t>(n - base) : static_cast<uint32_t>(TILE);
+	auto load_tile = [&](uint64_t base, uint32_t valid) {
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) {
 			const uint32_t l = w * 64u * KPT + j * 64u + lane;
@@ -600,10 +703,13 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const Control *ctrl, int pa
 	};
 	uint32_t k = blockIdx.x >> 3;
 	uint32_t tile = x * per + k;
-	if (k < per && tile < num_tiles) load_tile(tile);
-	while (k < per && tile < num_tiles) {
-		const uint64_t base = static_cast<uint64_t>(tile) * TILE;
-		const uint32_t valid = (n - base) < static_cast<uint64_t>(TILE) ? static_cast<uint32_t>(n - base) : static_cast<uint32_t>(TILE);
+	uint64_t base = 0;
+	uint32_t valid = 0;
+	if (k < per && tile < real_tiles) {
+		geom(tile, base, valid);
+		load_tile(base, valid);
+	}
+	while (k < per && tile < real_tiles) {
 		uint32_t dig[KPT], pos[KPT], total;
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) {
@@ -626,7 +732,12 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const Control *ctrl, int pa
 		}
 		// prefetch the next tile of this block
 		const uint32_t k2 = k + step, tile2 = x * per + k2;
-		if (k2 < per && tile2 < num_tiles) load_tile(tile2);
+		uint64_t base2 = 0;
+		uint32_t valid2 = 0;
+		if (k2 < per && tile2 < real_tiles) {
+			geom(tile2, base2, valid2);
+			load_tile(base2, valid2);
+		}
 		__syncthreads();
 		for (uint32_t i = t; i < valid; i += THREADS) {
 			const K kk = s_keys[i];
@@ -645,6 +756,266 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const Control *ctrl, int pa
 		__syncthreads();
 		k = k2;
 		tile = tile2;
+		base = base2;
+		valid = valid2;
+	}
+}
+
+// ---------------------------------------------------------------------------------------------
+// two-level MSD plan: segmented count and scan of the second digit, fallback switch, and the
+// per-bucket LDS sort of the remaining digits
+
+// per-tile counts of the second digit over the tiles of the segment map
+template <typename K, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_hyb_count(const PassPlan *ppp, uint32_t shift, KeyXform xf, uint32_t tile_keys,
+	SegMap seg, uint32_t *__restrict__ counts)
+{
+	const PassPlan pp = *ppp;
+	if (!pp.active) return;
+	const K *__restrict__ src = static_cast<const K *>(pp.src);
+	__shared__ uint32_t s_ts[kRadix + 1];
+	__shared__ uint32_t s_t[kRadix];
+	for (int i = threadIdx.x; i <= kRadix; i += THREADS) s_ts[i] = seg.tile_start[i];
+	__syncthreads();
+	const uint32_t num = s_ts[kRadix];
+	for (uint32_t tile = blockIdx.x; tile < num; tile += gridDim.x) {
+		for (int i = threadIdx.x; i < kRadix; i += THREADS) s_t[i] = 0;
+		__syncthreads();
+		uint64_t base;
+		uint32_t len;
+		seg_tile(s_ts, seg, tile_keys, tile, base, len);
+		for_each_key<K, THREADS>(src, base, len, [&](K x) { wave_count(s_t, digit_of<K>(x, xf, shift)); });
+		__syncthreads();
+		for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kRadix + i] = s_t[i];
+		__syncthreads();
+	}
+}
+
+// One block per top-digit segment, one thread per second digit: the bucket totals, their
+// exclusive offsets inside the segment, the bucket table, and the tile counts turned into global
+// output offsets. A bucket larger than cap marks the plan for the LSD fallback.
+__global__ __launch_bounds__(256) void k_hyb_segscan(Control *ctrl, SegTables T, uint32_t *__restrict__ counts, uint32_t cap)
+{
+	if (!ctrl->hyb[1].active) return;
+	__shared__ uint32_t s_wave[4];
+	const uint32_t sg = blockIdx.x, d = threadIdx.x, lane = d & 63u, w = d >> 6;
+	const uint32_t t0 = T.tile_start[sg], t1 = T.tile_start[sg + 1];
+	uint32_t tot = 0;
+	for (uint32_t tl = t0; tl < t1; tl += 8) {
+		uint32_t c[8];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) c[j] = (tl + j < t1) ? counts[static_cast<uint64_t>(tl + j) * kRadix + d] : 0u;
+#pragma unroll
+		for (int j = 0; j < 8; ++j) tot += c[j];
+	}
+	const uint32_t incl = wave_inclusive_sum(tot);
+	if (lane == 63) s_wave[w] = incl;
+	__syncthreads();
+	uint32_t before = 0;
+	for (uint32_t v = 0; v < w; ++v) before += s_wave[v];
+	uint32_t run = T.seg_start[sg] + before + incl - tot;
+	T.bucket_start[sg * kRadix + d] = run;
+	T.bucket_len[sg * kRadix + d] = tot;
+	if (tot > cap) atomicOr(&ctrl->hyb_over, 1u);
+	for (uint32_t tl = t0; tl < t1; tl += 8) {
+		uint32_t c[8];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) c[j] = (tl + j < t1) ? counts[static_cast<uint64_t>(tl + j) * kRadix + d] : 0u;
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			if (tl + j < t1) counts[static_cast<uint64_t>(tl + j) * kRadix + d] = run;
+			run += c[j];
+		}
+	}
+}
+
+// A bucket too large for the LDS sort: put the LSD plan back (the input is still intact: only the
+// second pass's scatter and the bucket sort would overwrite arrays the LSD plan reads).
+__global__ void k_hyb_check(Control *ctrl)
+{
+	if (threadIdx.x != 0 || !ctrl->hyb[1].active || !ctrl->hyb_over) return;
+	for (int p = 0; p < kMaxPasses; ++p) ctrl->pass[p] = ctrl->lsd_pass[p];
+	ctrl->pre = ctrl->lsd_pre;
+	ctrl->fin = ctrl->lsd_fin;
+	ctrl->counts_ready = ctrl->lsd_counts_ready;
+	ctrl->active_mask = ctrl->lsd_active_mask;
+	ctrl->hyb[1].active = 0;
+	ctrl->hyb[2].active = 0;
+}
+
+// One block per bucket (all keys share the top two digits, so the transformed key shifted left by
+// 16 keeps every digit below them). Each key becomes the word (transformed key << 16) | l, with l
+// its position in the bucket: distinct words whose order is the stable order. The words are
+// partitioned by their top PB bits with LDS atomics (in any order), then every word's final
+// position inside its run of equal top bits is the number of smaller words in the run. The raw
+// keys reach their positions through their owners. A bucket with a run longer than kLocalRunMax is
+// queued for k_local_lsd instead. The bucket is read whole before it is written, so source and
+// destination may coincide.
+constexpr uint32_t kLocalRunMax = 96;
+
+template <typename K, int THREADS, int KPT, int PB>
+__global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, const uint32_t *__restrict__ bstart,
+	const uint32_t *__restrict__ blen, uint32_t *__restrict__ queue)
+{
+	constexpr int CAP = THREADS * KPT;
+	constexpr int P = sizeof(K);
+	constexpr uint32_t NB = 1u << PB;            // partition buckets
+	constexpr uint32_t PER = NB / THREADS;       // partition buckets per thread in the scan
+	constexpr uint32_t kTop = 8u * P - PB;       // the partition bits' place in a packed word
+	static_assert(P >= 4, "the two-level MSD plan sorts 4- and 8-byte keys");
+	static_assert(CAP <= 65536, "positions are packed into 16 bits");
+	static_assert(NB % THREADS == 0 && PB <= 16, "partition buckets per thread");
+	using W = typename KeyTraits<K>::W;
+	static_assert(sizeof(W) == sizeof(K), "transformed keys share the key width");
+	constexpr uint32_t kUnionWords = (NB > CAP / 2) ? NB : CAP / 2;
+	__shared__ __align__(16) W s_t[CAP];          // packed words, then the raw keys in final order
+	__shared__ uint32_t s_u[kUnionWords];         // partition counters/cursors, then final positions (u16)
+	__shared__ uint32_t s_off[NB + 1];
+	__shared__ uint32_t s_wsum[THREADS / 64];
+	__shared__ uint32_t s_big;
+	const PassPlan pp = ctrl->hyb[2];
+	if (!pp.active) return;
+	const uint32_t len = blen[blockIdx.x];
+	if (len == 0) return;
+	const uint32_t start = bstart[blockIdx.x];
+	const K *__restrict__ src = static_cast<const K *>(pp.src) + start;
+	K *__restrict__ dst = static_cast<K *>(pp.dst) + start;
+	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+	if (len == 1) {
+		if (t == 0 && pp.src != pp.dst) dst[0] = src[0];
+		return;
+	}
+	K key[KPT];
+#pragma unroll
+	for (int j = 0; j < KPT; ++j) {
+		const uint32_t l = w * 64u * KPT + j * 64u + lane;
+		key[j] = l < len ? src[l] : K(0);
+	}
+	uint32_t *s_cur = s_u;
+	for (uint32_t i = t; i < NB; i += THREADS) s_cur[i] = 0;
+	if (t == 0) s_big = 0;
+	__syncthreads();
+#pragma unroll
+	for (int j = 0; j < KPT; ++j) {
+		const uint32_t l = w * 64u * KPT + j * 64u + lane;
+		if (l < len) atomicAdd(&s_cur[static_cast<uint32_t>(static_cast<W>(sort_key<K>(key[j], xf) << 16) >> kTop)], 1u);
+	}
+	__syncthreads();
+	// exclusive scan of the counters, PER consecutive ones per thread
+	uint32_t c[PER], sum = 0, big = 0;
+#pragma unroll
+	for (uint32_t k = 0; k < PER; ++k) {
+		c[k] = s_cur[t * PER + k];
+		sum += c[k];
+		big |= c[k] > kLocalRunMax;
+	}
+	const uint32_t incl = wave_inclusive_sum(sum);
+	if (lane == 63) s_wsum[w] = incl;
+	if (big) s_big = 1;
+	__syncthreads();
+	uint32_t run = incl - sum;
+	for (uint32_t v = 0; v < w; ++v) run += s_wsum[v];
+#pragma unroll
+	for (uint32_t k = 0; k < PER; ++k) {
+		s_off[t * PER + k] = run;
+		s_cur[t * PER + k] = run;
+		run += c[k];
+	}
+	if (t == THREADS - 1) s_off[NB] = run;
+	__syncthreads();
+	if (s_big) {
+		if (t == 0) queue[atomicAdd(&ctrl->hyb_queue, 1u)] = blockIdx.x;
+		return;
+	}
+#pragma unroll
+	for (int j = 0; j < KPT; ++j) {
+		const uint32_t l = w * 64u * KPT + j * 64u + lane;
+		if (l < len) {
+			const W word = static_cast<W>(sort_key<K>(key[j], xf) << 16) | static_cast<W>(l);
+			s_t[atomicAdd(&s_cur[static_cast<uint32_t>(word >> kTop)], 1u)] = word;
+		}
+	}
+	__syncthreads();
+	uint16_t *s_fin = reinterpret_cast<uint16_t *>(s_u);
+	uint32_t fin[KPT];
+	uint32_t at[KPT];
+#pragma unroll
+	for (int j = 0; j < KPT; ++j) {
+		const uint32_t i = j * THREADS + t;
+		at[j] = 0xFFFFFFFFu;
+		if (i < len) {
+			const W x = s_t[i];
+			const uint32_t d = static_cast<uint32_t>(x >> kTop);
+			const uint32_t lo = s_off[d], hi = s_off[d + 1];
+			uint32_t n_less = 0;
+			for (uint32_t q = lo; q < hi; ++q) n_less += static_cast<uint32_t>(s_t[q] < x);
+			fin[j] = lo + n_less;
+			at[j] = static_cast<uint32_t>(x) & 0xFFFFu;
+		}
+	}
+	__syncthreads();  // the cursors are dead: s_u now holds the final positions
+#pragma unroll
+	for (int j = 0; j < KPT; ++j)
+		if (at[j] != 0xFFFFFFFFu) s_fin[at[j]] = static_cast<uint16_t>(fin[j]);
+	__syncthreads();
+	K *s_raw = reinterpret_cast<K *>(s_t);
+#pragma unroll
+	for (int j = 0; j < KPT; ++j) {
+		const uint32_t l = w * 64u * KPT + j * 64u + lane;
+		if (l < len) s_raw[s_fin[l]] = key[j];
+	}
+	__syncthreads();
+	for (uint32_t i = t; i < len; i += THREADS) dst[i] = s_raw[i];
+}
+
+// The queued buckets (a run of more than kLocalRunMax keys share the partition bits): stable LDS LSD
+// passes over the digits below the top two, from the bucket's original order.
+template <typename K, int THREADS, int KPT>
+__global__ __launch_bounds__(THREADS) void k_local_lsd(const Control *ctrl, KeyXform xf, const uint32_t *__restrict__ bstart,
+	const uint32_t *__restrict__ blen, const uint32_t *__restrict__ queue)
+{
+	constexpr int CAP = THREADS * KPT;
+	constexpr int NW = THREADS / 64;
+	constexpr int P = sizeof(K);
+	__shared__ uint32_t s_whist[NW * kRadix];
+	__shared__ uint32_t s_digoff[kRadix];
+	__shared__ uint32_t s_wsum[4];
+	__shared__ __align__(16) K s_raw[CAP];
+	const PassPlan pp = ctrl->hyb[2];
+	if (!pp.active) return;
+	const uint32_t queued = ctrl->hyb_queue;
+	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+	for (uint32_t qi = blockIdx.x; qi < queued; qi += gridDim.x) {
+		const uint32_t b = queue[qi];
+		const uint32_t len = blen[b], start = bstart[b];
+		const K *__restrict__ src = static_cast<const K *>(pp.src) + start;
+		K *__restrict__ dst = static_cast<K *>(pp.dst) + start;
+		K key[KPT];
+#pragma unroll
+		for (int j = 0; j < KPT; ++j) {
+			const uint32_t l = w * 64u * KPT + j * 64u + lane;
+			key[j] = l < len ? src[l] : K(0);
+		}
+		uint32_t dig[KPT], pos[KPT], total;
+		for (int p = 0; p <= P - 3; ++p) {
+#pragma unroll
+			for (int j = 0; j < KPT; ++j) {
+				const uint32_t l = w * 64u * KPT + j * 64u + lane;
+				dig[j] = l < len ? digit_of<K>(key[j], xf, 8u * p) : 0xFFu;
+			}
+			block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total);
+#pragma unroll
+			for (int j = 0; j < KPT; ++j) s_raw[pos[j]] = key[j];
+			__syncthreads();
+#pragma unroll
+			for (int j = 0; j < KPT; ++j) key[j] = s_raw[w * 64u * KPT + j * 64u + lane];
+			__syncthreads();
+		}
+#pragma unroll
+		for (int j = 0; j < KPT; ++j) {
+			const uint32_t l = w * 64u * KPT + j * 64u + lane;
+			if (l < len) dst[l] = key[j];
+		}
 	}
 }
 
@@ -680,6 +1051,9 @@ __global__ void k_finish(Control *ctrl)
 	if (threadIdx.x == 0) {
 		ctrl->epoch = ctrl->epoch + 1u;
 		ctrl->check = 0;
+		ctrl->hyb_scan_counter = 0;
+		ctrl->hyb_over = 0;
+		ctrl->hyb_queue = 0;
 	}
 }
 
@@ -772,20 +1146,21 @@ hipError_t launch_pass_cfg(int cfg, Control *ctrl, const uint32_t *bucket, uint6
 	}
 }
 
-template <typename K, bool PAY, int THREADS, int KPT>
-hipError_t launch_scatter(Control *ctrl, int pass, const KeyXform &xf, uint64_t n, uint32_t tiles, const uint32_t *offsets,
-	hipStream_t stream)
+template <typename K, bool PAY, int THREADS, int KPT, bool SEG = false>
+hipError_t launch_scatter(const PassPlan *pp, uint32_t shift, const KeyXform &xf, uint64_t n, uint32_t tiles,
+	const uint32_t *offsets, SegMap seg, hipStream_t stream)
 {
 	static int occ = -1;
 	if (occ < 0) {
 		int o = 0;
-		if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_scatter<K, PAY, THREADS, KPT>, THREADS, 0) != hipSuccess || o < 1) o = 1;
+		if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_scatter<K, PAY, THREADS, KPT, SEG>, THREADS, 0) != hipSuccess || o < 1) o = 1;
 		occ = o;
 	}
 	const uint32_t per = (tiles + 7) / 8;
 	const uint32_t want = static_cast<uint32_t>(occ * devinfo().cus * 2 + 7) / 8;
 	const uint32_t grid = 8u * std::max<uint32_t>(1u, std::min<uint32_t>(per, want));
-	hipLaunchKernelGGL((k_scatter<K, PAY, THREADS, KPT>), dim3(grid), dim3(THREADS), 0, stream, ctrl, pass, xf, n, tiles, offsets);
+	hipLaunchKernelGGL((k_scatter<K, PAY, THREADS, KPT, SEG>), dim3(grid), dim3(THREADS), 0, stream, pp, shift, xf, n, tiles,
+		offsets, seg);
 	return hipGetLastError();
 }
 
@@ -793,12 +1168,115 @@ template <typename K, bool PAY>
 hipError_t launch_scatter_cfg(int cfg, Control *ctrl, int pass, const KeyXform &xf, uint64_t n, uint32_t tiles,
 	const uint32_t *offsets, hipStream_t stream)
 {
+	const PassPlan *pp = &ctrl->pass[pass];
+	const uint32_t sh = 8u * static_cast<uint32_t>(pass);
+	const SegMap none{};
 	switch (cfg) {
-	case 0: return launch_scatter<K, PAY, 256, 8>(ctrl, pass, xf, n, tiles, offsets, stream);
-	case 1: return launch_scatter<K, PAY, 256, 16>(ctrl, pass, xf, n, tiles, offsets, stream);
-	case 2: return launch_scatter<K, PAY, 512, 8>(ctrl, pass, xf, n, tiles, offsets, stream);
-	case 3: return launch_scatter<K, PAY, 512, 16>(ctrl, pass, xf, n, tiles, offsets, stream);
-	default: return launch_scatter<K, PAY, 256, 24>(ctrl, pass, xf, n, tiles, offsets, stream);
+	case 0: return launch_scatter<K, PAY, 256, 8>(pp, sh, xf, n, tiles, offsets, none, stream);
+	case 1: return launch_scatter<K, PAY, 256, 16>(pp, sh, xf, n, tiles, offsets, none, stream);
+	case 2: return launch_scatter<K, PAY, 512, 8>(pp, sh, xf, n, tiles, offsets, none, stream);
+	case 3: return launch_scatter<K, PAY, 512, 16>(pp, sh, xf, n, tiles, offsets, none, stream);
+	default: return launch_scatter<K, PAY, 256, 24>(pp, sh, xf, n, tiles, offsets, none, stream);
+	}
+}
+
+// Capacities (keys) of the per-bucket LDS sort: 256x4, 256x8, 256x12, 256x19, 512x16.
+constexpr int kNumLocalCfgs = 5;
+constexpr uint32_t kLocalCaps[kNumLocalCfgs] = {1024, 2048, 3072, 4864, 8192};
+
+// the smallest capacity that holds the largest of 65536 buckets of uniformly spread keys
+// (mean + 6 standard deviations + slack); -1 when none does
+int local_cfg_for(uint64_t n)
+{
+	const double mean = static_cast<double>(n) / kHybBuckets;
+	const double need = mean + 6.0 * std::sqrt(mean) + 64.0;
+	for (int c = 0; c < kNumLocalCfgs; ++c)
+		if (need <= kLocalCaps[c]) return c;
+	return -1;
+}
+
+template <typename K, int THREADS, int KPT, int PB>
+hipError_t launch_local_cfg(Control *ctrl, const KeyXform &xf, const SegTables &T, hipStream_t stream)
+{
+	hipLaunchKernelGGL((k_local<K, THREADS, KPT, PB>), dim3(kHybBuckets), dim3(THREADS), 0, stream, ctrl, xf, T.bucket_start,
+		T.bucket_len, T.queue);
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess) return e;
+	const uint32_t grid = static_cast<uint32_t>(4 * devinfo().cus);
+	hipLaunchKernelGGL((k_local_lsd<K, THREADS, KPT>), dim3(grid), dim3(THREADS), 0, stream, ctrl, xf, T.bucket_start,
+		T.bucket_len, T.queue);
+	return hipGetLastError();
+}
+
+// capacity and partition width per configuration (kLocalCaps): about four keys per partition run
+// at the mean bucket size each capacity is chosen for
+template <typename K>
+hipError_t launch_local(int lcfg, Control *ctrl, const KeyXform &xf, const SegTables &T, hipStream_t stream)
+{
+	switch (lcfg) {
+	case 0: return launch_local_cfg<K, 256, 4, 8>(ctrl, xf, T, stream);
+	case 1: return launch_local_cfg<K, 256, 8, 9>(ctrl, xf, T, stream);
+	case 2: return launch_local_cfg<K, 256, 12, 9>(ctrl, xf, T, stream);
+	case 3: return launch_local_cfg<K, 256, 19, 10>(ctrl, xf, T, stream);
+	default: return launch_local_cfg<K, 512, 16, 11>(ctrl, xf, T, stream);
+	}
+}
+
+// The two-level MSD plan's launches after k_plan (every one exits at once when the plan did not
+// choose it; k_hyb_check switches the rest off when a bucket is too large).
+template <typename K, int THREADS, int KPT>
+hipError_t launch_hybrid(Control *ctrl, const uint32_t *bucket, uint32_t *counts, uint64_t *lookback, const SegTables &T,
+	const KeyXform &xf, uint64_t n, uint32_t tiles, int lcfg, void *ws, hipStream_t stream)
+{
+	constexpr int P = sizeof(K);
+	constexpr uint32_t TILE = THREADS * KPT;
+	const int cus = devinfo().cus;
+	hipError_t e;
+	{
+		ProfScope ps(7, 0, ws, stream);
+		const uint32_t nchunks = (tiles + kScanChunk - 1) / kScanChunk;
+		const uint32_t sgrid = std::max<uint32_t>(1u, std::min<uint32_t>(nchunks, 4u * cus));
+		hipLaunchKernelGGL(k_scan, dim3(sgrid), dim3(256), 0, stream, ctrl, &ctrl->hyb[0], &ctrl->hyb_scan_counter,
+			bucket + (P - 1) * kRadix, kMaxPasses, counts, lookback, tiles);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+	}
+	{
+		ProfScope ps(5, 0, ws, stream);
+		e = launch_scatter<K, false, THREADS, KPT>(&ctrl->hyb[0], 8u * (P - 1), xf, n, tiles, counts, SegMap{}, stream);
+		if (e != hipSuccess) return e;
+	}
+	const uint32_t tiles2 = tiles + kRadix;  // bound: one partial tile per segment
+	const SegMap m{T.tile_start, T.seg_start, T.seg_len};
+	{
+		ProfScope ps(7, 1, ws, stream);
+		const uint32_t cgrid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles2, 8u * cus));
+		hipLaunchKernelGGL((k_hyb_count<K, 256>), dim3(cgrid), dim3(256), 0, stream, &ctrl->hyb[1], 8u * (P - 2), xf, TILE, m,
+			counts);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+		hipLaunchKernelGGL(k_hyb_segscan, dim3(kRadix), dim3(256), 0, stream, ctrl, T, counts, kLocalCaps[lcfg]);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+		hipLaunchKernelGGL(k_hyb_check, dim3(1), dim3(64), 0, stream, ctrl);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+	}
+	{
+		ProfScope ps(5, 1, ws, stream);
+		e = launch_scatter<K, false, THREADS, KPT, true>(&ctrl->hyb[1], 8u * (P - 2), xf, n, tiles2, counts, m, stream);
+		if (e != hipSuccess) return e;
+	}
+	ProfScope ps(6, 0, ws, stream);
+	return launch_local<K>(lcfg, ctrl, xf, T, stream);
+}
+
+template <typename K>
+hipError_t launch_hybrid_cfg(int cfg, Control *ctrl, const uint32_t *bucket, uint32_t *counts, uint64_t *lookback,
+	const SegTables &T, const KeyXform &xf, uint64_t n, uint32_t tiles, int lcfg, void *ws, hipStream_t stream)
+{
+	switch (cfg) {
+	case 0: return launch_hybrid<K, 256, 8>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
+	case 1: return launch_hybrid<K, 256, 16>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
+	case 2: return launch_hybrid<K, 512, 8>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
+	case 3: return launch_hybrid<K, 512, 16>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
+	default: return launch_hybrid<K, 256, 24>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
 	}
 }
 
@@ -816,7 +1294,8 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 
 	constexpr int HITEMS = 16 / sizeof(K) < 4 ? 4 : 16 / sizeof(K) * 2;
 	const int cfg = job.cfg >= 0 ? job.cfg : default_cfg(P, n, job.payload);
-	int algo = job.algo ? job.algo : default_algo(P, n);
+	const bool keys_only = !job.payload && !job.keys_final_any;
+	int algo = job.algo ? job.algo : default_algo(P, n, keys_only);
 	if (algo == kAlgoLds) {
 		if (!job.payload && n <= kLdsMax && !job.keys_final_any) {
 			// one block holds the whole array: every load happens before any store, so in place is fine
@@ -831,14 +1310,24 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 				hipLaunchKernelGGL((k_lds_sort<K, 512, 16>), dim3(1), dim3(512), 0, stream, static_cast<const K *>(job.in), static_cast<K *>(job.out), nn, xf, rev);
 			return hipGetLastError();
 		}
-		algo = default_algo(P, n);
+		algo = default_algo(P, n, false);
 	}
+	int lcfg = -1;
+	if (algo == kAlgoHybrid) {
+		lcfg = local_cfg_for(n);
+		if (P < 4 || !keys_only || !job.buf || lcfg < 0) lcfg = -1;
+		if (lcfg < 0) algo = kAlgoCountScan;
+	}
+	const bool hybrid = lcfg >= 0;
 	const uint32_t tile_keys = static_cast<uint32_t>(kTileCfgs[cfg].threads * kTileCfgs[cfg].kpt);
 	const uint32_t tiles = static_cast<uint32_t>((n + tile_keys - 1) / tile_keys);
 	uint32_t *counts = counts_of(ws, n);
 	const bool fill = (P == 1) && !job.payload && job.fill8 &&
 		(job.flags & 7u) != (kAbs | kSign) && (job.flags & 7u) != (kSign | kFltp) && (job.flags & 7u) != (kAbs | kSign | kFltp);
+	// the histogram read also counts, per tile, the digit of the first pass: digit 0 for the LSD
+	// count + scan plan, the top digit for the MSD plan
 	const bool counts0 = algo == kAlgoCountScan && !fill;
+	const bool count_tiles = counts0 || hybrid;
 	if (job.presort_check && !fill) {
 		const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 4095) / 4096, 4u * cus)));
 		hipLaunchKernelGGL((k_check<K, 256>), dim3(grid), dim3(256), 0, stream, static_cast<const K *>(job.in), n, xf, ctrl);
@@ -846,10 +1335,10 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	}
 	{
 		ProfScope ps(0, 0, ws, stream);
-		if (counts0) {
+		if (count_tiles) {
 			const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, 8u * cus));
 			hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS>), dim3(grid), dim3(256), 0, stream, static_cast<const K *>(job.in), n,
-				xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0, ctrl);
+				xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0, ctrl, hybrid ? 8u * (P - 1) : 0u);
 		} else {
 			const uint32_t hgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 256 * HITEMS - 1) / (256 * HITEMS), 8u * cus)));
 			hipLaunchKernelGGL((k_hist<K, 256, HITEMS>), dim3(hgrid), dim3(256), 0, stream, static_cast<const K *>(job.in), n, xf, hist,
@@ -867,9 +1356,22 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	pa.reverse = (job.flags & kRev) != 0;
 	pa.keys_final_any = job.keys_final_any;
 	pa.counts0 = counts0 ? 1 : 0;
+	pa.hybrid = hybrid ? 1 : 0;
+	pa.hyb_cap = hybrid ? kLocalCaps[lcfg] : 0;
+	pa.tile2 = tile_keys;
+	pa.segtab = segtab_of(ws, n);
 
 	hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, stream, pa, ctrl, hist, bucket);
 	if ((e = hipGetLastError()) != hipSuccess) return e;
+	if constexpr (P >= 4) {
+		if (hybrid) {
+			// the LSD count + scan launches below stay as the fallback (inactive unless needed)
+			e = launch_hybrid_cfg<K>(cfg, ctrl, bucket, counts, lookback, seg_tables(segtab_of(ws, n)), xf, n, tiles, lcfg, ws,
+				stream);
+			if (e != hipSuccess) return e;
+			algo = kAlgoCountScan;
+		}
+	}
 
 	const uint32_t cgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 8u * cus)));
 	if (fill) {
@@ -894,7 +1396,8 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 				if ((e = hipGetLastError()) != hipSuccess) return e;
 				const uint32_t nchunks = (tiles + kScanChunk - 1) / kScanChunk;
 				const uint32_t sgrid = std::max<uint32_t>(1u, std::min<uint32_t>(nchunks, 4u * cus));
-				hipLaunchKernelGGL(k_scan, dim3(sgrid), dim3(256), 0, stream, ctrl, p, bucket, counts, lookback, tiles);
+				hipLaunchKernelGGL(k_scan, dim3(sgrid), dim3(256), 0, stream, ctrl, &ctrl->pass[p], &ctrl->scan_counter[p],
+					bucket + p * kRadix, p, counts, lookback, tiles);
 				if ((e = hipGetLastError()) != hipSuccess) return e;
 			}
 			ProfScope ps(1, p, ws, stream);
@@ -923,7 +1426,8 @@ void profile_begin()
 
 // Sums the recorded launch times per kind: 0 histogram, 1 scatter launches of executed passes,
 // 2 scatter launches of skipped passes, 3 count+scan launches of executed passes, 4 count+scan
-// launches of skipped passes. A pass counts as active by the plan its workspace holds now (the
+// launches of skipped passes, and for the two-level MSD plan (counted only when it ran) 5 its two
+// partition scatters, 6 the per-bucket LDS sort, 7 its count/scan launches. A pass counts as active by the plan its workspace holds now (the
 // plan of the last sort run on that workspace).
 hipError_t profile_end(double *ms, unsigned *launches, int nkinds)
 {
@@ -945,6 +1449,11 @@ hipError_t profile_end(double *ms, unsigned *launches, int nkinds)
 			e = hipMemcpy(&mask, &static_cast<const Control *>(r.ws)->active_mask, sizeof(mask), hipMemcpyDeviceToHost);
 			if (!(mask >> r.pass & 1u)) kind += 1;
 		}
+		if (e == hipSuccess && kind >= 5) {
+			uint32_t on = 0;
+			e = hipMemcpy(&on, &static_cast<const Control *>(r.ws)->hyb[2].active, sizeof(on), hipMemcpyDeviceToHost);
+			if (!on) kind = nkinds;  // the MSD plan was not used: not counted
+		}
 		if (e != hipSuccess) err = e;
 		else if (kind < nkinds) { ms[kind] += t; ++launches[kind]; }
 		(void)hipEventDestroy(r.a);
@@ -954,25 +1463,27 @@ hipError_t profile_end(double *ms, unsigned *launches, int nkinds)
 }
 
 // measured on MI355X (2^20..2^28 uniform keys): the count + scan path wins from about 2^22 keys,
-// the onesweep path below that (fewer launches)
-int default_algo(int key_bytes, uint64_t n)
+// the onesweep path below that (fewer launches); keys-only sorts of 4- and 8-byte keys from 2^26
+// keys use the two-level MSD plan
+int default_algo(int key_bytes, uint64_t n, bool keys_only)
 {
-	(void)key_bytes;
+	if (keys_only && key_bytes >= 4 && n >= (1u << 26) && local_cfg_for(n) >= 0) return kAlgoHybrid;
 	return n >= (1u << 22) ? kAlgoCountScan : kAlgoOnesweep;
 }
 
 int default_cfg(int key_bytes, uint64_t n, bool payload)
 {
-	(void)payload;
 	if (n <= (1u << 16)) return 0;
-	if (default_algo(key_bytes, n) == kAlgoCountScan) return key_bytes >= 8 ? 4 : 1;
+	if (default_algo(key_bytes, n, false) == kAlgoCountScan) return key_bytes >= 8 ? 4 : 1;
+	(void)payload;
 	return key_bytes >= 8 ? 4 : 2;
 }
 
 size_t sort_workspace_bytes(uint64_t n)
 {
-	// look-back words (one per tile and digit) followed by the tile-count matrix
-	return kLookbackOffset + static_cast<size_t>(max_tiles(n)) * kRadix * (sizeof(uint64_t) + sizeof(uint32_t));
+	// look-back words (one per tile and digit), the tile-count matrix, the MSD plan's tables
+	return kLookbackOffset + static_cast<size_t>(max_tiles(n)) * kRadix * sizeof(uint64_t) +
+		static_cast<size_t>(count_rows(n)) * kRadix * sizeof(uint32_t) + kSegTableWords * sizeof(uint32_t);
 }
 
 hipError_t workspace_init(void *ws, size_t bytes, hipStream_t stream)

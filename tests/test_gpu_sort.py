@@ -91,9 +91,11 @@ def combos(nbytes):
     return out + [(KEYTYPES[nbytes][0], m) for m in range(8)]
 
 
-@pytest.fixture(params=[1, 2], ids=["onesweep", "countscan"])
+@pytest.fixture(params=[1, 2, 4], ids=["onesweep", "countscan", "msd2"])
 def algo(request):
-    """Runs a test under each pass algorithm (onesweep look-back, count + chained scan + scatter)."""
+    """Runs a test under each pass algorithm (onesweep look-back, count + chained scan + scatter,
+    two MSD partitions + per-bucket LDS sort, which keys-only 4/8-byte sorts take and the others
+    replace by count + scan)."""
     prev = r.set_algorithm(request.param)
     yield request.param
     r.set_algorithm(prev)
