@@ -44,13 +44,13 @@ struct Control {
 	PassPlan fin;          // copy when every digit is uniform (no pass executes)
 	// two-level MSD plan (keys only, 4- and 8-byte keys): hyb[0] partitions by the top digit,
 	// hyb[1] by the second digit inside each top-digit segment, hyb[2] sorts every resulting
-	// bucket in LDS. The LSD plan is kept in lsd_* and put back when a bucket turns out too large.
+	// bucket in LDS. Its histogram read counts only the top two digits (and finds the uniform
+	// ones among the others from the AND and OR of all keys); when the LSD plan is needed after
+	// all, need_full gates a full histogram read and a second plan.
 	PassPlan hyb[3];
-	PassPlan lsd_pass[kMaxPasses];
-	PassPlan lsd_pre;
-	PassPlan lsd_fin;
-	uint32_t lsd_counts_ready;
-	uint32_t lsd_active_mask;
+	uint64_t key_or;       // OR of every transformed key
+	uint64_t key_nor;      // OR of every complemented transformed key (the complement of their AND)
+	uint32_t need_full;
 	uint32_t hyb_scan_counter;
 	uint32_t hyb_tiles2;   // tiles of the segmented second pass
 	uint32_t hyb_over;     // some bucket exceeds the LDS capacity: use the LSD plan

@@ -235,6 +235,8 @@ struct PlanArgs {
 	uint32_t hyb_cap;   // LDS capacity (keys) of the per-bucket sort
 	uint32_t tile2;     // tile size of the segmented second pass
 	uint32_t *segtab;   // the MSD plan's tables
+	int partial;        // the histogram read counted only the top two digits (see k_hist_tiles)
+	const uint32_t *gate;  // non-null: plan only when it reads nonzero (the full plan after a fallback)
 };
 
 __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_t *hist, uint32_t *bucket)
@@ -243,6 +245,7 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 	__shared__ uint32_t s_uniform[kMaxPasses];
 	__shared__ uint32_t s_max[kMaxPasses];
 	__shared__ uint32_t s_top[kRadix];
+	if (a.gate && !*a.gate) return;
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	for (int p = 0; p < a.passes; ++p) {
 		const uint32_t c = hist[p * kRadix + t];
@@ -263,6 +266,11 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 	if (t != 0) return;
 	PassPlan none{};
 	for (int h = 0; h < 3; ++h) ctrl->hyb[h] = none;
+	if (a.partial) {
+		// digits below the top two: uniform exactly when the AND and the OR of the keys agree on them
+		const uint64_t differ = ctrl->key_or ^ ~ctrl->key_nor;
+		for (int p = 0; p < a.passes - 2; ++p) s_uniform[p] = ((differ >> (8 * p)) & 0xFFu) == 0;
+	}
 	const uint32_t pres = presort_result(ctrl->check, a.reverse);
 	int A = 0;
 	uint32_t mask = 0;
@@ -316,17 +324,26 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 	// estimated from the two histograms, must fit the per-bucket LDS sort with room to spare (the
 	// exact bucket sizes are checked after the second count; larger ones fall back to this LSD plan).
 	const int P = a.passes;
-	if (!a.hybrid || P < 4 || !a.other || a.payload || s_uniform[P - 1] || s_uniform[P - 2]) return;
-	// (the same margin as the host's capacity choice: six standard deviations of a Poisson count)
-	const float est = static_cast<float>(static_cast<double>(s_max[P - 1]) * s_max[P - 2] / static_cast<double>(a.n));
-	if (est + 6.f * sqrtf(est) + 64.f > static_cast<float>(a.hyb_cap)) return;
+	bool msd = a.hybrid && P >= 4 && a.other && !a.payload && !s_uniform[P - 1] && !s_uniform[P - 2];
+	if (msd) {
+		// (the same margin as the host's capacity choice: six standard deviations of a Poisson count)
+		const float est = static_cast<float>(static_cast<double>(s_max[P - 1]) * s_max[P - 2] / static_cast<double>(a.n));
+		msd = est + 6.f * sqrtf(est) + 64.f <= static_cast<float>(a.hyb_cap);
+	}
+	if (!msd && !a.partial) return;
+	// the LSD plan above is either replaced by the MSD plan or, after a partial histogram read,
+	// incomplete (no bucket starts below the top two digits): the gated full read and plan redo it
+	for (int p = 0; p < kMaxPasses; ++p) ctrl->pass[p] = none;
+	ctrl->pre = none;
+	ctrl->fin = none;
+	ctrl->counts_ready = 0;
+	ctrl->active_mask = 0;
+	if (!msd) {
+		ctrl->need_full = 1;
+		return;
+	}
 	void *X1 = a.other != a.src ? a.other : a.fin;
 	void *X2 = X1 == a.other ? a.fin : a.other;
-	for (int p = 0; p < kMaxPasses; ++p) { ctrl->lsd_pass[p] = ctrl->pass[p]; ctrl->pass[p] = none; }
-	ctrl->lsd_pre = ctrl->pre; ctrl->pre = none;
-	ctrl->lsd_fin = ctrl->fin; ctrl->fin = none;
-	ctrl->lsd_counts_ready = ctrl->counts_ready; ctrl->counts_ready = 0;
-	ctrl->lsd_active_mask = ctrl->active_mask; ctrl->active_mask = 0;
 	PassPlan h{};
 	h.active = 1;
 	h.src = a.src; h.dst = X1; h.reverse = a.reverse;
@@ -514,16 +531,24 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(Control *ctrl, const uint3
 // null, the per-tile counts of the digit at count_shift in the logical order of the first pass
 // (tile t holds the logical positions [t*tile_keys, (t+1)*tile_keys), read back to front under
 // isrevorder).
-template <typename K, int THREADS, int ITEMS>
+// PARTIAL (the two-level MSD plan's read): only the top two digits' histograms, plus the AND and
+// OR of all transformed keys (a digit is uniform exactly when they agree on it). A non-null gate
+// that reads zero skips the launch (the full read that only a fallback needs).
+template <typename K, int THREADS, int ITEMS, bool PARTIAL = false>
 __global__ __launch_bounds__(THREADS) void k_hist_tiles(const K *__restrict__ in, uint64_t n, KeyXform xf,
 	uint32_t *__restrict__ hist, uint32_t *__restrict__ counts, uint32_t tile_keys, uint32_t num_tiles, int reverse,
-	const Control *ctrl, uint32_t count_shift)
+	Control *ctrl, uint32_t count_shift, const uint32_t *gate)
 {
+	if (gate && !*gate) return;
 	if (presort_result(ctrl->check, reverse)) return;  // the input is already in order
 	constexpr int P = sizeof(K);
-	__shared__ uint32_t s_h[P * kRadix];
+	constexpr int P0 = PARTIAL ? P - 2 : 0;  // first digit counted
+	using W = typename KeyTraits<K>::W;
+	__shared__ uint32_t s_h[(P - P0) * kRadix];
 	__shared__ uint32_t s_t[kRadix];
-	for (int i = threadIdx.x; i < P * kRadix; i += THREADS) s_h[i] = 0;
+	__shared__ W s_or[THREADS / 64], s_nor[THREADS / 64];
+	W k_or = 0, k_nor = 0;
+	for (int i = threadIdx.x; i < (P - P0) * kRadix; i += THREADS) s_h[i] = 0;
 	for (uint32_t tile = blockIdx.x; tile < num_tiles; tile += gridDim.x) {
 		for (int i = threadIdx.x; i < kRadix; i += THREADS) s_t[i] = 0;
 		__syncthreads();
@@ -534,8 +559,9 @@ __global__ __launch_bounds__(THREADS) void k_hist_tiles(const K *__restrict__ in
 		for_each_key<K, THREADS>(in, lo, len, [&](K x) {
 			const auto k = sort_key<K>(x, xf);
 #pragma unroll
-			for (int p = 0; p < P; ++p) wave_count(s_h + p * kRadix, static_cast<uint32_t>(k >> (8 * p)) & 0xFFu);
+			for (int p = P0; p < P; ++p) wave_count(s_h + (p - P0) * kRadix, static_cast<uint32_t>(k >> (8 * p)) & 0xFFu);
 			if (counts) wave_count(s_t, static_cast<uint32_t>(k >> count_shift) & 0xFFu);
+			if constexpr (PARTIAL) { k_or |= k; k_nor |= ~k; }
 		});
 		__syncthreads();
 		if (counts)
@@ -543,8 +569,24 @@ __global__ __launch_bounds__(THREADS) void k_hist_tiles(const K *__restrict__ in
 		__syncthreads();
 	}
 	__syncthreads();
-	for (int i = threadIdx.x; i < P * kRadix; i += THREADS)
-		if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+	for (int i = threadIdx.x; i < (P - P0) * kRadix; i += THREADS)
+		if (s_h[i]) atomicAdd(&hist[P0 * kRadix + i], s_h[i]);
+	if constexpr (PARTIAL) {
+#pragma unroll
+		for (int o = 32; o > 0; o >>= 1) {
+			k_or |= __shfl_xor(k_or, o, 64);
+			k_nor |= __shfl_xor(k_nor, o, 64);
+		}
+		const uint32_t w = threadIdx.x >> 6;
+		if ((threadIdx.x & 63u) == 0) { s_or[w] = k_or; s_nor[w] = k_nor; }
+		__syncthreads();
+		if (threadIdx.x == 0) {
+			W a = 0, b = 0;
+			for (int v = 0; v < THREADS / 64; ++v) { a |= s_or[v]; b |= s_nor[v]; }
+			atomicOr(reinterpret_cast<unsigned long long *>(&ctrl->key_or), static_cast<unsigned long long>(a));
+			atomicOr(reinterpret_cast<unsigned long long *>(&ctrl->key_nor), static_cast<unsigned long long>(b));
+		}
+	}
 }
 
 // per-tile counts of one digit (passes after the first, or when the histogram read did not count)
@@ -829,18 +871,15 @@ __global__ __launch_bounds__(256) void k_hyb_segscan(Control *ctrl, SegTables T,
 	}
 }
 
-// A bucket too large for the LDS sort: put the LSD plan back (the input is still intact: only the
-// second pass's scatter and the bucket sort would overwrite arrays the LSD plan reads).
+// A bucket too large for the LDS sort: switch the rest of the MSD plan off and request the full
+// histogram read and the LSD plan (the input is still intact: only the second pass's scatter and
+// the bucket sort would overwrite arrays the LSD plan reads).
 __global__ void k_hyb_check(Control *ctrl)
 {
 	if (threadIdx.x != 0 || !ctrl->hyb[1].active || !ctrl->hyb_over) return;
-	for (int p = 0; p < kMaxPasses; ++p) ctrl->pass[p] = ctrl->lsd_pass[p];
-	ctrl->pre = ctrl->lsd_pre;
-	ctrl->fin = ctrl->lsd_fin;
-	ctrl->counts_ready = ctrl->lsd_counts_ready;
-	ctrl->active_mask = ctrl->lsd_active_mask;
 	ctrl->hyb[1].active = 0;
 	ctrl->hyb[2].active = 0;
+	ctrl->need_full = 1;
 }
 
 // One block per bucket (all keys share the top two digits, so the transformed key shifted left by
@@ -870,7 +909,7 @@ __global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, c
 	constexpr uint32_t kUnionWords = (NB > CAP / 2) ? NB : CAP / 2;
 	__shared__ __align__(16) W s_t[CAP];          // packed words, then the raw keys in final order
 	__shared__ uint32_t s_u[kUnionWords];         // partition counters/cursors, then final positions (u16)
-	__shared__ uint32_t s_off[NB + 1];
+	__shared__ uint16_t s_off[NB + 1];            // run starts (positions < CAP <= 65536)
 	__shared__ uint32_t s_wsum[THREADS / 64];
 	__shared__ uint32_t s_big;
 	const PassPlan pp = ctrl->hyb[2];
@@ -917,11 +956,11 @@ __global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, c
 	for (uint32_t v = 0; v < w; ++v) run += s_wsum[v];
 #pragma unroll
 	for (uint32_t k = 0; k < PER; ++k) {
-		s_off[t * PER + k] = run;
+		s_off[t * PER + k] = static_cast<uint16_t>(run);
 		s_cur[t * PER + k] = run;
 		run += c[k];
 	}
-	if (t == THREADS - 1) s_off[NB] = run;
+	if (t == THREADS - 1) s_off[NB] = static_cast<uint16_t>(run);
 	__syncthreads();
 	if (s_big) {
 		if (t == 0) queue[atomicAdd(&ctrl->hyb_queue, 1u)] = blockIdx.x;
@@ -947,7 +986,7 @@ __global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, c
 			const W x = s_t[i];
 			const uint32_t d = static_cast<uint32_t>(x >> kTop);
 			const uint32_t lo = s_off[d], hi = s_off[d + 1];
-			uint32_t n_less = 0;
+			uint32_t n_less = 0;  // the word itself is in the run and not smaller
 			for (uint32_t q = lo; q < hi; ++q) n_less += static_cast<uint32_t>(s_t[q] < x);
 			fin[j] = lo + n_less;
 			at[j] = static_cast<uint32_t>(x) & 0xFFFFu;
@@ -1054,6 +1093,9 @@ __global__ void k_finish(Control *ctrl)
 		ctrl->hyb_scan_counter = 0;
 		ctrl->hyb_over = 0;
 		ctrl->hyb_queue = 0;
+		ctrl->need_full = 0;
+		ctrl->key_or = 0;
+		ctrl->key_nor = 0;
 	}
 }
 
@@ -1208,16 +1250,16 @@ hipError_t launch_local_cfg(Control *ctrl, const KeyXform &xf, const SegTables &
 	return hipGetLastError();
 }
 
-// capacity and partition width per configuration (kLocalCaps): about four keys per partition run
-// at the mean bucket size each capacity is chosen for
+// capacity and partition width per configuration (kLocalCaps): one to three keys per partition
+// run at the mean bucket size each capacity is chosen for
 template <typename K>
 hipError_t launch_local(int lcfg, Control *ctrl, const KeyXform &xf, const SegTables &T, hipStream_t stream)
 {
 	switch (lcfg) {
-	case 0: return launch_local_cfg<K, 256, 4, 8>(ctrl, xf, T, stream);
-	case 1: return launch_local_cfg<K, 256, 8, 9>(ctrl, xf, T, stream);
-	case 2: return launch_local_cfg<K, 256, 12, 9>(ctrl, xf, T, stream);
-	case 3: return launch_local_cfg<K, 256, 19, 10>(ctrl, xf, T, stream);
+	case 0: return launch_local_cfg<K, 256, 4, 9>(ctrl, xf, T, stream);
+	case 1: return launch_local_cfg<K, 256, 8, 10>(ctrl, xf, T, stream);
+	case 2: return launch_local_cfg<K, 256, 12, 10>(ctrl, xf, T, stream);
+	case 3: return launch_local_cfg<K, 256, 19, 11>(ctrl, xf, T, stream);
 	default: return launch_local_cfg<K, 512, 16, 11>(ctrl, xf, T, stream);
 	}
 }
@@ -1335,10 +1377,14 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	}
 	{
 		ProfScope ps(0, 0, ws, stream);
-		if (count_tiles) {
-			const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, 8u * cus));
+		const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, 8u * cus));
+		if (hybrid) {
+			if constexpr (P >= 4)
+				hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS, true>), dim3(grid), dim3(256), 0, stream, static_cast<const K *>(job.in),
+					n, xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0, ctrl, 8u * (P - 1), nullptr);
+		} else if (count_tiles) {
 			hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS>), dim3(grid), dim3(256), 0, stream, static_cast<const K *>(job.in), n,
-				xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0, ctrl, hybrid ? 8u * (P - 1) : 0u);
+				xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0, ctrl, 0u, nullptr);
 		} else {
 			const uint32_t hgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 256 * HITEMS - 1) / (256 * HITEMS), 8u * cus)));
 			hipLaunchKernelGGL((k_hist<K, 256, HITEMS>), dim3(hgrid), dim3(256), 0, stream, static_cast<const K *>(job.in), n, xf, hist,
@@ -1360,16 +1406,29 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	pa.hyb_cap = hybrid ? kLocalCaps[lcfg] : 0;
 	pa.tile2 = tile_keys;
 	pa.segtab = segtab_of(ws, n);
+	pa.partial = hybrid ? 1 : 0;
+	pa.gate = nullptr;
 
 	hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, stream, pa, ctrl, hist, bucket);
 	if ((e = hipGetLastError()) != hipSuccess) return e;
 	if constexpr (P >= 4) {
 		if (hybrid) {
-			// the LSD count + scan launches below stay as the fallback (inactive unless needed)
 			e = launch_hybrid_cfg<K>(cfg, ctrl, bucket, counts, lookback, seg_tables(segtab_of(ws, n)), xf, n, tiles, lcfg, ws,
 				stream);
 			if (e != hipSuccess) return e;
+			// the LSD count + scan plan as the fallback: a full histogram read and a second plan,
+			// both gated on need_full, then the launches below (inactive unless planned)
 			algo = kAlgoCountScan;
+			const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, 8u * cus));
+			hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS>), dim3(grid), dim3(256), 0, stream, static_cast<const K *>(job.in), n,
+				xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0, ctrl, 0u, &ctrl->need_full);
+			if ((e = hipGetLastError()) != hipSuccess) return e;
+			pa.counts0 = 1;
+			pa.hybrid = 0;
+			pa.partial = 0;
+			pa.gate = &ctrl->need_full;
+			hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, stream, pa, ctrl, hist, bucket);
+			if ((e = hipGetLastError()) != hipSuccess) return e;
 		}
 	}
 
