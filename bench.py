@@ -46,6 +46,24 @@ def cpu_baseline(sample_log2: int, budget_s: float):
                       f"reference's 11-bit digit plan sliced over {threads} threads, median of {len(times)}"}
 
 
+def learned_vs_fixed(requests: int, seed: int):
+    """The committed selector (radixsortbidi8_amd/tuning/) against the committed fixed plan table on a
+    seeded request mix of 2^14-2^26 keys (six dtypes, six distributions): wall time of every sort
+    to completion, the learned one including its selection round trip above the size gate."""
+    from radixsortbidi8_amd import planner as pl
+    model = os.path.join(pl.TUNING_DIR, "selector_model.npz")
+    if not os.path.exists(model):
+        return None
+    sel = pl.Selector.load(model)
+    table = pl.load_table()
+    res = pl.evaluate(pl.request_stream(requests, seed, 14, 26), sel, table)
+    return {"requests": res["requests"], "fixed_wall_ms": round(res["fixed_wall_ms"], 3),
+            "learned_wall_ms": round(res["learned_ms"], 3),
+            "speedup": round(res["fixed_wall_ms"] / max(res["learned_ms"], 1e-9), 4),
+            "mismatches": res["mismatches"], "gate_min_count": sel.min_count,
+            "stream": f"planner.request_stream({requests}, {seed:#x}, 14, 26)"}
+
+
 def traffic_from_profile(n_log2: int, family: str):
     """HBM bytes per launch of the dominant kernel measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, scaled by what the same counters read for a plain copy of known size), as recorded
@@ -67,6 +85,7 @@ def main():
     ap.add_argument("--cpu-sample-log2", type=int, default=26)
     ap.add_argument("--cpu-budget-s", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-selector", action="store_true", help="skip the learned-vs-fixed plan comparison")
     ap.add_argument("--tile-config", type=int, default=-1)
     ap.add_argument("--algo", type=int, default=0, help="0 automatic, 1 onesweep, 2 count + scan, 4 MSD plan")
     args = ap.parse_args()
@@ -200,6 +219,11 @@ def main():
         "learned_vs_fixed": None,
         "cpu_baseline": None,
     }
+    if not args.no_selector:
+        print("bench: learned vs fixed plan", file=sys.stderr, flush=True)
+        del keys, out, buf, ws
+        torch.cuda.empty_cache()
+        line["learned_vs_fixed"] = learned_vs_fixed(64, 0x5EED0006)
     if not args.no_cpu_baseline:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample_log2, args.cpu_budget_s)

@@ -157,8 +157,8 @@ int rsbd8_hip_set_algorithm(int algo);
 
 /* A sort plan chosen per call (no reference counterpart; the reference picks its plan from
  * compile-time thresholds, H:12948-13152): algo 0 automatic, 1 onesweep, 2 count + scan + scatter,
- * 3 a single block sorting everything in LDS (keys only, count <= 8192; otherwise automatic);
- * tile_cfg -1 automatic or 0..4 as for rsbd8_hip_set_tile_config; presort_check first tests
+ * 3 a single block sorting everything in LDS (keys only, count <= 8192; otherwise automatic),
+ * 4 the two-level MSD plan (as for rsbd8_hip_set_algorithm); tile_cfg -1 automatic or 0..4 as for rsbd8_hip_set_tile_config; presort_check first tests
  * whether the input is already in order or strictly reversed (then the sort is one copy). */
 typedef struct rsbd8_hip_plan {
 	int algo;
@@ -172,8 +172,9 @@ int rsbd8_hip_radixsortcopynoalloc_plan(int keytype, unsigned flags, size_t coun
 	const void *input, void *output, void *buffer, const rsbd8_hip_plan *plan, rsbd8_hip_stream stream);
 
 /* The candidate plans, indexed in the order of the selector network's outputs (0 <= index <
- * rsbd8_hip_plan_count()): onesweep then count + scan, each over the tile configurations, each
- * without and with the presort check; the last is the single-block LDS sort (count <= 8192). */
+ * rsbd8_hip_plan_count()): onesweep, count + scan, then the two-level MSD plan, each over the tile
+ * configurations, each without and with the presort check; the last is the single-block LDS sort
+ * (count <= 8192). */
 int rsbd8_hip_plan_count(void);
 int rsbd8_hip_plan_at(int index, rsbd8_hip_plan *plan);
 
@@ -183,12 +184,15 @@ int rsbd8_hip_plan_at(int index, rsbd8_hip_plan *plan);
  * rsbd8_hip_plan_count() outputs, hidden <= 64, as trained by the mlp_* calls) and picks the plan with
  * the lowest predicted time among those valid for count. select_plan blocks until the choice is
  * known (it synchronizes the stream; the first call on a thread allocates one page of pinned host
- * memory, kept until the process ends). radixsortcopynoalloc_learned then sorts with the chosen plan
- * exactly as rsbd8_hip_radixsortcopynoalloc_plan does; plan_index (may be NULL) receives the choice. */
+ * memory, kept until the process ends). radixsortcopynoalloc_learned sorts with the chosen plan
+ * exactly as rsbd8_hip_radixsortcopynoalloc_plan does; below min_count keys it skips the selection
+ * (and its round trip) and uses the plan below (NULL: the automatic plan). plan_index (may be NULL)
+ * receives the choice, or -1 when the selection was skipped. */
 int rsbd8_hip_select_plan(int keytype, unsigned flags, size_t count, const void *input, const float *params, int hidden,
 	int *plan_index, rsbd8_hip_stream stream);
 int rsbd8_hip_radixsortcopynoalloc_learned(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
-	const void *input, void *output, void *buffer, const float *params, int hidden, int *plan_index, rsbd8_hip_stream stream);
+	const void *input, void *output, void *buffer, const float *params, int hidden, size_t min_count,
+	const rsbd8_hip_plan *below, int *plan_index, rsbd8_hip_stream stream);
 
 /* The plan selector's inputs and network (no reference counterpart). rsbd8_hip_features writes 24
  * floats describing a key array from a sample of its keys (size, width, flags, per-digit byte

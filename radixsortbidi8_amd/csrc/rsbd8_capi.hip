@@ -162,9 +162,13 @@ struct IndirectWs {
 	uint32_t *perm_a, *perm_b;
 };
 
-IndirectWs carve_indirect(void *ws, size_t count, int kb)
+// The gathered keys and the permutations sit at the end of the workspace: a larger sort's tile
+// records (which grow from the start) never reach where a smaller sort left its arrays, since the
+// workspace holds both the records and the arrays of its largest count.
+IndirectWs carve_indirect(void *ws, size_t ws_bytes, size_t count, int kb)
 {
-	char *p = static_cast<char *>(ws) + align_up(sort_workspace_bytes(count));
+	const size_t arrays = 2 * align_up(count * kb) + 2 * align_up(count * 4);
+	char *p = static_cast<char *>(ws) + ((ws_bytes - arrays) & ~static_cast<size_t>(255));
 	IndirectWs r;
 	r.keys_a = p; p += align_up(count * kb);
 	r.keys_b = p; p += align_up(count * kb);
@@ -346,7 +350,7 @@ int rsbd8_hip_sort_indices(int keytype, unsigned flags, size_t count, void *work
 	if (idx_in == idx_out) return fail(RSBD8_HIP_EINVAL, "arrays alias");
 	const int kb = key_bytes_of(keytype);
 	hipStream_t s = static_cast<hipStream_t>(stream);
-	IndirectWs w = carve_indirect(workspace, count, kb);
+	IndirectWs w = carve_indirect(workspace, workspace_bytes, count, kb);
 	hipError_t e = gather_by_index(kb, count, base, stride, key_offset, idx_in, w.keys_a, s);
 	if (e != hipSuccess) return fail_hip(e, "gather_by_index");
 	SortJob job;
@@ -364,7 +368,7 @@ static int sort_pointers(int keytype, unsigned flags, size_t count, void *worksp
 	const void *const *input, void **output, ptrdiff_t ind1, int second, ptrdiff_t ind2, size_t index2, hipStream_t s)
 {
 	const int kb = key_bytes_of(keytype);
-	IndirectWs w = carve_indirect(workspace, count, kb);
+	IndirectWs w = carve_indirect(workspace, workspace_bytes, count, kb);
 	hipError_t e = gather_by_pointer(kb, count, input, ind1, second, ind2, index2, w.keys_a, s);
 	if (e != hipSuccess) return fail_hip(e, "gather_by_pointer");
 	SortJob job;
@@ -538,14 +542,17 @@ void rsbd8_hip_segmented_plan_destroy(rsbd8_hip_segplan *plan)
 
 namespace {
 
-// The candidate plans in the order of the selector network's outputs: (onesweep, count + scan) x
-// tile configuration x (no presort check, presort check), then the single-block LDS sort.
-constexpr int kNumPlans = 2 * kNumTileCfgs * 2 + 1;
+// The candidate plans in the order of the selector network's outputs: (onesweep, count + scan,
+// two-level MSD) x tile configuration x (no presort check, presort check), then the single-block
+// LDS sort.
+constexpr int kPlanAlgos[3] = {kAlgoOnesweep, kAlgoCountScan, kAlgoHybrid};
+constexpr int kNumPlans = 3 * kNumTileCfgs * 2 + 1;
+static_assert(kNumPlans <= 32, "plans are selected through a 32-bit validity mask");
 
 rsbd8_hip_plan plan_at(int i)
 {
 	if (i == kNumPlans - 1) return rsbd8_hip_plan{kAlgoLds, -1, 0, 0};
-	return rsbd8_hip_plan{1 + i / (2 * kNumTileCfgs), (i / 2) % kNumTileCfgs, i & 1, 0};
+	return rsbd8_hip_plan{kPlanAlgos[i / (2 * kNumTileCfgs)], (i / 2) % kNumTileCfgs, i & 1, 0};
 }
 
 // plans that can run on count keys, as a bit mask over plan indices
@@ -630,13 +637,19 @@ int rsbd8_hip_select_plan(int keytype, unsigned flags, size_t count, const void 
 }
 
 int rsbd8_hip_radixsortcopynoalloc_learned(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
-	const void *input, void *output, void *buffer, const float *params, int hidden, int *plan_index, rsbd8_hip_stream stream)
+	const void *input, void *output, void *buffer, const float *params, int hidden, size_t min_count,
+	const rsbd8_hip_plan *below, int *plan_index, rsbd8_hip_stream stream)
 {
-	int idx = 0;
-	if (count > 1) {
-		const int r = rsbd8_hip_select_plan(keytype, flags, count, input, params, hidden, &idx, stream);
-		if (r) return r;
+	if (count < min_count || count < 2) {
+		// too small for the selection's round trip to pay: the given plan (or the automatic one)
+		if (plan_index) *plan_index = -1;
+		const rsbd8_hip_plan autoplan{0, -1, 0, 0};
+		return sort_with_plan(keytype, flags, count, workspace, workspace_bytes, input, output, buffer,
+			below ? *below : autoplan, static_cast<hipStream_t>(stream), "radixsortcopynoalloc_learned");
 	}
+	int idx = 0;
+	const int r = rsbd8_hip_select_plan(keytype, flags, count, input, params, hidden, &idx, stream);
+	if (r) return r;
 	if (plan_index) *plan_index = idx;
 	return sort_with_plan(keytype, flags, count, workspace, workspace_bytes, input, output, buffer, plan_at(idx),
 		static_cast<hipStream_t>(stream), "radixsortcopynoalloc_learned");

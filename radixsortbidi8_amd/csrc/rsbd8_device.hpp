@@ -180,10 +180,10 @@ namespace rsbd8hip {
 
 // Decoupled look-back for one digit: sums the published counts of the tiles before `tile` back to
 // the nearest inclusive prefix, reading W predecessors per round so that W memory latencies overlap.
-// `words` points at the look-back word of this digit for tile 0 (stride 256 words per tile). Every
+// `words` points at the look-back word of this digit for tile 0 (STRIDE words per tile). Every
 // tile at or after `first` (the first tile of the sorted range) publishes before it looks back, and
 // tile `first` publishes an inclusive prefix, so the walk always ends.
-template <int W>
+template <int W, int64_t STRIDE = 256>
 __device__ __forceinline__ uint32_t lookback_window(const uint64_t *words, int64_t tile, int64_t first, uint32_t tag)
 {
 	uint32_t excl = 0;
@@ -191,7 +191,7 @@ __device__ __forceinline__ uint32_t lookback_window(const uint64_t *words, int64
 	for (;;) {
 		uint64_t v[W];
 #pragma unroll
-		for (int i = 0; i < W; ++i) v[i] = (prev - i >= first) ? atomic_load_agent(words + (prev - i) * 256) : 0ull;
+		for (int i = 0; i < W; ++i) v[i] = (prev - i >= first) ? atomic_load_agent(words + (prev - i) * STRIDE) : 0ull;
 		int used = 0;
 		bool done = false;
 #pragma unroll

@@ -52,14 +52,12 @@ struct Control {
 	uint64_t key_nor;      // OR of every complemented transformed key (the complement of their AND)
 	uint32_t need_full;
 	uint32_t hyb_scan_counter;
-	uint32_t hyb_tiles2;   // tiles of the segmented second pass
 	uint32_t hyb_over;     // some bucket exceeds the LDS capacity: use the LSD plan
 	uint32_t hyb_queue;    // buckets queued for the LDS LSD passes (long runs of one third digit)
 };
 
 constexpr size_t kCtrlBytes = 4096;
 constexpr size_t kHistBytes = kMaxPasses * kRadix * sizeof(uint32_t);
-constexpr size_t kLookbackOffset = kCtrlBytes + 2 * kHistBytes;
 constexpr int kScanChunk = 64;  // tiles per block of the chained scan over the tile counts
 constexpr int kTagSlots = 16;   // look-back tag slots per sort: LSD passes 0..7, the MSD plan's scan 8
 constexpr uint32_t kHybBuckets = kRadix * kRadix;  // buckets after the two MSD partitions
@@ -68,6 +66,16 @@ constexpr uint32_t kHybBuckets = kRadix * kRadix;  // buckets after the two MSD 
 // the queue of buckets for the LDS LSD passes (65536)
 constexpr size_t kSegTableWords = 257 + 256 + 256 + 3 * kHybBuckets;
 static_assert(sizeof(Control) <= kCtrlBytes, "control block too large");
+
+// Workspace layout: [control][histograms][bucket starts][MSD tables][tile records ...]. Tile record
+// t holds the 256 look-back words of tile (or scan chunk) t followed by the 256 digit counts of
+// tile t. Every region sits at the same address for every sort on a workspace, whatever its size,
+// so a look-back word is only ever overwritten by another (epoch-tagged) look-back word: counts,
+// tables or keys left by a sort of another size can never pass for a published one.
+constexpr size_t kHeaderBytes = kCtrlBytes + 2 * kHistBytes + ((kSegTableWords * 4 + 255) / 256) * 256;
+constexpr size_t kRecordBytes = kRadix * (sizeof(uint64_t) + sizeof(uint32_t));
+constexpr size_t kLbStride = kRecordBytes / sizeof(uint64_t);     // u64 words from one tile's look-back row to the next
+constexpr size_t kCountStride = kRecordBytes / sizeof(uint32_t);  // u32 words from one tile's count row to the next
 
 struct SortJob {
 	int key_bytes = 4;             // 1, 2, 4 or 8

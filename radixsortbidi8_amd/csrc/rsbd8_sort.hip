@@ -31,16 +31,18 @@ namespace {
 inline Control *ctrl_of(void *ws) { return reinterpret_cast<Control *>(ws); }
 inline uint32_t *hist_of(void *ws) { return reinterpret_cast<uint32_t *>(static_cast<char *>(ws) + kCtrlBytes); }
 inline uint32_t *bucket_of(void *ws) { return reinterpret_cast<uint32_t *>(static_cast<char *>(ws) + kCtrlBytes + kHistBytes); }
-inline uint64_t *lookback_of(void *ws) { return reinterpret_cast<uint64_t *>(static_cast<char *>(ws) + kLookbackOffset); }
-inline uint64_t max_tiles(uint64_t n) { return (n + kMinTile - 1) / kMinTile + 1; }
-// rows of the tile-count matrix: the tiles of a pass, plus one partial tile per top-digit segment
-// when the tiles of the segmented MSD pass are aligned to the segments
-inline uint64_t count_rows(uint64_t n) { return max_tiles(n) + kRadix; }
-inline uint32_t *counts_of(void *ws, uint64_t n)
+inline uint32_t *segtab_of(void *ws) { return reinterpret_cast<uint32_t *>(static_cast<char *>(ws) + kCtrlBytes + 2 * kHistBytes); }
+// look-back word (tile, digit d) at lookback_of(ws)[tile * kLbStride + d]; count (tile, d) at
+// counts_of(ws)[tile * kCountStride + d] (the tile records of the workspace layout)
+inline uint64_t *lookback_of(void *ws) { return reinterpret_cast<uint64_t *>(static_cast<char *>(ws) + kHeaderBytes); }
+inline uint32_t *counts_of(void *ws)
 {
-	return reinterpret_cast<uint32_t *>(static_cast<char *>(ws) + kLookbackOffset + max_tiles(n) * kRadix * sizeof(uint64_t));
+	return reinterpret_cast<uint32_t *>(static_cast<char *>(ws) + kHeaderBytes + kRadix * sizeof(uint64_t));
 }
-inline uint32_t *segtab_of(void *ws, uint64_t n) { return counts_of(ws, n) + count_rows(n) * kRadix; }
+inline uint64_t max_tiles(uint64_t n) { return (n + kMinTile - 1) / kMinTile + 1; }
+// tile records: the tiles of a pass, plus one partial tile per top-digit segment when the tiles of
+// the segmented MSD pass are aligned to the segments
+inline uint64_t record_count(uint64_t n) { return max_tiles(n) + kRadix; }
 
 __device__ __forceinline__ uint32_t pass_tag(uint32_t epoch, int slot)
 {
@@ -244,13 +246,12 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 	__shared__ uint32_t s_wave[4];
 	__shared__ uint32_t s_uniform[kMaxPasses];
 	__shared__ uint32_t s_max[kMaxPasses];
-	__shared__ uint32_t s_top[kRadix];
 	if (a.gate && !*a.gate) return;
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+	uint32_t top_count = 0, top_start = 0;
 	for (int p = 0; p < a.passes; ++p) {
 		const uint32_t c = hist[p * kRadix + t];
 		hist[p * kRadix + t] = 0;  // leave the histogram zeroed for the next sort
-		if (p == a.passes - 1) s_top[t] = c;
 		const uint32_t incl = wave_inclusive_sum(c);
 		if (lane == 63) s_wave[w] = incl;
 		const bool full = static_cast<uint64_t>(c) == a.n;
@@ -259,9 +260,25 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 		uint32_t before = 0;
 		for (uint32_t v = 0; v < w; ++v) before += s_wave[v];
 		bucket[p * kRadix + t] = before + incl - c;
+		if (p == a.passes - 1) { top_count = c; top_start = before + incl - c; }
 		if (full) s_uniform[p] = 1;
 		atomicMax(&s_max[p], c);
 		__syncthreads();
+	}
+	if (a.hybrid) {
+		// the MSD plan's segment map (used only if thread 0 below chooses that plan): tiles of the
+		// second pass never straddle a top-digit segment
+		const SegTables T = seg_tables(a.segtab);
+		const uint32_t tiles = (top_count + a.tile2 - 1) / a.tile2;
+		const uint32_t incl = wave_inclusive_sum(tiles);
+		if (lane == 63) s_wave[w] = incl;
+		__syncthreads();
+		uint32_t before = 0;
+		for (uint32_t v = 0; v < w; ++v) before += s_wave[v];
+		T.tile_start[t] = before + incl - tiles;
+		if (t == kRadix - 1) T.tile_start[kRadix] = before + incl;
+		T.seg_start[t] = top_start;
+		T.seg_len[t] = top_count;
 	}
 	if (t != 0) return;
 	PassPlan none{};
@@ -352,17 +369,6 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 	ctrl->hyb[1] = h;
 	h.src = X2; h.dst = a.fin;
 	ctrl->hyb[2] = h;
-	// tiles of the second pass never straddle a top-digit segment
-	const SegTables T = seg_tables(a.segtab);
-	uint32_t tiles = 0;
-	for (int sg = 0; sg < kRadix; ++sg) {
-		T.tile_start[sg] = tiles;
-		T.seg_start[sg] = bucket[(P - 1) * kRadix + sg];
-		T.seg_len[sg] = s_top[sg];
-		tiles += (s_top[sg] + a.tile2 - 1) / a.tile2;
-	}
-	T.tile_start[kRadix] = tiles;
-	ctrl->hyb_tiles2 = tiles;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -473,7 +479,7 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(Control *ctrl, const uint3
 				total += c;
 			}
 			pub = total - ((t == kRadix - 1) ? static_cast<uint32_t>(TILE) - valid : 0u);
-			atomic_store_agent(&lookback[static_cast<uint64_t>(tile) * kRadix + t],
+			atomic_store_agent(&lookback[static_cast<uint64_t>(tile) * kLbStride + t],
 				lb_word(pub, tile == 0 ? kLbInclusive : kLbAggregate, tag));
 			const uint32_t incl = wave_inclusive_sum(total);
 			if (lane == 63) s_wsum[w] = incl;
@@ -488,8 +494,8 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(Control *ctrl, const uint3
 			// decoupled look-back over the preceding tiles of this digit
 			uint32_t excl = 0;
 			if (tile > 0) {
-				excl = lookback_window<16>(lookback + t, tile, 0, tag);
-				atomic_store_agent(&lookback[static_cast<uint64_t>(tile) * kRadix + t], lb_word(excl + pub, kLbInclusive, tag));
+				excl = lookback_window<16, kLbStride>(lookback + t, tile, 0, tag);
+				atomic_store_agent(&lookback[static_cast<uint64_t>(tile) * kLbStride + t], lb_word(excl + pub, kLbInclusive, tag));
 			}
 			s_glob[t] = pbucket[t] + excl - digoff;
 		}
@@ -565,7 +571,7 @@ __global__ __launch_bounds__(THREADS) void k_hist_tiles(const K *__restrict__ in
 		});
 		__syncthreads();
 		if (counts)
-			for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kRadix + i] = s_t[i];
+			for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kCountStride + i] = s_t[i];
 		__syncthreads();
 	}
 	__syncthreads();
@@ -607,7 +613,7 @@ __global__ __launch_bounds__(THREADS) void k_count(const Control *ctrl, int pass
 		const uint64_t lo = pp.reverse ? n - base - len : base;
 		for_each_key<K, THREADS>(src, lo, len, [&](K x) { wave_count(s_t, digit_of<K>(x, xf, shift)); });
 		__syncthreads();
-		for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kRadix + i] = s_t[i];
+		for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kCountStride + i] = s_t[i];
 		__syncthreads();
 	}
 }
@@ -639,24 +645,24 @@ __global__ __launch_bounds__(256) void k_scan(const Control *ctrl, const PassPla
 		for (uint32_t t = t0; t < t1; t += 8) {
 			uint32_t c[8];
 #pragma unroll
-			for (int j = 0; j < 8; ++j) c[j] = (t + j < t1) ? counts[static_cast<uint64_t>(t + j) * kRadix + d] : 0u;
+			for (int j = 0; j < 8; ++j) c[j] = (t + j < t1) ? counts[static_cast<uint64_t>(t + j) * kCountStride + d] : 0u;
 #pragma unroll
 			for (int j = 0; j < 8; ++j) {
-				if (t + j < t1) counts[static_cast<uint64_t>(t + j) * kRadix + d] = run;
+				if (t + j < t1) counts[static_cast<uint64_t>(t + j) * kCountStride + d] = run;
 				run += c[j];
 			}
 		}
 		uint32_t excl = 0;
-		uint64_t *lb = lookback + static_cast<uint64_t>(chunk) * kRadix + d;
+		uint64_t *lb = lookback + static_cast<uint64_t>(chunk) * kLbStride + d;
 		if (chunk == 0) {
 			atomic_store_agent(lb, lb_word(run, kLbInclusive, tag));
 		} else {
 			atomic_store_agent(lb, lb_word(run, kLbAggregate, tag));
-			excl = lookback_window<8>(lookback + d, chunk, 0, tag);
+			excl = lookback_window<8, kLbStride>(lookback + d, chunk, 0, tag);
 			atomic_store_agent(lb, lb_word(excl + run, kLbInclusive, tag));
 		}
 		const uint32_t add = start + excl;
-		for (uint32_t t = t0; t < t1; ++t) counts[static_cast<uint64_t>(t) * kRadix + d] += add;
+		for (uint32_t t = t0; t < t1; ++t) counts[static_cast<uint64_t>(t) * kCountStride + d] += add;
 	}
 }
 
@@ -759,7 +765,7 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 			dig[j] = l < valid ? digit_of<K>(keys[j], xf, shift) : 0xFFu;
 		}
 		block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total);
-		if (t < kRadix) s_glob[t] = offsets[static_cast<uint64_t>(tile) * kRadix + t] - s_digoff[t];
+		if (t < kRadix) s_glob[t] = offsets[static_cast<uint64_t>(tile) * kCountStride + t] - s_digoff[t];
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) s_keys[pos[j]] = keys[j];
 		uint32_t ppos[PAY ? KPT : 1];
@@ -828,7 +834,7 @@ __global__ __launch_bounds__(THREADS) void k_hyb_count(const PassPlan *ppp, uint
 		seg_tile(s_ts, seg, tile_keys, tile, base, len);
 		for_each_key<K, THREADS>(src, base, len, [&](K x) { wave_count(s_t, digit_of<K>(x, xf, shift)); });
 		__syncthreads();
-		for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kRadix + i] = s_t[i];
+		for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kCountStride + i] = s_t[i];
 		__syncthreads();
 	}
 }
@@ -846,7 +852,7 @@ __global__ __launch_bounds__(256) void k_hyb_segscan(Control *ctrl, SegTables T,
 	for (uint32_t tl = t0; tl < t1; tl += 8) {
 		uint32_t c[8];
 #pragma unroll
-		for (int j = 0; j < 8; ++j) c[j] = (tl + j < t1) ? counts[static_cast<uint64_t>(tl + j) * kRadix + d] : 0u;
+		for (int j = 0; j < 8; ++j) c[j] = (tl + j < t1) ? counts[static_cast<uint64_t>(tl + j) * kCountStride + d] : 0u;
 #pragma unroll
 		for (int j = 0; j < 8; ++j) tot += c[j];
 	}
@@ -862,10 +868,10 @@ __global__ __launch_bounds__(256) void k_hyb_segscan(Control *ctrl, SegTables T,
 	for (uint32_t tl = t0; tl < t1; tl += 8) {
 		uint32_t c[8];
 #pragma unroll
-		for (int j = 0; j < 8; ++j) c[j] = (tl + j < t1) ? counts[static_cast<uint64_t>(tl + j) * kRadix + d] : 0u;
+		for (int j = 0; j < 8; ++j) c[j] = (tl + j < t1) ? counts[static_cast<uint64_t>(tl + j) * kCountStride + d] : 0u;
 #pragma unroll
 		for (int j = 0; j < 8; ++j) {
-			if (tl + j < t1) counts[static_cast<uint64_t>(tl + j) * kRadix + d] = run;
+			if (tl + j < t1) counts[static_cast<uint64_t>(tl + j) * kCountStride + d] = run;
 			run += c[j];
 		}
 	}
@@ -1363,7 +1369,7 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	const bool hybrid = lcfg >= 0;
 	const uint32_t tile_keys = static_cast<uint32_t>(kTileCfgs[cfg].threads * kTileCfgs[cfg].kpt);
 	const uint32_t tiles = static_cast<uint32_t>((n + tile_keys - 1) / tile_keys);
-	uint32_t *counts = counts_of(ws, n);
+	uint32_t *counts = counts_of(ws);
 	const bool fill = (P == 1) && !job.payload && job.fill8 &&
 		(job.flags & 7u) != (kAbs | kSign) && (job.flags & 7u) != (kSign | kFltp) && (job.flags & 7u) != (kAbs | kSign | kFltp);
 	// the histogram read also counts, per tile, the digit of the first pass: digit 0 for the LSD
@@ -1405,7 +1411,7 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	pa.hybrid = hybrid ? 1 : 0;
 	pa.hyb_cap = hybrid ? kLocalCaps[lcfg] : 0;
 	pa.tile2 = tile_keys;
-	pa.segtab = segtab_of(ws, n);
+	pa.segtab = segtab_of(ws);
 	pa.partial = hybrid ? 1 : 0;
 	pa.gate = nullptr;
 
@@ -1413,7 +1419,7 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	if ((e = hipGetLastError()) != hipSuccess) return e;
 	if constexpr (P >= 4) {
 		if (hybrid) {
-			e = launch_hybrid_cfg<K>(cfg, ctrl, bucket, counts, lookback, seg_tables(segtab_of(ws, n)), xf, n, tiles, lcfg, ws,
+			e = launch_hybrid_cfg<K>(cfg, ctrl, bucket, counts, lookback, seg_tables(segtab_of(ws)), xf, n, tiles, lcfg, ws,
 				stream);
 			if (e != hipSuccess) return e;
 			// the LSD count + scan plan as the fallback: a full histogram read and a second plan,
@@ -1540,9 +1546,8 @@ int default_cfg(int key_bytes, uint64_t n, bool payload)
 
 size_t sort_workspace_bytes(uint64_t n)
 {
-	// look-back words (one per tile and digit), the tile-count matrix, the MSD plan's tables
-	return kLookbackOffset + static_cast<size_t>(max_tiles(n)) * kRadix * sizeof(uint64_t) +
-		static_cast<size_t>(count_rows(n)) * kRadix * sizeof(uint32_t) + kSegTableWords * sizeof(uint32_t);
+	// the header, then one record (look-back row + count row) per tile
+	return kHeaderBytes + static_cast<size_t>(record_count(n)) * kRecordBytes;
 }
 
 hipError_t workspace_init(void *ws, size_t bytes, hipStream_t stream)

@@ -4,6 +4,8 @@ A *plan* is (algorithm, tile configuration, presort check):
   algorithm 1  onesweep: each scatter pass resolves its tile offsets by decoupled look-back
   algorithm 2  count + chained scan + scatter per pass
   algorithm 3  one block sorts everything in LDS (keys only, n <= 8192)
+  algorithm 4  two MSD partitions by the top two digits, then one LDS sort per bucket (keys only,
+               4/8-byte keys; narrower keys and skewed inputs end in algorithm 2)
   tile config  index into TILE_CONFIGS (threads x keys per thread of the scatter kernels)
   presort      first test whether the input is already in order (or strictly reversed)
 Every plan produces the same, bit-identical output; they differ only in time.
@@ -35,7 +37,7 @@ NUM_FEATURES = 24
 HIDDEN = 32
 LDS_MAX = 8192
 
-PLANS = [(a, c, chk) for a in (1, 2) for c in range(len(TILE_CONFIGS)) for chk in (0, 1)] + [(3, -1, 0)]
+PLANS = [(a, c, chk) for a in (1, 2, 4) for c in range(len(TILE_CONFIGS)) for chk in (0, 1)] + [(3, -1, 0)]
 NUM_PLANS = len(PLANS)
 
 DTYPES = ("u8", "u16", "u32", "u64", "f32", "f64")
@@ -61,8 +63,8 @@ def _lib():
             "rsbd8_hip_plan_count": ([], ci),
             "rsbd8_hip_plan_at": ([ci, ctypes.POINTER(_Plan)], ci),
             "rsbd8_hip_select_plan": ([ci, cu, sz, vp, vp, ci, ctypes.POINTER(ci), vp], ci),
-            "rsbd8_hip_radixsortcopynoalloc_learned": ([ci, cu, sz, vp, sz, vp, vp, vp, vp, ci, ctypes.POINTER(ci), vp],
-                                                        ci),
+            "rsbd8_hip_radixsortcopynoalloc_learned": ([ci, cu, sz, vp, sz, vp, vp, vp, vp, ci, sz, ctypes.POINTER(_Plan),
+                                                         ctypes.POINTER(ci), vp], ci),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -257,6 +259,7 @@ class Selector:
         assert params.size == self.count
         self.params = torch.from_numpy(params.astype(np.float32)).to(device)
         self.device = device
+        self.min_count = 0  # below this many keys the fixed plan is used without a selection
 
     def forward(self, x, acts=None):
         torch = _torch()
@@ -303,13 +306,15 @@ class Selector:
                                             self.H, ctypes.byref(idx), _stream_ptr(stream)))
         return idx.value
 
-    def sort(self, keys, out, buf, ws: Workspace, flags: int, stream=None) -> int:
-        """rsbd8_hip_radixsortcopynoalloc_learned: select on the device, then sort; returns the plan index."""
+    def sort(self, keys, out, buf, ws: Workspace, flags: int, below=None, stream=None) -> int:
+        """rsbd8_hip_radixsortcopynoalloc_learned: select on the device, then sort; returns the plan
+        index, or -1 when the request is below the gate (min_count) and the plan below was used."""
         idx = ctypes.c_int(-1)
+        bp = ctypes.byref(_Plan(int(below[0]), int(below[1]), int(below[2]), 0)) if below is not None else None
         _check(_lib().rsbd8_hip_radixsortcopynoalloc_learned(keytype_of(keys.dtype), flags, keys.numel(), ws.ptr(),
                                                              ws.nbytes, _ptr(keys), _ptr(out), _ptr(buf),
-                                                             _ptr(self.params), self.H, ctypes.byref(idx),
-                                                             _stream_ptr(stream)))
+                                                             _ptr(self.params), self.H, int(self.min_count), bp,
+                                                             ctypes.byref(idx), _stream_ptr(stream)))
         return idx.value
 
     def choose(self, feats, n: int):
@@ -321,13 +326,35 @@ class Selector:
         return int(np.argmin(y))
 
     def save(self, path):
-        np.savez(path, params=self.params.cpu().numpy(), plans=np.array(PLANS), F=self.F, H=self.H, O=self.O)
+        np.savez(path, params=self.params.cpu().numpy(), plans=np.array(PLANS), F=self.F, H=self.H, O=self.O,
+                 min_count=self.min_count)
 
     @classmethod
     def load(cls, path, device="cuda"):
         d = np.load(path, allow_pickle=False)
         assert [tuple(p) for p in d["plans"].tolist()] == PLANS
-        return cls(d["params"], device=device)
+        sel = cls(d["params"], device=device)
+        sel.min_count = int(d["min_count"]) if "min_count" in d else 0
+        return sel
+
+    def calibrate_gate(self, requests, X, T, M, table: dict, overhead_ms: float):
+        """Chooses min_count (a power of two, or 0) minimising, over requests with measured per-plan
+        timings, the fixed plan's time below the gate plus the picked plan's time and the selection
+        overhead above it. Returns {gate_log2: total ms}."""
+        torch = _torch()
+        y = self.forward(torch.from_numpy(X).to(self.device)).cpu().numpy()
+        ms = np.where(M > 0, np.exp2(T) * 1e-3, np.nan)
+        tf, tl, ns = [], [], []
+        for i, (dt, n, _, _) in enumerate(requests):
+            nb = torch.tensor([], dtype=torch_dtype(dt)).element_size()
+            tf.append(float(ms[i, PLANS.index(fixed_plan(table, nb, n))]))
+            tl.append(float(ms[i, int(np.argmin(np.where(M[i] > 0, y[i], np.inf)))]) + overhead_ms)
+            ns.append(n)
+        tf, tl, ns = np.array(tf), np.array(tl), np.array(ns)
+        totals = {g: float(np.where(ns < (1 << g), tf, tl).sum()) for g in range(0, 31)}
+        best = min(totals, key=lambda g: (totals[g], g))
+        self.min_count = 0 if best == 0 else (1 << best)
+        return totals
 
 
 def init_params(F, H, O, seed):
@@ -450,6 +477,23 @@ def collect(requests, reps: int = 3, log=None):
     return X, T, M
 
 
+def selection_overhead_ms(selector: Selector, reps: int = 50) -> float:
+    """Median wall time of rsbd8_hip_select_plan (sampling, network, the choice's round trip) on a
+    2^20-key array: the fixed cost a learned choice must earn back."""
+    torch = _torch()
+    gen = torch.Generator(device="cuda").manual_seed(99)
+    keys = make_keys("u32", 1 << 20, "uniform", gen)
+    flags = decode(keytype_of(keys.dtype))
+    selector.select(keys, flags)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        selector.select(keys, flags)
+        times.append((time.perf_counter() - t0) * 1e3)
+    return float(np.median(times))
+
+
 def regret(requests, X, T, M, selector: Selector, table: dict):
     """Offline comparison on measured per-plan timings (no selection overhead): total time of the
     fixed plan, of the network's pick and of the best plan per request, in ms."""
@@ -497,7 +541,7 @@ def evaluate(requests, selector: Selector, table: dict, verify: bool = True, log
         tfw = (time.perf_counter() - t0) * 1e3
 
         def learned():
-            return selector.sort(keys, out2[: n * nb].view(keys.dtype), buf, arena.ws, flags)
+            return selector.sort(keys, out2[: n * nb].view(keys.dtype), buf, arena.ws, flags, below=pf)
 
         idx = learned()
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -507,7 +551,8 @@ def evaluate(requests, selector: Selector, table: dict, verify: bool = True, log
         b.record()
         b.synchronize()
         tl = max(a.elapsed_time(b), (time.perf_counter() - t0) * 1e3)
-        chosen[PLANS[idx]] = chosen.get(PLANS[idx], 0) + 1
+        key = PLANS[idx] if idx >= 0 else "below gate (fixed plan)"
+        chosen[key] = chosen.get(key, 0) + 1
         if verify and not torch.equal(out.view(torch.uint8)[: n * nb], out2[: n * nb]):
             mismatches += 1
         tot_fixed += tf

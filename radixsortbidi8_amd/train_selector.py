@@ -5,7 +5,8 @@
 1. autotune: the fixed plan table (fastest plan per key width and size on uniform keys);
 2. collect: features and per-plan timings of a seeded request stream generated and timed by the
    sorter itself (sizes log-uniform in [2^10, 2^24], the six dtypes and six distributions);
-3. train: the selector network, on the GPU, with the library's forward/backward/Adam kernels;
+3. train: the selector network, on the GPU, with the library's forward/backward/Adam kernels, and
+   the size gate below which the fixed plan is used (the selection's round trip does not pay);
 4. evaluate: a held-out stream of other seeds: per-plan timings (the regret of the fixed and the
    learned pick against the best plan), then live sorts with the fixed and the learned plan, the
    learned one including its selection cost, outputs compared.
@@ -53,8 +54,11 @@ def main():
     dev = "cuda"
     loss = sel.train(torch.from_numpy(X).to(dev), torch.from_numpy(T).to(dev), torch.from_numpy(M).to(dev),
                      steps=args.steps, log=log)
+    overhead = pl.selection_overhead_ms(sel)
+    totals = sel.calibrate_gate(train, X, T, M, table, overhead)
     sel.save(os.path.join(args.out, "selector_model.npz"))
-    log(f"train done in {time.time() - t0:.1f} s, loss {loss:.4f}")
+    log(f"train done in {time.time() - t0:.1f} s, loss {loss:.4f}, selection overhead {overhead:.4f} ms, "
+        f"gate {sel.min_count} keys")
 
     held = pl.request_stream(args.eval_requests, 0x5EED2000, 10, args.max_log2)
     HX, HT, HM = pl.collect(held, log=log)
@@ -66,6 +70,9 @@ def main():
     res["offline"] = offline
     res["train_loss_log2_us"] = loss
     res["train_requests"] = args.train_requests
+    res["selection_overhead_ms"] = overhead
+    res["gate_min_count"] = sel.min_count
+    res["gate_train_totals_ms"] = {str(g): round(v, 4) for g, v in totals.items()}
     with open(os.path.join(args.out, "report.json"), "w") as f:
         json.dump(res, f, indent=1)
     log(json.dumps(res))
