@@ -141,7 +141,7 @@ int rsbd8_hip_segmented_plan_info(const rsbd8_hip_segplan *plan, uint64_t info[7
 void rsbd8_hip_segmented_plan_destroy(rsbd8_hip_segplan *plan);
 
 /* Process-wide tile configuration of the scatter passes: 0..4 selects one of
- * {256x8, 256x16, 512x8, 512x16, 256x24} threads x keys per thread, -1 restores the automatic
+ * {256x8, 256x16, 512x8, 1024x12, 256x24} threads x keys per thread, -1 restores the automatic
  * choice. Returns the previous setting. (No reference counterpart: the reference's static
  * thread thresholds H:12948-13152 are its closest analogue.) */
 int rsbd8_hip_set_tile_config(int cfg);

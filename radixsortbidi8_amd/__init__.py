@@ -56,7 +56,8 @@ class sortingdirection(enum.IntEnum):
 ABS, SIGN, FLTP, DESC, REV = 1, 2, 4, 8, 16
 LAYOUT_KEYS, LAYOUT_PAIRS, LAYOUT_INDIRECT = 0, 1, 2
 U8, U16, U32, U64, I8, I16, I32, I64, F32, F64 = range(10)
-TILE_CONFIGS = ((256, 8), (256, 16), (512, 8), (512, 16), (256, 24))
+TILE_CONFIGS = ((256, 8), (256, 16), (512, 8), (1024, 12), (256, 24))
+SEGMENTED_TILE_CONFIGS = ((256, 8), (256, 16), (512, 8), (512, 16), (256, 24))
 
 _lib = None
 

@@ -14,7 +14,7 @@ struct TileCfg {
 	int kpt;
 };
 constexpr int kNumTileCfgs = 5;
-constexpr TileCfg kTileCfgs[kNumTileCfgs] = {{256, 8}, {256, 16}, {512, 8}, {512, 16}, {256, 24}};
+constexpr TileCfg kTileCfgs[kNumTileCfgs] = {{256, 8}, {256, 16}, {512, 8}, {1024, 12}, {256, 24}};
 constexpr int kMinTile = 256 * 8;
 
 // One planned pass as the device sees it (written by the plan kernel, read by the scatter kernel).

@@ -302,7 +302,7 @@ struct SegPlan {
 	int key_bytes = 4;
 	unsigned flags = 0;
 	uint64_t nseg = 0, total = 0;
-	int tile_cfg = 1;    // onesweep tile of the long segments (kTileCfgs)
+	int tile_cfg = 1;    // onesweep tile of the long segments (kSegTileCfgs)
 	int small_cfg = 1;   // LDS capacity tier (kSmallCaps)
 	uint32_t nsmall = 0, nlarge = 0, ntiles = 0, tile_keys = 0, small_cap = 0;
 	char *mem = nullptr;  // one device allocation for every table and the workspace
@@ -318,6 +318,10 @@ struct SegPlan {
 	hipEvent_t fork = nullptr, join = nullptr;
 };
 
+// tiles of the long segments' onesweep passes (threads x keys per thread), in the order of the
+// tile_cfg argument of the segmented plan
+constexpr int kNumSegTileCfgs = 5;
+constexpr TileCfg kSegTileCfgs[kNumSegTileCfgs] = {{256, 8}, {256, 16}, {512, 8}, {512, 16}, {256, 24}};
 constexpr int kNumSmallCaps = 3;
 constexpr int kSmallCaps[kNumSmallCaps] = {256 * 4, 256 * 16, 512 * 16};
 
@@ -400,7 +404,7 @@ hipError_t run_typed(SegPlan &pl, const void *in, void *out, void *buf, const ui
 
 } // namespace
 
-int tile_keys_of(int cfg) { return kTileCfgs[cfg].threads * kTileCfgs[cfg].kpt; }
+int tile_keys_of(int cfg) { return kSegTileCfgs[cfg].threads * kSegTileCfgs[cfg].kpt; }
 int small_cap_of(int cfg) { return kSmallCaps[cfg]; }
 
 hipError_t segplan_create(int key_bytes, unsigned flags, uint64_t nseg, const uint64_t *host_off, int tile_cfg, int small_cfg,
@@ -408,7 +412,7 @@ hipError_t segplan_create(int key_bytes, unsigned flags, uint64_t nseg, const ui
 {
 	*out = nullptr;
 	if (key_bytes != 1 && key_bytes != 2 && key_bytes != 4 && key_bytes != 8) return hipErrorInvalidValue;
-	if (tile_cfg < 0 || tile_cfg >= kNumTileCfgs || small_cfg < 0 || small_cfg >= kNumSmallCaps) return hipErrorInvalidValue;
+	if (tile_cfg < 0 || tile_cfg >= kNumSegTileCfgs || small_cfg < 0 || small_cfg >= kNumSmallCaps) return hipErrorInvalidValue;
 	SegPlan *pl = new (std::nothrow) SegPlan;
 	if (!pl) return hipErrorOutOfMemory;
 	pl->key_bytes = key_bytes; pl->flags = flags; pl->nseg = nseg; pl->tile_cfg = tile_cfg; pl->small_cfg = small_cfg;

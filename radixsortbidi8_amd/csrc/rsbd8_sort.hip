@@ -1189,7 +1189,7 @@ hipError_t launch_pass_cfg(int cfg, Control *ctrl, const uint32_t *bucket, uint6
 	case 0: return launch_pass<K, PAY, 256, 8>(ctrl, bucket, lookback, n, xf, pass, stream);
 	case 1: return launch_pass<K, PAY, 256, 16>(ctrl, bucket, lookback, n, xf, pass, stream);
 	case 2: return launch_pass<K, PAY, 512, 8>(ctrl, bucket, lookback, n, xf, pass, stream);
-	case 3: return launch_pass<K, PAY, 512, 16>(ctrl, bucket, lookback, n, xf, pass, stream);
+	case 3: return launch_pass<K, PAY, 1024, 12>(ctrl, bucket, lookback, n, xf, pass, stream);
 	default: return launch_pass<K, PAY, 256, 24>(ctrl, bucket, lookback, n, xf, pass, stream);
 	}
 }
@@ -1223,7 +1223,7 @@ hipError_t launch_scatter_cfg(int cfg, Control *ctrl, int pass, const KeyXform &
 	case 0: return launch_scatter<K, PAY, 256, 8>(pp, sh, xf, n, tiles, offsets, none, stream);
 	case 1: return launch_scatter<K, PAY, 256, 16>(pp, sh, xf, n, tiles, offsets, none, stream);
 	case 2: return launch_scatter<K, PAY, 512, 8>(pp, sh, xf, n, tiles, offsets, none, stream);
-	case 3: return launch_scatter<K, PAY, 512, 16>(pp, sh, xf, n, tiles, offsets, none, stream);
+	case 3: return launch_scatter<K, PAY, 1024, 12>(pp, sh, xf, n, tiles, offsets, none, stream);
 	default: return launch_scatter<K, PAY, 256, 24>(pp, sh, xf, n, tiles, offsets, none, stream);
 	}
 }
@@ -1323,7 +1323,7 @@ hipError_t launch_hybrid_cfg(int cfg, Control *ctrl, const uint32_t *bucket, uin
 	case 0: return launch_hybrid<K, 256, 8>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
 	case 1: return launch_hybrid<K, 256, 16>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
 	case 2: return launch_hybrid<K, 512, 8>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
-	case 3: return launch_hybrid<K, 512, 16>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
+	case 3: return launch_hybrid<K, 1024, 12>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
 	default: return launch_hybrid<K, 256, 24>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
 	}
 }
@@ -1536,10 +1536,11 @@ int default_algo(int key_bytes, uint64_t n, bool keys_only)
 	return n >= (1u << 22) ? kAlgoCountScan : kAlgoOnesweep;
 }
 
+// measured on MI355X: 1024x12 tiles (long digit runs per tile) for 8-byte keys from 2^22 keys
 int default_cfg(int key_bytes, uint64_t n, bool payload)
 {
 	if (n <= (1u << 16)) return 0;
-	if (default_algo(key_bytes, n, false) == kAlgoCountScan) return key_bytes >= 8 ? 4 : 1;
+	if (default_algo(key_bytes, n, false) == kAlgoCountScan) return key_bytes >= 8 ? 3 : 1;
 	(void)payload;
 	return key_bytes >= 8 ? 4 : 2;
 }

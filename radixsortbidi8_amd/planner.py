@@ -221,15 +221,19 @@ class Arena:
 
 
 def time_call(fn, reps: int = 3):
+    """Median completion time (ms) of fn from an idle device: the host time from the call to the
+    end of its last kernel, the cost a caller that waits for the result pays (it includes the launch
+    overhead that a caller pipelining many sorts would hide)."""
     torch = _torch()
     times = []
+    torch.cuda.synchronize()
     for _ in range(reps):
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
+        b = torch.cuda.Event()
+        t0 = time.perf_counter()
         fn()
         b.record()
         b.synchronize()
-        times.append(a.elapsed_time(b))
+        times.append((time.perf_counter() - t0) * 1e3)
     return float(np.median(times))
 
 
@@ -514,17 +518,16 @@ def regret(requests, X, T, M, selector: Selector, table: dict):
 
 
 def evaluate(requests, selector: Selector, table: dict, verify: bool = True, log=None, max_n=None):
-    """Sorts every request with the fixed and the learned plan. Returns totals (ms) and checks
-    that both outputs are bit-identical. fixed_ms is the device time of the fixed plan's sort,
-    fixed_wall_ms the host wall time of the same call up to its completion (it includes the launch
-    overhead a pipelined caller hides). The learned time is the wall time of the whole
-    rsbd8_hip_radixsortcopynoalloc_learned call: sampling, network, the host round trip of the
-    choice, and the sort launched from an idle device."""
+    """Sorts every request with the fixed and the learned plan. Returns the total completion times
+    (time_call: from an idle device to the end of the sort, in ms) and checks that both outputs are
+    bit-identical. The learned time is that of the whole rsbd8_hip_radixsortcopynoalloc_learned
+    call: sampling, network and the host round trip of the choice (above the size gate), then the
+    sort."""
     torch = _torch()
     max_n = max_n or max(r[1] for r in requests)
     arena = Arena(max_n)
     out2 = torch.empty(max_n * 8, dtype=torch.uint8, device="cuda")
-    tot_fixed = tot_fixed_wall = tot_learned = 0.0
+    tot_fixed = tot_learned = 0.0
     chosen = {}
     mismatches = 0
     for i, (dt, n, dist, seed) in enumerate(requests):
@@ -535,32 +538,23 @@ def evaluate(requests, selector: Selector, table: dict, verify: bool = True, log
         nb = keys.element_size()
         pf = fixed_plan(table, nb, n)
         sort_with_plan(keys, out, buf, arena.ws, pf, flags)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
         tf = time_call(lambda: sort_with_plan(keys, out, buf, arena.ws, pf, flags), 1)
-        tfw = (time.perf_counter() - t0) * 1e3
 
         def learned():
             return selector.sort(keys, out2[: n * nb].view(keys.dtype), buf, arena.ws, flags, below=pf)
 
         idx = learned()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        t0 = time.perf_counter()
-        a.record()
+        tl = time_call(learned, 1)
         idx = learned()
-        b.record()
-        b.synchronize()
-        tl = max(a.elapsed_time(b), (time.perf_counter() - t0) * 1e3)
         key = PLANS[idx] if idx >= 0 else "below gate (fixed plan)"
         chosen[key] = chosen.get(key, 0) + 1
         if verify and not torch.equal(out.view(torch.uint8)[: n * nb], out2[: n * nb]):
             mismatches += 1
         tot_fixed += tf
-        tot_fixed_wall += max(tf, tfw)
         tot_learned += tl
         if log and i % 256 == 0:
             log(f"eval {i}/{len(requests)} fixed {tot_fixed:.1f} ms learned {tot_learned:.1f} ms")
         del keys
-    return {"requests": len(requests), "fixed_ms": tot_fixed, "fixed_wall_ms": tot_fixed_wall, "learned_ms": tot_learned,
+    return {"requests": len(requests), "fixed_ms": tot_fixed, "learned_ms": tot_learned,
             "speedup": tot_fixed / max(tot_learned, 1e-9), "mismatches": mismatches,
             "chosen": {str(k): v for k, v in sorted(chosen.items(), key=lambda kv: -kv[1])}}

@@ -4,7 +4,7 @@
 
 1. autotune: the fixed plan table (fastest plan per key width and size on uniform keys);
 2. collect: features and per-plan timings of a seeded request stream generated and timed by the
-   sorter itself (sizes log-uniform in [2^10, 2^24], the six dtypes and six distributions);
+   sorter itself (sizes log-uniform in [2^10, 2^29], the six dtypes and six distributions);
 3. train: the selector network, on the GPU, with the library's forward/backward/Adam kernels, and
    the size gate below which the fixed plan is used (the selection's round trip does not pay);
 4. evaluate: a held-out stream of other seeds: per-plan timings (the regret of the fixed and the
@@ -28,7 +28,7 @@ def main():
     ap.add_argument("--train-requests", type=int, default=360)
     ap.add_argument("--eval-requests", type=int, default=240)
     ap.add_argument("--steps", type=int, default=3000)
-    ap.add_argument("--max-log2", type=float, default=24.0)
+    ap.add_argument("--max-log2", type=float, default=29.0)
     args = ap.parse_args()
     os.makedirs(args.out, exist_ok=True)
     import torch
