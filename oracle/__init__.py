@@ -65,6 +65,8 @@ def lib() -> ctypes.CDLL:
         L.rsbd8o_sort_segmented.restype = ctypes.c_int
         L.rsbd8o_histogram8.argtypes = [c_u, c_u, c_size, c_vp, c_vp]
         L.rsbd8o_histogram8.restype = None
+        L.rsbd8o_sortkeys.argtypes = [c_u, c_u, c_size, c_vp, c_vp]
+        L.rsbd8o_sortkeys.restype = None
         L.rsbd8o_radixsort_mt.argtypes = [c_u, c_u, c_size, c_vp, c_vp, c_u]
         L.rsbd8o_radixsort_mt.restype = ctypes.c_int
         _lib = L
@@ -81,6 +83,44 @@ def decode(keytype: int, mode: int, direction: int, indirect: bool = False) -> i
 
 def sortkey(width_bits: int, flags: int, x: int) -> int:
     return int(lib().rsbd8o_sortkey(width_bits, flags, x))
+
+
+def sortkeys(keys: np.ndarray, flags: int) -> np.ndarray:
+    """rsbd8o_sortkey of every key (the descending bit not applied), as uint64."""
+    kb, w = _bits(keys)
+    out = np.empty(kb.size, np.uint64)
+    lib().rsbd8o_sortkeys(w, flags, kb.size, _ptr(kb), _ptr(out))
+    return out
+
+
+def is_stable_sort(keys_in: np.ndarray, keys_out: np.ndarray, perm: np.ndarray, flags: int) -> bool:
+    """Whether keys_out with source permutation perm is exactly the reference's stable sort of
+    keys_in under flags, checked through properties that hold at any size: perm is a permutation,
+    keys_out is keys_in gathered by perm (bit for bit), and consecutive outputs are ordered by
+    (sort key ascending, or descending with the descending bit; then source index ascending, or
+    descending with the tie reversal), the order rsbd8o_mergesort sorts by."""
+    n = keys_in.size
+    perm = np.ascontiguousarray(perm).astype(np.int64, copy=False)
+    if keys_out.size != n or perm.size != n:
+        return False
+    if n == 0:
+        return True
+    if perm.min() < 0 or perm.max() >= n:
+        return False
+    seen = np.zeros(n, bool)
+    seen[perm] = True
+    if not seen.all():
+        return False
+    kin, _ = _bits(keys_in)
+    kout, _ = _bits(keys_out)
+    if not np.array_equal(kin[perm], kout):
+        return False
+    sk = sortkeys(keys_out, flags)
+    a, b = sk[:-1], sk[1:]
+    pa, pb = perm[:-1], perm[1:]
+    strict = (a > b) if (flags & 8) else (a < b)
+    ties = (pa > pb) if (flags & 16) else (pa < pb)
+    return bool(np.all(strict | ((a == b) & ties)))
 
 
 def _ptr(a):

@@ -7,6 +7,7 @@
  * Pinning (what checks each function; all run in tests/test_oracle.py):
  *   rsbd8o_decode       -- the reference's mode table read as text (H:59778-59904) restated as an
  *                          independent Python table, plus the known-answer tests below.
+ *   rsbd8o_sortkeys     -- equality with rsbd8o_sortkey element by element.
  *   rsbd8o_sortkey      -- brute force over all 2^8 and 2^16 patterns against the ordering
  *                          documented at H:130-155 and H:258-319 (IEEE order by float value for
  *                          forcefloatingp, |x| order for the absolute modes, the listed tiered and
@@ -87,6 +88,14 @@ uint64_t rsbd8o_sortkey(unsigned w, unsigned flags, uint64_t x)
 }
 
 /* ---- the reference algorithm --------------------------------------------------------------- */
+
+static uint64_t load_key(const uint8_t *p, unsigned bytes);
+
+void rsbd8o_sortkeys(unsigned w, unsigned flags, size_t n, const void *in, uint64_t *out)
+{
+	const unsigned bytes = w / 8u;
+	for (size_t i = 0; i < n; ++i) out[i] = rsbd8o_sortkey(w, flags, load_key((const uint8_t *)in + i * bytes, bytes));
+}
 
 /* digit geometry on 64-bit hosts: typeradix H:1848, offsetsloopcount H:1864,
  * typeradixremainder H:1869 */

@@ -47,6 +47,9 @@ unsigned rsbd8o_decode(int keytype, unsigned mode, unsigned direction, int indir
  * transform H:11649-11701. */
 uint64_t rsbd8o_sortkey(unsigned width_bits, unsigned flags, uint64_t x);
 
+/* rsbd8o_sortkey over an array of n width_bits/8-byte keys, into n 64-bit words. */
+void rsbd8o_sortkeys(unsigned width_bits, unsigned flags, size_t n, const void *in, uint64_t *out);
+
 /* The reference's single-threaded algorithm: one counting pass over all digits with the
  * reference's digit geometry (H:1848-1869), per-digit bidirectional offsets with pass skip
  * (H:11626-11809), and stable bidirectional scatter passes (H:31649-31982). keys: width_bits/8

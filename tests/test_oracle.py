@@ -298,3 +298,30 @@ def test_indirect_and_segmented():
     out = oracle.sort_segmented(keys, seg, flags)
     for b, e in zip(seg[:-1], seg[1:]):
         assert np.array_equal(out[int(b):int(e)], np.sort(keys[int(b):int(e)], kind="stable"))
+
+
+def test_vector_sortkeys_and_order_checker():
+    """sortkeys equals the scalar sortkey; is_stable_sort accepts the oracle's own result and
+    rejects a broken permutation, a swapped tie and a swapped pair of distinct keys."""
+    rng = np.random.default_rng(77)
+    for w, dt in ((8, np.uint8), (16, np.uint16), (32, np.uint32), (64, np.uint64)):
+        keys = rng.integers(0, np.iinfo(dt).max, 500, dtype=dt, endpoint=True)
+        for flags in range(32):
+            v = oracle.sortkeys(keys, flags)
+            assert all(int(v[i]) == oracle.sortkey(w, flags, int(keys[i])) for i in range(0, 500, 7))
+    keys = rng.integers(0, 50, 4000).astype(np.uint32)
+    for flags in (0, 8, 16, 24, 2, 26):
+        out, perm = oracle.radixsort(keys, flags, want_payload=True)
+        assert oracle.is_stable_sort(keys, out, perm, flags)
+        bad = perm.copy()
+        bad[0] = bad[1]
+        assert not oracle.is_stable_sort(keys, keys[bad], bad, flags)
+        sk = oracle.sortkeys(out, flags)
+        i = int(np.nonzero(sk[:-1] == sk[1:])[0][0])          # swap two equal keys: the tie order breaks
+        p2 = perm.copy()
+        p2[i], p2[i + 1] = p2[i + 1], p2[i]
+        assert not oracle.is_stable_sort(keys, keys[p2], p2, flags)
+        j = int(np.nonzero(sk[:-1] != sk[1:])[0][0])          # swap two different keys: the order breaks
+        p3 = perm.copy()
+        p3[j], p3[j + 1] = p3[j + 1], p3[j]
+        assert not oracle.is_stable_sort(keys, keys[p3], p3, flags)
