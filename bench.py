@@ -172,6 +172,9 @@ def main():
     # the reference's own plan for this key type: one counting read, then six 11-bit-digit passes
     # (H:1848-1896) that each read and write every key
     ref_bytes = k * n + 6 * 2 * k * n
+    # the 8-bit LSD plan's bytes (one histogram read, eight read+write passes): the bytes-moved
+    # roofline the workload's target is stated against (100% = 136 B/key at 8 TB/s)
+    lsd8_bytes = k * n + 8 * 2 * k * n
     sort_gbs = sort_bytes / (ms_per_step * 1e-3) / 1e9
     traffic = traffic_from_profile(args.n_log2, dom)
     phases = {f: {"ms_per_step": round(v[0][0] / steps, 4), "launches_per_step": round(v[0][1] / steps, 2)}
@@ -214,6 +217,8 @@ def main():
             "frac": round(sort_gbs / HBM_PEAK_GBS, 4),
             "reference_plan_bytes": ref_bytes,
             "reference_plan_frac": round(ref_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "lsd8_plan_bytes": lsd8_bytes,
+            "lsd8_plan_frac": round(lsd8_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "phases": phases,
         },
         "verified": sorted_ok and perm_ok,

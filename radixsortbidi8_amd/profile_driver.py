@@ -4,6 +4,7 @@
                                                [--tile-config C] [--copy]
 
 Sorts 2^n seeded uniform keys (the benchmark's distribution) reps times after one warm-up sort,
+or (--segmented) sorts a batch of 65,536 int32 segments with log-uniform lengths totalling 2^n keys,
 then, with --copy, copies the same number of bytes device to device reps times: a plain copy of
 known size that calibrates the HBM byte counters (FETCH_SIZE, WRITE_SIZE) against the bytes it
 must move.
@@ -20,11 +21,16 @@ def main():
     ap.add_argument("--algo", type=int, default=0)
     ap.add_argument("--tile-config", type=int, default=-1)
     ap.add_argument("--copy", action="store_true")
+    ap.add_argument("--segmented", action="store_true",
+                    help="instead: 65,536 int32 segments, log-uniform lengths scaled to 2^n keys in total")
     args = ap.parse_args()
     import torch
 
     import radixsortbidi8_amd as r
     from radixsortbidi8_amd import planner as pl
+
+    if args.segmented:
+        return segmented(args, torch, r)
 
     n = 1 << args.n_log2
     gen = torch.Generator(device="cuda").manual_seed(0x5EED0001)
@@ -48,6 +54,37 @@ def main():
             torch.cuda.synchronize()
             print(f"profile_driver: copy {i + 1}/{args.reps} of {keys.numel() * keys.element_size()} bytes done",
                   flush=True)
+    return 0
+
+
+def segmented(args, torch, r):
+    import numpy as np
+    rng = np.random.default_rng(0x5EED0005)
+    nseg = 65536
+    lens = np.floor(2.0 ** rng.uniform(4, 20, nseg))
+    lens = np.maximum(1, np.floor(lens * ((1 << args.n_log2) / lens.sum()))).astype(np.int64)
+    off = np.zeros(nseg + 1, np.uint64)
+    off[1:] = np.cumsum(lens)
+    total = int(off[-1])
+    keys = torch.randint(-(1 << 31), (1 << 31) - 1, (total,), dtype=torch.int32, device="cuda",
+                         generator=torch.Generator(device="cuda").manual_seed(0x5EED0005))
+    out, buf = torch.empty_like(keys), torch.empty_like(keys)
+    plan = r.SegmentedPlan(np.int32, off)
+    print(f"profile_driver: segmented plan {plan.info()}", flush=True)
+    plan.run(keys, out, buf)
+    plan.capture(keys, out, buf)
+    torch.cuda.synchronize()
+    for name, fn in (("run", lambda: plan.run(keys, out, buf)), ("graph", lambda: plan.launch())):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(args.reps):
+            fn()
+        b.record()
+        b.synchronize()
+        ms = a.elapsed_time(b) / args.reps
+        print(f"profile_driver: segmented {name}: {ms:.3f} ms per batch, {total / ms / 1e6:.2f} GKeys/s "
+              f"({nseg} segments, {total} keys)", flush=True)
+    plan.close()
     return 0
 
 
