@@ -94,6 +94,18 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m)
 		__builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
 }
 
+// adds 1 to s[d] for every active lane, with one atomic for the whole wave when all lanes share d
+__device__ __forceinline__ void wave_count(uint32_t *s, uint32_t d)
+{
+	const uint64_t active = __ballot(1);
+	const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+	if (__ballot(d == d0) == active) {
+		if (lanes_below(active) == 0) atomicAdd(&s[d0], static_cast<uint32_t>(__popcll(active)));
+	} else {
+		atomicAdd(&s[d], 1u);
+	}
+}
+
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
 // inclusive wave64 prefix sum

@@ -111,18 +111,6 @@ __device__ __forceinline__ void for_each_key(const K *__restrict__ p, uint64_t l
 	for (uint32_t i = done + threadIdx.x; i < len; i += THREADS) f(q[i]);
 }
 
-// adds 1 to s[d] for every lane, with one atomic for the whole wave when all lanes share d
-__device__ __forceinline__ void wave_count(uint32_t *s, uint32_t d)
-{
-	const uint64_t active = __ballot(1);
-	const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-	if (__ballot(d == d0) == active) {
-		if (lanes_below(active) == 0) atomicAdd(&s[d0], static_cast<uint32_t>(__popcll(active)));
-	} else {
-		atomicAdd(&s[d], 1u);
-	}
-}
-
 template <typename K, int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_hist(const K *__restrict__ in, uint64_t n, KeyXform xf,
 	uint32_t *__restrict__ hist, const Control *ctrl, int reverse)

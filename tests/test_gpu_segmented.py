@@ -32,7 +32,21 @@ def make_batch(rng, nseg, lo_log2, hi_log2, dtype):
 
 def check(keys, off, out, flags):
     exp = oracle.sort_segmented(keys, off, flags)
-    assert np.array_equal(out.view(np.uint8), exp.view(np.uint8))
+    if np.array_equal(out.view(np.uint8), exp.view(np.uint8)):
+        return
+    # name the first segments that differ: length, and whether only the order of equal sort keys
+    # (stability) or the sort-key order itself is wrong
+    bad = []
+    for s in range(off.size - 1):
+        a, b = int(off[s]), int(off[s + 1])
+        if not np.array_equal(out[a:b].view(np.uint8), exp[a:b].view(np.uint8)):
+            ko, ke = oracle.sortkeys(out[a:b], flags), oracle.sortkeys(exp[a:b], flags)
+            diff = (out[a:b].view(np.uint8).reshape(b - a, -1) != exp[a:b].view(np.uint8).reshape(b - a, -1)).any(axis=1)
+            bad.append((s, b - a, "tie order" if np.array_equal(ko, ke) else "key order", int(np.flatnonzero(diff)[0]),
+                        int(diff.sum())))
+            if len(bad) >= 6:
+                break
+    raise AssertionError(f"segments differ (segment, length, kind, first index, differing): {bad}")
 
 
 @pytest.mark.parametrize("dtype", [np.int32, np.float32, np.uint64, np.uint16, np.uint8])
@@ -85,3 +99,32 @@ def test_segmented_all_equal_and_tiny():
         torch.cuda.synchronize()
         assert np.array_equal(d_out.cpu().numpy(), oracle.sort_segmented(keys, off, r.decode(r.I64, 64, d)))
         plan.close()
+
+
+@pytest.mark.parametrize("dtype", [np.uint32, np.float64, np.uint16])
+def test_segmented_long_equal_runs(dtype):
+    """Long segments whose top-digit buckets fit the LDS capacity but hold long runs of keys that
+    share the partition bits (random top byte, the rest constant or from a tiny set): the items go
+    through the LDS LSD passes; short segments of the same shape take the same route."""
+    rng = np.random.default_rng(91 + np.dtype(dtype).itemsize)
+    nb = np.dtype(dtype).itemsize
+    ut = {2: np.uint16, 4: np.uint32, 8: np.uint64}[nb]
+    lens = np.concatenate([rng.integers(20000, 200000, 12), rng.integers(200, 4000, 300)])
+    rng.shuffle(lens)
+    off = np.zeros(lens.size + 1, np.uint64)
+    off[1:] = np.cumsum(lens)
+    n = int(off[-1])
+    top = rng.integers(0, 256, n, dtype=np.uint64) << np.uint64(8 * nb - 8)
+    low = rng.integers(0, 3, n, dtype=np.uint64)
+    keys = (top | low).astype(ut).view(dtype)
+    d_in = torch.from_numpy(keys).cuda()
+    d_out, d_buf = torch.empty_like(d_in), torch.empty_like(d_in)
+    kt = r.keytype_of(keys.dtype)
+    for small_cfg in (0, 2):
+        for mode, d in ((64, 0), (64, 1), (65, 3)):
+            plan = r.SegmentedPlan(keys.dtype, off, d, mode, small_cfg=small_cfg)
+            assert plan.info()["long"] >= 12
+            plan.run(d_in, d_out, d_buf)
+            torch.cuda.synchronize()
+            check(keys, off, d_out.cpu().numpy(), r.decode(kt, mode, d))
+            plan.close()
