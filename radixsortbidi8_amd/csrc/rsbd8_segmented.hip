@@ -54,6 +54,16 @@ struct SegSel {
 	uint32_t sel;
 	uint32_t mask;
 };
+// one tile of a long segment: keys [y * tile, (y + 1) * tile) of segment L, which starts at sb and
+// holds len keys (logical order; a physical range read back to front under reverse)
+struct __align__(16) SegTile {
+	uint64_t sb;
+	uint32_t len;
+	uint32_t L;
+	uint32_t y;
+	uint32_t pad[3];
+};
+
 constexpr uint32_t kSelCopy = 1u << 30;     // no pass executes: the segment is copied
 constexpr uint32_t kSelReverse = 1u << 31;  // the first executed pass (or the copy) reads back to front
 
@@ -65,39 +75,56 @@ __device__ __forceinline__ K *pick(uint32_t code, const K *in, K *out, K *buf)
 
 // ---------------------------------------------------------------------------------------------
 
-// Words of the item sort: (transformed key << 16) | position where they fit in one integer (their
-// order is then the stable order); 8-byte keys keep the position in a separate array.
+// Words of the item sort. 2-byte keys pack (transformed key << 16) | position into one 32-bit word,
+// whose order is then the stable order; 4- and 8-byte keys keep the transformed key as the word and
+// the position in a separate 16-bit array. (1-byte keys go straight to k_item_lsd: one LDS pass
+// ranks them by the whole key.)
 template <typename K> struct ItemWord;
-template <> struct ItemWord<uint8_t> { using W = uint32_t; static constexpr bool packed = true; static constexpr int kbits = 8; };
 template <> struct ItemWord<uint16_t> { using W = uint32_t; static constexpr bool packed = true; static constexpr int kbits = 16; };
-template <> struct ItemWord<uint32_t> { using W = uint64_t; static constexpr bool packed = true; static constexpr int kbits = 32; };
+template <> struct ItemWord<uint32_t> { using W = uint32_t; static constexpr bool packed = false; static constexpr int kbits = 32; };
 template <> struct ItemWord<uint64_t> { using W = uint64_t; static constexpr bool packed = false; static constexpr int kbits = 64; };
 
-constexpr uint32_t kItemRunMax = 96;
+constexpr uint32_t kItemRunMax = 48;
+
+// transformed key in the item word's type, bits above the key width cleared
+template <typename K>
+__device__ __forceinline__ typename ItemWord<K>::W item_key(K raw, const KeyXform &xf)
+{
+	using W = typename ItemWord<K>::W;
+	constexpr int KB = ItemWord<K>::kbits;
+	const W k = static_cast<W>(sort_key<K>(raw, xf));
+	if constexpr (KB < 8 * static_cast<int>(sizeof(W))) return k & ((W(1) << KB) - 1u);
+	else return k;
+}
 
 // Items i < *count (or count_value when count is null): keys [start, start + len) of src, read back
-// to front under reverse, sorted into the same range of dst. Items with a run of more than
-// kItemRunMax keys sharing the partition bits are queued for k_item_lsd.
+// to front under reverse, sorted into the same range of dst. The keys of an item are partitioned
+// with LDS atomics by (key - min) >> sh, sh chosen so that the item's key range spans the PB
+// partition bits; each word's final position is its partition's start plus the words of the
+// partition that precede it in the stable order (smaller key, or equal key read earlier), counted
+// with independent LDS reads; the raw keys are staged in LDS at their final positions. An item
+// whose keys are all equal is copied in order. Items with a partition of more than kItemRunMax
+// keys are queued for k_item_lsd.
 template <typename K, int THREADS, int KPT, int PB>
 __global__ __launch_bounds__(THREADS) void k_item_sort(const K *__restrict__ src, K *__restrict__ dst,
 	const uint64_t *__restrict__ istart, const uint32_t *__restrict__ ilen, const uint32_t *count, uint32_t count_value,
-	KeyXform xf, int reverse, uint32_t *__restrict__ queue, uint32_t *qcount)
+	KeyXform xf, int reverse, uint32_t *__restrict__ queue, uint32_t *qcount, uint32_t qbase)
 {
 	constexpr int CAP = THREADS * KPT;
+	constexpr int NW = THREADS / 64;
 	using IW = ItemWord<K>;
 	using W = typename IW::W;
 	constexpr int KB = IW::kbits;
 	constexpr int PBE = PB < KB ? PB : KB;
 	constexpr uint32_t NB = 1u << PBE;
 	constexpr uint32_t PER = (NB + THREADS - 1) / THREADS;
-	constexpr uint32_t kFinWords = CAP / 2;
-	constexpr uint32_t kUnionWords = NB > kFinWords ? NB : kFinWords;
-	static_assert(CAP <= 65536, "positions are packed into 16 bits");
-	__shared__ __align__(16) W s_w[CAP];                 // words, then the raw keys in final order
-	__shared__ uint16_t s_l[IW::packed ? 1 : CAP];       // positions of unpacked words
-	__shared__ uint32_t s_u[kUnionWords];                // partition counters/cursors, then final positions
-	__shared__ uint16_t s_off[NB + 1];
-	__shared__ uint32_t s_wsum[THREADS / 64];
+	static_assert(CAP <= 65536, "positions are held in 16 bits");
+	static_assert(sizeof(K) <= sizeof(W), "the raw keys are staged in the word array");
+	__shared__ __align__(16) W s_w[CAP];     // words, then the raw keys in final order
+	__shared__ uint16_t s_l[CAP];            // positions of unpacked words, then each key's final position
+	__shared__ uint32_t s_cur[NB];           // partition counters, then cursors (after the scatter: ends)
+	__shared__ uint32_t s_wsum[NW];
+	__shared__ W s_min[NW], s_max[NW];
 	__shared__ uint32_t s_big;
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	const uint32_t nitems = count ? *count : count_value;
@@ -110,20 +137,55 @@ __global__ __launch_bounds__(THREADS) void k_item_sort(const K *__restrict__ src
 			continue;
 		}
 		K key[KPT];
+		W tk[KPT];
+		W mn = ~W(0), mx = 0;
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) {
 			const uint32_t l = w * 64u * KPT + j * 64u + lane;
 			key[j] = l < len ? src[start + (reverse ? len - 1 - l : l)] : K(0);
 		}
-		uint32_t *s_cur = s_u;
-		__syncthreads();  // the previous item's last reads of s_u and s_w are done
-		for (uint32_t i = t; i < NB; i += THREADS) s_cur[i] = 0;
-		if (t == 0) s_big = 0;
-		__syncthreads();
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) {
 			const uint32_t l = w * 64u * KPT + j * 64u + lane;
-			if (l < len) atomicAdd(&s_cur[static_cast<uint32_t>(sort_key<K>(key[j], xf) >> (KB - PBE))], 1u);
+			tk[j] = item_key<K>(key[j], xf);
+			if (l < len) {
+				mn = min(mn, tk[j]);
+				mx = max(mx, tk[j]);
+			}
+		}
+#pragma unroll
+		for (int o = 32; o > 0; o >>= 1) {
+			mn = min(mn, static_cast<W>(__shfl_xor(mn, o, 64)));
+			mx = max(mx, static_cast<W>(__shfl_xor(mx, o, 64)));
+		}
+		__syncthreads();  // the previous item's last reads of the shared arrays are done
+		for (uint32_t i = t; i < NB; i += THREADS) s_cur[i] = 0;
+		if (t == 0) s_big = 0;
+		if (lane == 0) { s_min[w] = mn; s_max[w] = mx; }
+		__syncthreads();
+		mn = s_min[0];
+		mx = s_max[0];
+#pragma unroll
+		for (int v = 1; v < NW; ++v) { mn = min(mn, s_min[v]); mx = max(mx, s_max[v]); }
+		const W diff = mx - mn;
+		if (diff == 0) {
+			// every key is equal: the stable order is the order of reading
+#pragma unroll
+			for (int j = 0; j < KPT; ++j) {
+				const uint32_t l = w * 64u * KPT + j * 64u + lane;
+				if (l < len) dst[start + l] = key[j];
+			}
+			continue;
+		}
+		// highest bit of the key range; the partition digit is (key - min) >> sh, below NB
+		int hb;
+		if constexpr (sizeof(W) == 8) hb = 63 - __clzll(static_cast<long long>(diff));
+		else hb = 31 - __clz(static_cast<int>(diff));
+		const uint32_t sh = hb + 1 >= PBE ? static_cast<uint32_t>(hb + 1 - PBE) : 0u;
+#pragma unroll
+		for (int j = 0; j < KPT; ++j) {
+			const uint32_t l = w * 64u * KPT + j * 64u + lane;
+			if (l < len) atomicAdd(&s_cur[static_cast<uint32_t>((tk[j] - mn) >> sh)], 1u);
 		}
 		__syncthreads();
 		uint32_t c[PER], sum = 0, big = 0;
@@ -143,28 +205,23 @@ __global__ __launch_bounds__(THREADS) void k_item_sort(const K *__restrict__ src
 #pragma unroll
 		for (uint32_t k = 0; k < PER; ++k) {
 			const uint32_t b = t * PER + k;
-			if (b < NB) {
-				s_off[b] = static_cast<uint16_t>(run);
-				s_cur[b] = run;
-			}
+			if (b < NB) s_cur[b] = run;
 			run += c[k];
 		}
-		if (t == 0) s_off[NB] = static_cast<uint16_t>(len);
 		__syncthreads();
 		if (s_big) {
-			if (t == 0) queue[atomicAdd(qcount, 1u)] = it;
+			if (t == 0) queue[atomicAdd(qcount, 1u)] = qbase + it;  // an index into the whole item table
 			continue;
 		}
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) {
 			const uint32_t l = w * 64u * KPT + j * 64u + lane;
 			if (l < len) {
-				const auto tk = sort_key<K>(key[j], xf);
-				const uint32_t p = atomicAdd(&s_cur[static_cast<uint32_t>(tk >> (KB - PBE))], 1u);
+				const uint32_t p = atomicAdd(&s_cur[static_cast<uint32_t>((tk[j] - mn) >> sh)], 1u);
 				if constexpr (IW::packed) {
-					s_w[p] = (static_cast<W>(tk) << 16) | static_cast<W>(l);
+					s_w[p] = (tk[j] << 16) | static_cast<W>(l);
 				} else {
-					s_w[p] = static_cast<W>(tk);
+					s_w[p] = tk[j];
 					s_l[p] = static_cast<uint16_t>(l);
 				}
 			}
@@ -177,15 +234,18 @@ __global__ __launch_bounds__(THREADS) void k_item_sort(const K *__restrict__ src
 			at[j] = 0xFFFFFFFFu;
 			if (i < len) {
 				const W x = s_w[i];
-				uint32_t d, lx;
+				W kx;
+				uint32_t lx;
 				if constexpr (IW::packed) {
-					d = static_cast<uint32_t>(x >> (16 + KB - PBE));
+					kx = x >> 16;
 					lx = static_cast<uint32_t>(x) & 0xFFFFu;
 				} else {
-					d = static_cast<uint32_t>(x >> (KB - PBE));
+					kx = x;
 					lx = s_l[i];
 				}
-				const uint32_t lo = s_off[d], hi = s_off[d + 1];
+				// the cursors have advanced to the ends: partition d holds words [end of d - 1, end of d)
+				const uint32_t d = static_cast<uint32_t>((kx - mn) >> sh);
+				const uint32_t lo = d ? s_cur[d - 1] : 0u, hi = s_cur[d];
 				uint32_t n_less = 0;
 				if constexpr (IW::packed) {
 					for (uint32_t q = lo; q < hi; ++q) n_less += static_cast<uint32_t>(s_w[q] < x);
@@ -199,20 +259,48 @@ __global__ __launch_bounds__(THREADS) void k_item_sort(const K *__restrict__ src
 				at[j] = lx;
 			}
 		}
-		__syncthreads();  // the cursors are dead: s_u holds the final positions from here
-		uint16_t *s_fin = reinterpret_cast<uint16_t *>(s_u);
+		__syncthreads();  // the positions are read: s_l holds the final positions from here
 #pragma unroll
 		for (int j = 0; j < KPT; ++j)
-			if (at[j] != 0xFFFFFFFFu) s_fin[at[j]] = static_cast<uint16_t>(fin[j]);
+			if (at[j] != 0xFFFFFFFFu) s_l[at[j]] = static_cast<uint16_t>(fin[j]);
 		__syncthreads();
 		K *s_raw = reinterpret_cast<K *>(s_w);
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) {
 			const uint32_t l = w * 64u * KPT + j * 64u + lane;
-			if (l < len) s_raw[s_fin[l]] = key[j];
+			if (l < len) s_raw[s_l[l]] = key[j];
 		}
 		__syncthreads();
 		for (uint32_t i = t; i < len; i += THREADS) dst[start + i] = s_raw[i];
+	}
+}
+
+// Items of at most 64 keys, one wave each (no LDS, no block barrier): lane l holds the key read
+// at position l and counts the keys that precede it in the stable order.
+template <typename K, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_item_wave(const K *__restrict__ src, K *__restrict__ dst,
+	const uint64_t *__restrict__ istart, const uint32_t *__restrict__ ilen, uint32_t nitems, KeyXform xf, int reverse)
+{
+	using T = typename KeyTraits<K>::W;
+	constexpr int KB = KeyTraits<K>::bits;
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint32_t waves = gridDim.x * (THREADS / 64);
+	for (uint32_t it = blockIdx.x * (THREADS / 64) + (threadIdx.x >> 6); it < nitems; it += waves) {
+		const uint32_t len = ilen[it];
+		const uint64_t start = istart[it];
+		K key = K(0);
+		T k = 0;
+		if (lane < len) {
+			key = src[start + (reverse ? len - 1 - lane : lane)];
+			k = sort_key<K>(key, xf);
+			if constexpr (KB < 8 * static_cast<int>(sizeof(T))) k &= (T(1) << KB) - 1u;
+		}
+		uint32_t rank = 0;
+		for (uint32_t j = 0; j < len; ++j) {
+			const T y = __shfl(k, static_cast<int>(j), 64);
+			rank += static_cast<uint32_t>(y < k) | (static_cast<uint32_t>(y == k) & static_cast<uint32_t>(j < lane));
+		}
+		if (lane < len) dst[start + rank] = key;
 	}
 }
 
@@ -268,30 +356,22 @@ __global__ __launch_bounds__(THREADS) void k_item_lsd(const K *__restrict__ src,
 // ---------------------------------------------------------------------------------------------
 // long segments: one stable partition by the top digit
 
-// per-tile counts of the top digit (a tile is keys [i*tile_keys, (i+1)*tile_keys) of its segment in
-// logical order: a physical range read back to front under reverse)
+// per-tile counts of the top digit, one block per tile
 template <typename K, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_seg_count1(const K *__restrict__ in, const uint64_t *__restrict__ seg,
-	const uint32_t *__restrict__ large_ids, const uint2 *__restrict__ tiles, uint32_t ntiles, uint32_t tile_keys,
-	KeyXform xf, int reverse, uint32_t *__restrict__ counts)
+__global__ __launch_bounds__(THREADS) void k_seg_count1(const K *__restrict__ in, const SegTile *__restrict__ tiles,
+	uint32_t tile_keys, KeyXform xf, int reverse, uint32_t *__restrict__ counts)
 {
 	constexpr int P = sizeof(K);
 	__shared__ uint32_t s_t[256];
-	for (uint32_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
-		for (int i = threadIdx.x; i < 256; i += THREADS) s_t[i] = 0;
-		__syncthreads();
-		const uint2 tl = tiles[g];
-		const uint32_t s = large_ids[tl.x];
-		const uint64_t sb = seg[s];
-		const uint32_t len = static_cast<uint32_t>(seg[s + 1] - sb);
-		const uint32_t lo = tl.y * tile_keys, hi = min(len, lo + tile_keys);
-		const uint64_t plo = sb + (reverse ? len - hi : lo);
-		for (uint32_t i = threadIdx.x; i < hi - lo; i += THREADS)
-			wave_count(s_t, digit_of<K>(in[plo + i], xf, 8u * (P - 1)));
-		__syncthreads();
-		for (int i = threadIdx.x; i < 256; i += THREADS) counts[static_cast<uint64_t>(g) * 256 + i] = s_t[i];
-		__syncthreads();
-	}
+	const uint32_t g = blockIdx.x;
+	for (int i = threadIdx.x; i < 256; i += THREADS) s_t[i] = 0;
+	const SegTile tl = tiles[g];
+	const uint32_t lo = tl.y * tile_keys, hi = min(tl.len, lo + tile_keys);
+	__syncthreads();
+	for_each_key<K, THREADS>(in, tl.sb + (reverse ? tl.len - hi : lo), hi - lo,
+		[&](K x) { wave_count(s_t, digit_of<K>(x, xf, 8u * (P - 1))); });
+	__syncthreads();
+	for (int i = threadIdx.x; i < 256; i += THREADS) counts[static_cast<uint64_t>(g) * 256 + i] = s_t[i];
 }
 
 // One block per long segment, one thread per top digit: bucket sizes and starts, the segment's
@@ -302,7 +382,8 @@ __global__ __launch_bounds__(256) void k_seg_scan1(const uint64_t *__restrict__ 
 	uint32_t *__restrict__ ftiles, SegCtrl *ctrl, uint64_t *__restrict__ istart, uint32_t *__restrict__ ilen)
 {
 	__shared__ uint32_t s_wave[4];
-	__shared__ uint32_t s_tot[256];
+	__shared__ uint32_t s_pre[257];
+	__shared__ uint16_t s_next[256];
 	__shared__ uint32_t s_at;
 	const uint32_t L = blockIdx.x, d = threadIdx.x, lane = d & 63u, w = d >> 6;
 	const uint32_t s = large_ids[L];
@@ -316,7 +397,6 @@ __global__ __launch_bounds__(256) void k_seg_scan1(const uint64_t *__restrict__ 
 #pragma unroll
 		for (int j = 0; j < 8; ++j) tot += c[j];
 	}
-	s_tot[d] = tot;
 	const uint32_t incl = wave_inclusive_sum(tot);
 	if (lane == 63) s_wave[w] = incl;
 	if (__syncthreads_or(tot > cap)) {
@@ -332,46 +412,52 @@ __global__ __launch_bounds__(256) void k_seg_scan1(const uint64_t *__restrict__ 
 	}
 	uint32_t before = 0;
 	for (uint32_t v = 0; v < w; ++v) before += s_wave[v];
-	uint64_t run = sb + before + incl - tot;
+	const uint32_t excl = before + incl - tot;  // bucket start, relative to the segment start
+	s_pre[d] = excl;
+	if (d == 255) s_pre[256] = excl + tot;
+	uint32_t run = excl;
 	for (uint32_t tl = t0; tl < t1; tl += 8) {
 		uint32_t c[8];
 #pragma unroll
 		for (int j = 0; j < 8; ++j) c[j] = (tl + j < t1) ? counts[static_cast<uint64_t>(tl + j) * 256 + d] : 0u;
 #pragma unroll
 		for (int j = 0; j < 8; ++j) {
-			// offsets relative to the segment start (a segment holds at most 2^32 - 1 keys)
-			if (tl + j < t1) counts[static_cast<uint64_t>(tl + j) * 256 + d] = static_cast<uint32_t>(run - sb);
+			if (tl + j < t1) counts[static_cast<uint64_t>(tl + j) * 256 + d] = run;
 			run += c[j];
 		}
 	}
+	__syncthreads();
+	// The buckets are packed greedily, in digit order, into items of at most cap keys. An item that
+	// starts at bucket b ends before the first bucket e past the largest prefix within cap:
+	// s_next[b] = max{e : s_pre[e] - s_pre[b] <= cap} (> b, as every bucket fits on its own).
+	{
+		const uint32_t base = s_pre[d];
+		uint32_t e = d;
+#pragma unroll
+		for (uint32_t step = 256; step; step >>= 1)
+			if (e + step <= 256u && s_pre[e + step] - base <= cap) e += step;
+		s_next[d] = static_cast<uint16_t>(e);
+	}
+	__syncthreads();
 	if (d != 0) return;
-	// greedy packing of the buckets in digit order into items of at most cap keys
-	uint32_t nitems = 0, acc = 0;
-	for (int b = 0; b < 256; ++b) {
-		const uint32_t n = s_tot[b];
-		if (acc + n > cap) { ++nitems; acc = 0; }
-		acc += n;
-	}
-	if (acc) ++nitems;
+	const uint32_t total = s_pre[256];
+	uint32_t nitems = 0;
+	for (uint32_t b = 0; b < 256u && s_pre[b] < total; b = s_next[b]) ++nitems;
 	uint32_t it = atomicAdd(&ctrl->n_sub, nitems);
-	uint64_t first = sb;
-	acc = 0;
-	for (int b = 0; b < 256; ++b) {
-		const uint32_t n = s_tot[b];
-		if (acc + n > cap) {
-			istart[it] = first; ilen[it] = acc; ++it;
-			first += acc; acc = 0;
-		}
-		acc += n;
+	for (uint32_t b = 0; b < 256u && s_pre[b] < total; b = s_next[b], ++it) {
+		istart[it] = sb + s_pre[b];
+		ilen[it] = (s_next[b] < 256u ? s_pre[s_next[b]] : total) - s_pre[b];
 	}
-	if (acc) { istart[it] = first; ilen[it] = acc; }
 }
 
-// the stable partition of the long segments by the top digit, into the same ranges of dst
+// The stable partition of the long segments by the top digit, into the same ranges of dst.
+// Persistent blocks; blocks are dealt to the XCDs round robin, so block b works through the tile
+// range of XCD b % 8 (neighbouring tiles, whose digit runs are neighbours in dst, are written
+// through one L2), loading its next tile before it writes the current one.
 template <typename K, int THREADS, int KPT>
 __global__ __launch_bounds__(THREADS) void k_seg_scatter1(const K *__restrict__ in, K *__restrict__ dst,
-	const uint64_t *__restrict__ seg, const uint32_t *__restrict__ large_ids, const uint2 *__restrict__ tiles, uint32_t ntiles,
-	const uint32_t *__restrict__ offsets, const uint32_t *__restrict__ flag, KeyXform xf, int reverse)
+	const SegTile *__restrict__ tiles, uint32_t ntiles, const uint32_t *__restrict__ offsets,
+	const uint32_t *__restrict__ flag, KeyXform xf, int reverse)
 {
 	constexpr int TILE = THREADS * KPT;
 	constexpr int NW = THREADS / 64;
@@ -383,37 +469,53 @@ __global__ __launch_bounds__(THREADS) void k_seg_scatter1(const K *__restrict__ 
 	__shared__ uint32_t s_wsum[4];
 	__shared__ K s_keys[TILE];
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
-	for (uint32_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
-		const uint2 tl = tiles[g];
-		if (flag[tl.x]) continue;  // block-uniform: the segment takes the LSD passes
-		const uint32_t s = large_ids[tl.x];
-		const uint64_t sb = seg[s];
-		const uint32_t len = static_cast<uint32_t>(seg[s + 1] - sb);
+	const uint32_t per = (ntiles + 7) / 8;
+	const uint32_t x = blockIdx.x & 7u, step = gridDim.x >> 3;
+	uint32_t k = blockIdx.x >> 3;
+	uint32_t g = x * per + k;
+	bool have = k < per && g < ntiles;
+	SegTile tl{};
+	uint32_t valid = 0, skip = 0;
+	K keys[KPT];
+	auto fetch = [&](uint32_t gg) {
+		tl = tiles[gg];
 		const uint32_t base = tl.y * TILE;
-		const uint32_t valid = min(static_cast<uint32_t>(TILE), len - base);
-		K keys[KPT];
-		uint32_t dig[KPT], pos[KPT], total;
+		valid = min(static_cast<uint32_t>(TILE), tl.len - base);
+		skip = flag[tl.L];  // block-uniform: the segment takes the LSD passes
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) {
 			const uint32_t l = w * 64u * KPT + j * 64u + lane;
-			if (l < valid) {
-				const uint32_t r = base + l;
-				keys[j] = in[sb + (reverse ? len - 1 - r : r)];
-				dig[j] = digit_of<K>(keys[j], xf, kShift);
-			} else {
-				keys[j] = K(0);
-				dig[j] = 0xFFu;
-			}
+			const uint32_t r = base + l;
+			keys[j] = l < valid ? in[tl.sb + (reverse ? tl.len - 1 - r : r)] : K(0);
 		}
-		block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total);
-		if (t < 256) s_glob[t] = offsets[static_cast<uint64_t>(g) * 256 + t] - s_digoff[t];
+	};
+	if (have) fetch(g);
+	while (have) {
+		const uint64_t sb = tl.sb;
+		const uint32_t cur = g, cvalid = valid, cskip = skip;
+		if (!cskip) {
+			uint32_t dig[KPT], pos[KPT], total;
 #pragma unroll
-		for (int j = 0; j < KPT; ++j) s_keys[pos[j]] = keys[j];
+			for (int j = 0; j < KPT; ++j) {
+				const uint32_t l = w * 64u * KPT + j * 64u + lane;
+				dig[j] = l < cvalid ? digit_of<K>(keys[j], xf, kShift) : 0xFFu;
+			}
+			block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total);
+			if (t < 256) s_glob[t] = offsets[static_cast<uint64_t>(cur) * 256 + t] - s_digoff[t];
+#pragma unroll
+			for (int j = 0; j < KPT; ++j) s_keys[pos[j]] = keys[j];
+		}
+		k += step;
+		g = x * per + k;
+		have = k < per && g < ntiles;
+		if (have) fetch(g);  // the current keys are staged in LDS
 		__syncthreads();
-		K *dseg = dst + sb;
-		for (uint32_t i = t; i < valid; i += THREADS) {
-			const K k = s_keys[i];
-			dseg[s_glob[digit_of<K>(k, xf, kShift)] + i] = k;
+		if (!cskip) {
+			K *dseg = dst + sb;
+			for (uint32_t i = t; i < cvalid; i += THREADS) {
+				const K key = s_keys[i];
+				dseg[s_glob[digit_of<K>(key, xf, kShift)] + i] = key;
+			}
 		}
 		__syncthreads();
 	}
@@ -423,27 +525,24 @@ __global__ __launch_bounds__(THREADS) void k_seg_scatter1(const K *__restrict__ 
 // long segments with a bucket above the LDS capacity: LSD passes
 
 template <typename K, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_seg_hist(const K *__restrict__ in, const uint64_t *__restrict__ seg,
-	const uint32_t *__restrict__ large_ids, const uint2 *__restrict__ tiles, const uint32_t *__restrict__ ftiles,
-	const SegCtrl *ctrl, uint32_t tile_keys, KeyXform xf, uint32_t *__restrict__ hist)
+__global__ __launch_bounds__(THREADS) void k_seg_hist(const K *__restrict__ in, const SegTile *__restrict__ tiles,
+	const uint32_t *__restrict__ ftiles, const SegCtrl *ctrl, uint32_t tile_keys, KeyXform xf, uint32_t *__restrict__ hist)
 {
 	constexpr int P = sizeof(K);
 	__shared__ uint32_t s_h[P * 256];
 	const uint32_t nf = ctrl->n_ftiles;  // tiles of the segments flagged by k_seg_scan1
 	for (uint32_t q = blockIdx.x; q < nf; q += gridDim.x) {
-		const uint2 tl = tiles[ftiles[q]];
+		const SegTile tl = tiles[ftiles[q]];
 		for (int i = threadIdx.x; i < P * 256; i += THREADS) s_h[i] = 0;
 		__syncthreads();
-		const uint32_t s = large_ids[tl.x];
-		const uint64_t b = seg[s] + static_cast<uint64_t>(tl.y) * tile_keys;
-		const uint64_t e = min(seg[s + 1], b + tile_keys);
-		for (uint64_t i = b + threadIdx.x; i < e; i += THREADS) {
-			const auto k = sort_key<K>(in[i], xf);
+		const uint32_t lo = tl.y * tile_keys, hi = min(tl.len, lo + tile_keys);
+		for_each_key<K, THREADS>(in, tl.sb + lo, hi - lo, [&](K x) {
+			const auto k = sort_key<K>(x, xf);
 #pragma unroll
 			for (int p = 0; p < P; ++p) wave_count(s_h + p * 256, static_cast<uint32_t>(k >> (8 * p)) & 0xFFu);
-		}
+		});
 		__syncthreads();
-		uint32_t *h = hist + static_cast<uint64_t>(tl.x) * P * 256;
+		uint32_t *h = hist + static_cast<uint64_t>(tl.L) * P * 256;
 		for (int i = threadIdx.x; i < P * 256; i += THREADS)
 			if (s_h[i]) atomicAdd(&h[i], s_h[i]);
 		__syncthreads();
@@ -507,9 +606,8 @@ __global__ __launch_bounds__(256) void k_seg_plan(const uint64_t *__restrict__ s
 
 template <typename K, int THREADS, int KPT>
 __global__ __launch_bounds__(THREADS) void k_seg_pass(const K *__restrict__ in, K *__restrict__ out, K *__restrict__ buf,
-	const uint64_t *__restrict__ seg, const uint32_t *__restrict__ large_ids, const uint2 *__restrict__ tiles,
-	const uint32_t *__restrict__ ftiles, const uint32_t *__restrict__ bucket, const SegSel *__restrict__ sel,
-	uint64_t *__restrict__ lookback, SegCtrl *ctrl, KeyXform xf, int pass)
+	const SegTile *__restrict__ tiles, const uint32_t *__restrict__ ftiles, const uint32_t *__restrict__ bucket,
+	const SegSel *__restrict__ sel, uint64_t *__restrict__ lookback, SegCtrl *ctrl, KeyXform xf, int pass)
 {
 	constexpr int TILE = THREADS * KPT;
 	constexpr int NW = THREADS / 64;
@@ -529,8 +627,8 @@ __global__ __launch_bounds__(THREADS) void k_seg_pass(const K *__restrict__ in, 
 		__syncthreads();
 		const uint32_t g = s_tile;  // position in the flagged list: the look-back row of the tile
 		if (g >= num_tiles) break;
-		const uint2 tl = tiles[ftiles[g]];
-		const SegSel sl = sel[tl.x];
+		const SegTile tl = tiles[ftiles[g]];
+		const SegSel sl = sel[tl.L];
 		if (!(sl.mask >> pass & 1u)) {
 			__syncthreads();
 			continue;  // this segment skips the digit: none of its tiles takes part
@@ -539,9 +637,8 @@ __global__ __launch_bounds__(THREADS) void k_seg_pass(const K *__restrict__ in, 
 		const K *src = pick<K>(code & 3u, in, out, buf);
 		K *dst = pick<K>(code >> 2 & 3u, in, out, buf);
 		const bool rev = (sl.mask & kSelReverse) && ((sl.mask & ((1u << pass) - 1u)) == 0);  // first executed pass
-		const uint32_t s = large_ids[tl.x];
-		const uint64_t sb = seg[s];
-		const uint32_t len = static_cast<uint32_t>(seg[s + 1] - sb);
+		const uint64_t sb = tl.sb;
+		const uint32_t len = tl.len;
 		const uint32_t base = tl.y * TILE;
 		const uint32_t valid = min(static_cast<uint32_t>(TILE), len - base);
 		K keys[KPT];
@@ -570,7 +667,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_pass(const K *__restrict__ in, 
 				excl = lookback_window<16>(lookback + t, static_cast<int64_t>(g), static_cast<int64_t>(g) - tl.y, tag);
 				atomic_store_agent(&lb[t], lb_word(excl + pub, kLbInclusive, tag));
 			}
-			s_glob[t] = bucket[(static_cast<uint64_t>(tl.x) * P + pass) * 256 + t] + excl - s_digoff[t];
+			s_glob[t] = bucket[(static_cast<uint64_t>(tl.L) * P + pass) * 256 + t] + excl - s_digoff[t];
 		}
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) s_keys[pos[j]] = keys[j];
@@ -631,6 +728,15 @@ int occupancy_of(F kernel)
 
 } // namespace
 
+// tiles of the long segments' partition and LSD passes (threads x keys per thread), in the order of
+// the tile_cfg argument of the segmented plan
+constexpr int kNumSegTileCfgs = 5;
+constexpr TileCfg kSegTileCfgs[kNumSegTileCfgs] = {{256, 8}, {256, 16}, {512, 8}, {512, 16}, {256, 24}};
+// item sort capacity tiers (256x4, 256x16, 512x16)
+constexpr int kNumSmallCaps = 3;
+constexpr int kSmallCaps[kNumSmallCaps] = {256 * 4, 256 * 16, 512 * 16};
+constexpr uint32_t kWaveItemMax = 64;  // short segments up to this length: one wave each
+
 struct SegPlan {
 	int key_bytes = 4;
 	unsigned flags = 0;
@@ -638,6 +744,7 @@ struct SegPlan {
 	int tile_cfg = 1;    // tile of the long segments' partition and LSD passes (kSegTileCfgs)
 	int small_cfg = 1;   // LDS capacity tier of the item sort (kSmallCaps)
 	uint32_t nsmall = 0, nlarge = 0, ntiles = 0, tile_keys = 0, small_cap = 0;
+	uint32_t tier_count[3] = {0, 0, 0};  // short segments per capacity tier (the rest: at most 64 keys)
 	char *mem = nullptr;  // one device allocation for every table and the workspace
 	size_t mem_bytes = 0;
 	SegCtrl *ctrl = nullptr;
@@ -645,7 +752,7 @@ struct SegPlan {
 	uint32_t *small_len = nullptr, *sub_len = nullptr, *q_small = nullptr, *q_sub = nullptr;
 	uint32_t *large_ids = nullptr, *tile_first = nullptr, *counts1 = nullptr, *flag = nullptr, *ftiles = nullptr;
 	uint32_t *hist = nullptr, *bucket = nullptr;
-	uint2 *tiles = nullptr;
+	SegTile *tiles = nullptr;
 	SegSel *sel = nullptr;
 	uint64_t *lookback = nullptr;
 	hipGraph_t graph = nullptr;
@@ -654,61 +761,93 @@ struct SegPlan {
 	hipEvent_t fork = nullptr, join = nullptr;
 };
 
-// tiles of the long segments' partition and LSD passes (threads x keys per thread), in the order of
-// the tile_cfg argument of the segmented plan
-constexpr int kNumSegTileCfgs = 5;
-constexpr TileCfg kSegTileCfgs[kNumSegTileCfgs] = {{256, 8}, {256, 16}, {512, 8}, {512, 16}, {256, 24}};
-// item sort capacities (256x4, 256x16, 512x16) and partition widths (about two keys per run)
-constexpr int kNumSmallCaps = 3;
-constexpr int kSmallCaps[kNumSmallCaps] = {256 * 4, 256 * 16, 512 * 16};
 
 namespace {
 
-// item sort + queued LSD items for one capacity tier; count null means count_value items
+// One capacity tier of the item sort over items [0, count) of the given table (count null: count_value
+// items); a long-run item is queued as qbase + its index. 1-byte keys take one LDS LSD pass instead.
 template <typename K, int THREADS, int KPT, int PB>
-hipError_t launch_items_cfg(const void *src, void *dst, const uint64_t *istart, const uint32_t *ilen, const uint32_t *count,
-	uint32_t count_value, const KeyXform &xf, int reverse, uint32_t *queue, uint32_t *qcount, hipStream_t s)
+hipError_t launch_sort_t(const void *src, void *dst, const uint64_t *istart, const uint32_t *ilen, const uint32_t *count,
+	uint32_t count_value, const KeyXform &xf, int reverse, uint32_t *queue, uint32_t *qcount, uint32_t qbase, hipStream_t s)
 {
-	static const int occ = occupancy_of<THREADS, KPT>(k_item_sort<K, THREADS, KPT, PB>);
-	static const int occ_lsd = occupancy_of<THREADS, KPT>(k_item_lsd<K, THREADS, KPT>);
 	const uint32_t cus = static_cast<uint32_t>(device_cu_count());
-	hipError_t e;
 	if constexpr (sizeof(K) == 1) {
-		// one LSD pass ranks 8-bit keys stably by the whole key: every item goes there
-		uint32_t grid = static_cast<uint32_t>(occ_lsd) * cus;
-		if (!count) grid = std::min<uint32_t>(grid, std::max<uint32_t>(1u, count_value));
+		static const int occ = occupancy_of<THREADS, KPT>(k_item_lsd<K, THREADS, KPT>);
+		const uint32_t full = static_cast<uint32_t>(occ) * cus;
+		const uint32_t grid = count ? full : std::min<uint32_t>(full, std::max<uint32_t>(1u, count_value));
 		hipLaunchKernelGGL((k_item_lsd<K, THREADS, KPT>), dim3(grid), dim3(THREADS), 0, s, static_cast<const K *>(src),
 			static_cast<K *>(dst), istart, ilen, nullptr, count, count_value, xf, reverse);
-		return hipGetLastError();
+	} else {
+		static const int occ = occupancy_of<THREADS, KPT>(k_item_sort<K, THREADS, KPT, PB>);
+		const uint32_t full = static_cast<uint32_t>(occ) * cus;
+		const uint32_t grid = count ? full : std::min<uint32_t>(full, std::max<uint32_t>(1u, count_value));
+		hipLaunchKernelGGL((k_item_sort<K, THREADS, KPT, PB>), dim3(grid), dim3(THREADS), 0, s, static_cast<const K *>(src),
+			static_cast<K *>(dst), istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase);
 	}
-	uint32_t grid = static_cast<uint32_t>(occ) * cus;
-	if (!count) grid = std::min<uint32_t>(grid, std::max<uint32_t>(1u, count_value));
-	hipLaunchKernelGGL((k_item_sort<K, THREADS, KPT, PB>), dim3(grid), dim3(THREADS), 0, s, static_cast<const K *>(src),
-		static_cast<K *>(dst), istart, ilen, count, count_value, xf, reverse, queue, qcount);
-	if ((e = hipGetLastError()) != hipSuccess) return e;
-	hipLaunchKernelGGL((k_item_lsd<K, THREADS, KPT>), dim3(static_cast<uint32_t>(occ_lsd) * cus), dim3(THREADS), 0, s,
-		static_cast<const K *>(src), static_cast<K *>(dst), istart, ilen, queue, qcount, 0u, xf, reverse);
 	return hipGetLastError();
 }
 
-template <typename K>
-hipError_t launch_items(const SegPlan &pl, const void *src, void *dst, const uint64_t *istart, const uint32_t *ilen,
-	const uint32_t *count, uint32_t count_value, const KeyXform &xf, int reverse, uint32_t *queue, uint32_t *qcount,
-	hipStream_t s)
+// the queued items of a table (no queue for 1-byte keys)
+template <typename K, int THREADS, int KPT>
+hipError_t launch_lsd_t(const void *src, void *dst, const uint64_t *istart, const uint32_t *ilen, const uint32_t *queue,
+	const uint32_t *qcount, const KeyXform &xf, int reverse, hipStream_t s)
 {
-	switch (pl.small_cfg) {
-	case 0: return launch_items_cfg<K, 256, 4, 9>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, s);
-	case 1: return launch_items_cfg<K, 256, 16, 10>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, s);
-	default: return launch_items_cfg<K, 512, 16, 11>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, s);
+	if constexpr (sizeof(K) == 1) return hipSuccess;
+	static const int occ = occupancy_of<THREADS, KPT>(k_item_lsd<K, THREADS, KPT>);
+	hipLaunchKernelGGL((k_item_lsd<K, THREADS, KPT>), dim3(static_cast<uint32_t>(occ) * static_cast<uint32_t>(device_cu_count())),
+		dim3(THREADS), 0, s, static_cast<const K *>(src), static_cast<K *>(dst), istart, ilen, queue, qcount, 0u, xf, reverse);
+	return hipGetLastError();
+}
+
+// capacity tiers: 0 = 256x4 (1024 keys), 1 = 256x16 (4096), 2 = 512x16 (8192), partition bits chosen
+// for about four keys per partition
+template <typename K>
+hipError_t launch_sort_cfg(int cfg, const void *src, void *dst, const uint64_t *istart, const uint32_t *ilen,
+	const uint32_t *count, uint32_t count_value, const KeyXform &xf, int reverse, uint32_t *queue, uint32_t *qcount,
+	uint32_t qbase, hipStream_t s)
+{
+	switch (cfg) {
+	case 0: return launch_sort_t<K, 256, 4, 8>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase, s);
+	case 1: return launch_sort_t<K, 256, 16, 10>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase, s);
+	default: return launch_sort_t<K, 512, 16, 11>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase, s);
 	}
 }
 
+template <typename K>
+hipError_t launch_lsd_cfg(int cfg, const void *src, void *dst, const uint64_t *istart, const uint32_t *ilen,
+	const uint32_t *queue, const uint32_t *qcount, const KeyXform &xf, int reverse, hipStream_t s)
+{
+	switch (cfg) {
+	case 0: return launch_lsd_t<K, 256, 4>(src, dst, istart, ilen, queue, qcount, xf, reverse, s);
+	case 1: return launch_lsd_t<K, 256, 16>(src, dst, istart, ilen, queue, qcount, xf, reverse, s);
+	default: return launch_lsd_t<K, 512, 16>(src, dst, istart, ilen, queue, qcount, xf, reverse, s);
+	}
+}
+
+// The short segments, longest first: one launch per capacity tier down to 65 keys, one wave per
+// segment of at most 64 keys, then the queued long-run segments.
 template <typename K>
 hipError_t launch_small(SegPlan &pl, const void *in, void *out, const KeyXform &xf, hipStream_t s)
 {
 	if (!pl.nsmall) return hipSuccess;
-	return launch_items<K>(pl, in, out, pl.small_start, pl.small_len, nullptr, pl.nsmall, xf, (pl.flags & kRev) ? 1 : 0,
-		pl.q_small, &pl.ctrl->q_small, s);
+	const int rev = (pl.flags & kRev) ? 1 : 0;
+	hipError_t e;
+	uint32_t off = 0;
+	for (int cfg = kNumSmallCaps - 1; cfg >= 0; --cfg) {
+		const uint32_t n = pl.tier_count[cfg];
+		if (!n) continue;
+		if ((e = launch_sort_cfg<K>(cfg, in, out, pl.small_start + off, pl.small_len + off, nullptr, n, xf, rev, pl.q_small,
+			&pl.ctrl->q_small, off, s)) != hipSuccess) return e;
+		off += n;
+	}
+	if (off < pl.nsmall) {
+		const uint32_t n = pl.nsmall - off;
+		const uint32_t grid = std::min<uint32_t>((n + 3) / 4, 8u * static_cast<uint32_t>(device_cu_count()));
+		hipLaunchKernelGGL((k_item_wave<K, 256>), dim3(grid), dim3(256), 0, s, static_cast<const K *>(in), static_cast<K *>(out),
+			pl.small_start + off, pl.small_len + off, n, xf, rev);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+	}
+	return launch_lsd_cfg<K>(pl.small_cfg, in, out, pl.small_start, pl.small_len, pl.q_small, &pl.ctrl->q_small, xf, rev, s);
 }
 
 template <typename K, int THREADS, int KPT>
@@ -718,8 +857,7 @@ hipError_t launch_seg_pass(SegPlan &pl, const void *in, void *out, void *buf, co
 	static int occ = occupancy_of<THREADS, KPT>(k_seg_pass<K, THREADS, KPT>);
 	const uint32_t grid = std::min<uint32_t>(pl.ntiles, static_cast<uint32_t>(occ * device_cu_count()));
 	hipLaunchKernelGGL((k_seg_pass<K, THREADS, KPT>), dim3(grid), dim3(THREADS), 0, s, static_cast<const K *>(in),
-		static_cast<K *>(out), static_cast<K *>(buf), seg, pl.large_ids, pl.tiles, pl.ftiles, pl.bucket, pl.sel, pl.lookback,
-		pl.ctrl, xf, pass);
+		static_cast<K *>(out), static_cast<K *>(buf), pl.tiles, pl.ftiles, pl.bucket, pl.sel, pl.lookback, pl.ctrl, xf, pass);
 	return hipGetLastError();
 }
 
@@ -728,9 +866,9 @@ hipError_t launch_scatter1(SegPlan &pl, const void *in, void *buf, const uint64_
 	hipStream_t s)
 {
 	static int occ = occupancy_of<THREADS, KPT>(k_seg_scatter1<K, THREADS, KPT>);
-	const uint32_t grid = std::min<uint32_t>(pl.ntiles, static_cast<uint32_t>(occ * device_cu_count()));
+	const uint32_t grid = 8u * std::min<uint32_t>((pl.ntiles + 7) / 8, static_cast<uint32_t>(occ * device_cu_count()) / 8u);
 	hipLaunchKernelGGL((k_seg_scatter1<K, THREADS, KPT>), dim3(grid), dim3(THREADS), 0, s, static_cast<const K *>(in),
-		static_cast<K *>(buf), seg, pl.large_ids, pl.tiles, pl.ntiles, pl.counts1, pl.flag, xf, reverse);
+		static_cast<K *>(buf), pl.tiles, pl.ntiles, pl.counts1, pl.flag, xf, reverse);
 	return hipGetLastError();
 }
 
@@ -743,8 +881,8 @@ hipError_t launch_large(SegPlan &pl, const void *in, void *out, void *buf, const
 	const uint32_t cus = static_cast<uint32_t>(device_cu_count());
 	hipError_t e;
 	// the partition by the top digit, into buf, then the items of its buckets into out
-	hipLaunchKernelGGL((k_seg_count1<K, 256>), dim3(std::min<uint32_t>(pl.ntiles, 8u * cus)), dim3(256), 0, s,
-		static_cast<const K *>(in), seg, pl.large_ids, pl.tiles, pl.ntiles, pl.tile_keys, xf, rev, pl.counts1);
+	hipLaunchKernelGGL((k_seg_count1<K, 256>), dim3(pl.ntiles), dim3(256), 0, s, static_cast<const K *>(in), pl.tiles,
+		pl.tile_keys, xf, rev, pl.counts1);
 	if ((e = hipGetLastError()) != hipSuccess) return e;
 	hipLaunchKernelGGL(k_seg_scan1, dim3(pl.nlarge), dim3(256), 0, s, seg, pl.large_ids, pl.tile_first, pl.counts1,
 		pl.small_cap, pl.flag, pl.ftiles, pl.ctrl, pl.sub_start, pl.sub_len);
@@ -757,11 +895,13 @@ hipError_t launch_large(SegPlan &pl, const void *in, void *out, void *buf, const
 	default: e = launch_scatter1<K, 256, 24>(pl, in, buf, seg, xf, rev, s); break;
 	}
 	if (e != hipSuccess) return e;
-	if ((e = launch_items<K>(pl, buf, out, pl.sub_start, pl.sub_len, &pl.ctrl->n_sub, 0, xf, 0, pl.q_sub, &pl.ctrl->q_sub,
-		s)) != hipSuccess) return e;
+	if ((e = launch_sort_cfg<K>(pl.small_cfg, buf, out, pl.sub_start, pl.sub_len, &pl.ctrl->n_sub, 0, xf, 0, pl.q_sub,
+		&pl.ctrl->q_sub, 0, s)) != hipSuccess) return e;
+	if ((e = launch_lsd_cfg<K>(pl.small_cfg, buf, out, pl.sub_start, pl.sub_len, pl.q_sub, &pl.ctrl->q_sub, xf, 0, s)) !=
+		hipSuccess) return e;
 	// segments with a bucket above the LDS capacity: the LSD passes (every launch skips the others)
 	hipLaunchKernelGGL((k_seg_hist<K, 256>), dim3(std::min<uint32_t>(pl.ntiles, 8u * cus)), dim3(256), 0, s,
-		static_cast<const K *>(in), seg, pl.large_ids, pl.tiles, pl.ftiles, pl.ctrl, pl.tile_keys, xf, pl.hist);
+		static_cast<const K *>(in), pl.tiles, pl.ftiles, pl.ctrl, pl.tile_keys, xf, pl.hist);
 	if ((e = hipGetLastError()) != hipSuccess) return e;
 	hipLaunchKernelGGL((k_seg_plan<P>), dim3(pl.nlarge), dim3(256), 0, s, seg, pl.large_ids, pl.hist, pl.bucket, pl.sel, rev,
 		pl.flag);
@@ -817,7 +957,7 @@ hipError_t segplan_create(int key_bytes, unsigned flags, uint64_t nseg, const ui
 	pl->total = nseg ? host_off[nseg] - host_off[0] : 0;
 	std::vector<std::pair<uint64_t, uint32_t>> small;
 	std::vector<uint32_t> large, tile_first;
-	std::vector<uint2> tiles;
+	std::vector<SegTile> tiles;
 	for (uint64_t s = 0; s < nseg; ++s) {
 		if (host_off[s + 1] < host_off[s] || host_off[s + 1] - host_off[s] > 0xFFFFFFFFull) { delete pl; return hipErrorInvalidValue; }
 		const uint64_t len = host_off[s + 1] - host_off[s];
@@ -829,7 +969,7 @@ hipError_t segplan_create(int key_bytes, unsigned flags, uint64_t nseg, const ui
 			large.push_back(static_cast<uint32_t>(s));
 			tile_first.push_back(static_cast<uint32_t>(tiles.size()));
 			const uint32_t nt = static_cast<uint32_t>((len + pl->tile_keys - 1) / pl->tile_keys);
-			for (uint32_t i = 0; i < nt; ++i) tiles.push_back(make_uint2(L, i));
+			for (uint32_t i = 0; i < nt; ++i) tiles.push_back(SegTile{host_off[s], static_cast<uint32_t>(len), L, i, {0, 0, 0}});
 		}
 	}
 	tile_first.push_back(static_cast<uint32_t>(tiles.size()));
@@ -842,6 +982,12 @@ hipError_t segplan_create(int key_bytes, unsigned flags, uint64_t nseg, const ui
 		small_len[i] = static_cast<uint32_t>(small[i].first);
 	}
 	pl->nsmall = static_cast<uint32_t>(small.size());
+	for (const auto &it : small) {
+		if (it.first <= kWaveItemMax) continue;
+		int cfg = 0;
+		while (cfg < small_cfg && it.first > static_cast<uint64_t>(kSmallCaps[cfg])) ++cfg;
+		++pl->tier_count[cfg];
+	}
 	pl->nlarge = static_cast<uint32_t>(large.size());
 	pl->ntiles = static_cast<uint32_t>(tiles.size());
 	const int P = key_bytes;
@@ -855,7 +1001,7 @@ hipError_t segplan_create(int key_bytes, unsigned flags, uint64_t nseg, const ui
 	const size_t o_tfirst = off; off += up256(tile_first.size() * 4 + 4);
 	const size_t o_flag = off; off += up256(large.size() * 4 + 4);
 	const size_t o_ftiles = off; off += up256(tiles.size() * 4 + 4);
-	const size_t o_tiles = off; off += up256(tiles.size() * sizeof(uint2) + 8);
+	const size_t o_tiles = off; off += up256(tiles.size() * sizeof(SegTile) + 8);
 	const size_t o_counts = off; off += up256(tiles.size() * 256 * 4 + 4);
 	const size_t o_substart = off; off += up256(nsub * 8 + 8);
 	const size_t o_sublen = off; off += up256(nsub * 4 + 4);
@@ -875,7 +1021,7 @@ hipError_t segplan_create(int key_bytes, unsigned flags, uint64_t nseg, const ui
 	pl->tile_first = reinterpret_cast<uint32_t *>(pl->mem + o_tfirst);
 	pl->flag = reinterpret_cast<uint32_t *>(pl->mem + o_flag);
 	pl->ftiles = reinterpret_cast<uint32_t *>(pl->mem + o_ftiles);
-	pl->tiles = reinterpret_cast<uint2 *>(pl->mem + o_tiles);
+	pl->tiles = reinterpret_cast<SegTile *>(pl->mem + o_tiles);
 	pl->counts1 = reinterpret_cast<uint32_t *>(pl->mem + o_counts);
 	pl->sub_start = reinterpret_cast<uint64_t *>(pl->mem + o_substart);
 	pl->sub_len = reinterpret_cast<uint32_t *>(pl->mem + o_sublen);
@@ -893,7 +1039,7 @@ hipError_t segplan_create(int key_bytes, unsigned flags, uint64_t nseg, const ui
 	put(pl->small_len, small_len.data(), small_len.size() * 4);
 	put(pl->large_ids, large.data(), large.size() * 4);
 	put(pl->tile_first, tile_first.data(), tile_first.size() * 4);
-	put(pl->tiles, tiles.data(), tiles.size() * sizeof(uint2));
+	put(pl->tiles, tiles.data(), tiles.size() * sizeof(SegTile));
 	if (e == hipSuccess) e = hipStreamCreateWithFlags(&pl->side, hipStreamNonBlocking);
 	if (e == hipSuccess) e = hipEventCreateWithFlags(&pl->fork, hipEventDisableTiming);
 	if (e == hipSuccess) e = hipEventCreateWithFlags(&pl->join, hipEventDisableTiming);

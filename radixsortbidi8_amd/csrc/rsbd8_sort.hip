@@ -78,39 +78,6 @@ __device__ __forceinline__ uint32_t presort_result(uint32_t check, int reverse)
 // ---------------------------------------------------------------------------------------------
 // histogram of every digit in one read
 
-// Calls f(key) for every key of the physical range [lo, lo + len) of p, with the keys of a block
-// spread over its threads. 16-byte vector loads (four in flight per thread) when the range is
-// 16-byte aligned; the order in which a thread sees its keys is unspecified (counting only).
-template <typename K, int THREADS, typename F>
-__device__ __forceinline__ void for_each_key(const K *__restrict__ p, uint64_t lo, uint32_t len, F &&f)
-{
-	constexpr uint32_t VEC = 16 / sizeof(K);
-	const K *q = p + lo;
-	uint32_t done = 0;
-	if ((reinterpret_cast<uintptr_t>(q) & 15u) == 0) {
-		const uint32_t nvec = len / VEC;
-		const uint4 *qv = reinterpret_cast<const uint4 *>(q);
-		for (uint32_t v = threadIdx.x; v < nvec; v += THREADS * 4) {
-			uint4 u[4];
-#pragma unroll
-			for (int jj = 0; jj < 4; ++jj) {
-				const uint32_t idx = v + jj * THREADS;
-				if (idx < nvec) u[jj] = qv[idx];
-			}
-#pragma unroll
-			for (int jj = 0; jj < 4; ++jj) {
-				if (v + jj * THREADS >= nvec) break;
-				K ks[VEC];
-				__builtin_memcpy(ks, &u[jj], 16);
-#pragma unroll
-				for (uint32_t e = 0; e < VEC; ++e) f(ks[e]);
-			}
-		}
-		done = nvec * VEC;
-	}
-	for (uint32_t i = done + threadIdx.x; i < len; i += THREADS) f(q[i]);
-}
-
 template <typename K, int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_hist(const K *__restrict__ in, uint64_t n, KeyXform xf,
 	uint32_t *__restrict__ hist, const Control *ctrl, int reverse)

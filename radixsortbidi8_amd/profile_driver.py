@@ -2,10 +2,11 @@
 
   python -m radixsortbidi8_amd.profile_driver [--n-log2 28] [--dtype u64] [--reps 2] [--algo A]
                                                [--tile-config C] [--copy]
+                                               [--segmented [--small-cfg S] [--sweep]]
 
 Sorts 2^n seeded uniform keys (the benchmark's distribution) reps times after one warm-up sort,
-or (--segmented) sorts a batch of 65,536 int32 segments with log-uniform lengths totalling 2^n keys,
-then, with --copy, copies the same number of bytes device to device reps times: a plain copy of
+or (--segmented) sorts a batch of 65,536 int32 segments with log-uniform lengths totalling 2^n keys
+(plan tile configuration --tile-config, LDS tier --small-cfg; --sweep times every pair), then, with --copy, copies the same number of bytes device to device reps times: a plain copy of
 known size that calibrates the HBM byte counters (FETCH_SIZE, WRITE_SIZE) against the bytes it
 must move.
 """
@@ -23,6 +24,8 @@ def main():
     ap.add_argument("--copy", action="store_true")
     ap.add_argument("--segmented", action="store_true",
                     help="instead: 65,536 int32 segments, log-uniform lengths scaled to 2^n keys in total")
+    ap.add_argument("--small-cfg", type=int, default=-1, help="segmented: LDS tier of the plan (-1 automatic)")
+    ap.add_argument("--sweep", action="store_true", help="segmented: time every tile configuration and LDS tier")
     args = ap.parse_args()
     import torch
 
@@ -69,22 +72,26 @@ def segmented(args, torch, r):
     keys = torch.randint(-(1 << 31), (1 << 31) - 1, (total,), dtype=torch.int32, device="cuda",
                          generator=torch.Generator(device="cuda").manual_seed(0x5EED0005))
     out, buf = torch.empty_like(keys), torch.empty_like(keys)
-    plan = r.SegmentedPlan(np.int32, off)
-    print(f"profile_driver: segmented plan {plan.info()}", flush=True)
-    plan.run(keys, out, buf)
-    plan.capture(keys, out, buf)
-    torch.cuda.synchronize()
-    for name, fn in (("run", lambda: plan.run(keys, out, buf)), ("graph", lambda: plan.launch())):
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
-        for _ in range(args.reps):
-            fn()
-        b.record()
-        b.synchronize()
-        ms = a.elapsed_time(b) / args.reps
-        print(f"profile_driver: segmented {name}: {ms:.3f} ms per batch, {total / ms / 1e6:.2f} GKeys/s "
-              f"({nseg} segments, {total} keys)", flush=True)
-    plan.close()
+    pairs = [(args.tile_config, args.small_cfg)]
+    if args.sweep:
+        pairs = [(tc, sc) for sc in range(len(r.SMALL_CAPS)) for tc in range(len(r.SEGMENTED_TILE_CONFIGS))]
+    for tc, sc in pairs:
+        plan = r.SegmentedPlan(np.int32, off, tile_cfg=tc, small_cfg=sc)
+        print(f"profile_driver: segmented plan tile_cfg {tc} small_cfg {sc} {plan.info()}", flush=True)
+        plan.run(keys, out, buf)
+        plan.capture(keys, out, buf)
+        torch.cuda.synchronize()
+        for name, fn in (("run", lambda: plan.run(keys, out, buf)), ("graph", lambda: plan.launch())):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(args.reps):
+                fn()
+            b.record()
+            b.synchronize()
+            ms = a.elapsed_time(b) / args.reps
+            print(f"profile_driver: segmented {name}: {ms:.3f} ms per batch, {total / ms / 1e6:.2f} GKeys/s "
+                  f"({nseg} segments, {total} keys; tile_cfg {tc} small_cfg {sc})", flush=True)
+        plan.close()
     return 0
 
 
