@@ -104,8 +104,10 @@ __device__ __forceinline__ typename ItemWord<K>::W item_key(K raw, const KeyXfor
 // partition that precede it in the stable order (smaller key, or equal key read earlier), counted
 // with independent LDS reads; the raw keys are staged in LDS at their final positions. An item
 // whose keys are all equal is copied in order. Items with a partition of more than kItemRunMax
-// keys are queued for k_item_lsd.
-template <typename K, int THREADS, int KPT, int PB>
+// keys are queued for k_item_lsd. With INJ (the transform only complements bits, so equal sort keys
+// are equal keys and a key is its sort key xor E) the order among equal keys is unobservable: ties
+// are broken by the index in the partitioned array, and the sorted words are turned back into keys.
+template <typename K, int THREADS, int KPT, int PB, bool INJ>
 __global__ __launch_bounds__(THREADS) void k_item_sort(const K *__restrict__ src, K *__restrict__ dst,
 	const uint64_t *__restrict__ istart, const uint32_t *__restrict__ ilen, const uint32_t *count, uint32_t count_value,
 	KeyXform xf, int reverse, uint32_t *__restrict__ queue, uint32_t *qcount, uint32_t qbase)
@@ -121,7 +123,8 @@ __global__ __launch_bounds__(THREADS) void k_item_sort(const K *__restrict__ src
 	static_assert(CAP <= 65536, "positions are held in 16 bits");
 	static_assert(sizeof(K) <= sizeof(W), "the raw keys are staged in the word array");
 	__shared__ __align__(16) W s_w[CAP];     // words, then the raw keys in final order
-	__shared__ uint16_t s_l[CAP];            // positions of unpacked words, then each key's final position
+	__shared__ uint16_t s_l[INJ ? 1 : CAP];  // positions of unpacked words, then each key's final position
+	__shared__ __align__(16) W s_o[INJ ? CAP : 1];  // with INJ: the sorted words
 	__shared__ uint32_t s_cur[NB];           // partition counters, then cursors (after the scatter: ends)
 	__shared__ uint32_t s_wsum[NW];
 	__shared__ W s_min[NW], s_max[NW];
@@ -211,6 +214,29 @@ __global__ __launch_bounds__(THREADS) void k_item_sort(const K *__restrict__ src
 		__syncthreads();
 		if (s_big) {
 			if (t == 0) queue[atomicAdd(qcount, 1u)] = qbase + it;  // an index into the whole item table
+			continue;
+		}
+		if constexpr (INJ) {
+#pragma unroll
+			for (int j = 0; j < KPT; ++j) {
+				const uint32_t l = w * 64u * KPT + j * 64u + lane;
+				if (l < len) s_w[atomicAdd(&s_cur[static_cast<uint32_t>((tk[j] - mn) >> sh)], 1u)] = tk[j];
+			}
+			__syncthreads();
+			for (uint32_t i = t; i < len; i += THREADS) {
+				const W x = s_w[i];
+				const uint32_t d = static_cast<uint32_t>((x - mn) >> sh);
+				const uint32_t lo = d ? s_cur[d - 1] : 0u, hi = s_cur[d];
+				uint32_t n_less = 0;
+				for (uint32_t q = lo; q < hi; ++q) {
+					const W y = s_w[q];
+					n_less += static_cast<uint32_t>(y < x) | (static_cast<uint32_t>(y == x) & static_cast<uint32_t>(q < i));
+				}
+				s_o[lo + n_less] = x;
+			}
+			__syncthreads();
+			const W e = static_cast<W>(xf.E);
+			for (uint32_t i = t; i < len; i += THREADS) dst[start + i] = static_cast<K>(s_o[i] ^ e);
 			continue;
 		}
 #pragma unroll
@@ -450,11 +476,15 @@ __global__ __launch_bounds__(256) void k_seg_scan1(const uint64_t *__restrict__ 
 	}
 }
 
-// The stable partition of the long segments by the top digit, into the same ranges of dst.
-// Persistent blocks; blocks are dealt to the XCDs round robin, so block b works through the tile
-// range of XCD b % 8 (neighbouring tiles, whose digit runs are neighbours in dst, are written
-// through one L2), loading its next tile before it writes the current one.
-template <typename K, int THREADS, int KPT>
+// The partition of the long segments by the top digit, into the same ranges of dst. Persistent
+// blocks; blocks are dealt to the XCDs round robin, so block b works through the tile range of XCD
+// b % 8 (neighbouring tiles, whose digit runs are neighbours in dst, are written through one L2),
+// loading its next tile before it writes the current one. With STABLE the keys of a digit keep
+// their order (block ranking); without, a key's place within its digit's run of the tile comes from
+// an LDS atomic, which is exact only up to the order of equal sort keys. That is enough when the key
+// transform is injective (equal sort keys are equal keys), because the item sort that follows
+// orders every bucket by the whole sort key.
+template <typename K, int THREADS, int KPT, bool STABLE>
 __global__ __launch_bounds__(THREADS) void k_seg_scatter1(const K *__restrict__ in, K *__restrict__ dst,
 	const SegTile *__restrict__ tiles, uint32_t ntiles, const uint32_t *__restrict__ offsets,
 	const uint32_t *__restrict__ flag, KeyXform xf, int reverse)
@@ -469,6 +499,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_scatter1(const K *__restrict__ 
 	__shared__ uint32_t s_wsum[4];
 	__shared__ K s_keys[TILE];
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+	(void)lane;
 	const uint32_t per = (ntiles + 7) / 8;
 	const uint32_t x = blockIdx.x & 7u, step = gridDim.x >> 3;
 	uint32_t k = blockIdx.x >> 3;
@@ -494,16 +525,49 @@ __global__ __launch_bounds__(THREADS) void k_seg_scatter1(const K *__restrict__ 
 		const uint64_t sb = tl.sb;
 		const uint32_t cur = g, cvalid = valid, cskip = skip;
 		if (!cskip) {
-			uint32_t dig[KPT], pos[KPT], total;
+			uint32_t dig[KPT];
 #pragma unroll
 			for (int j = 0; j < KPT; ++j) {
 				const uint32_t l = w * 64u * KPT + j * 64u + lane;
 				dig[j] = l < cvalid ? digit_of<K>(keys[j], xf, kShift) : 0xFFu;
 			}
-			block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total);
-			if (t < 256) s_glob[t] = offsets[static_cast<uint64_t>(cur) * 256 + t] - s_digoff[t];
+			if constexpr (STABLE) {
+				uint32_t pos[KPT], total;
+				block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total);
+				if (t < 256) s_glob[t] = offsets[static_cast<uint64_t>(cur) * 256 + t] - s_digoff[t];
 #pragma unroll
-			for (int j = 0; j < KPT; ++j) s_keys[pos[j]] = keys[j];
+				for (int j = 0; j < KPT; ++j) s_keys[pos[j]] = keys[j];
+			} else {
+				uint32_t *s_cnt = s_whist;  // 256 counters, then cursors
+				for (uint32_t i = t; i < 256; i += THREADS) s_cnt[i] = 0;
+				__syncthreads();
+#pragma unroll
+				for (int j = 0; j < KPT; ++j) {
+					const uint32_t l = w * 64u * KPT + j * 64u + lane;
+					if (l < cvalid) wave_count(s_cnt, dig[j]);
+				}
+				__syncthreads();
+				if (t < 256) {
+					const uint32_t c = s_cnt[t];
+					const uint32_t incl = wave_inclusive_sum(c);
+					if ((t & 63u) == 63u) s_wsum[t >> 6] = incl;
+					s_digoff[t] = incl - c;
+				}
+				__syncthreads();
+				if (t < 256) {
+					uint32_t before = 0;
+					for (uint32_t v = 0; v < (t >> 6); ++v) before += s_wsum[v];
+					const uint32_t start = s_digoff[t] + before;
+					s_cnt[t] = start;
+					s_glob[t] = offsets[static_cast<uint64_t>(cur) * 256 + t] - start;
+				}
+				__syncthreads();
+#pragma unroll
+				for (int j = 0; j < KPT; ++j) {
+					const uint32_t l = w * 64u * KPT + j * 64u + lane;
+					if (l < cvalid) s_keys[atomicAdd(&s_cnt[dig[j]], 1u)] = keys[j];
+				}
+			}
 		}
 		k += step;
 		g = x * per + k;
@@ -764,6 +828,13 @@ struct SegPlan {
 
 namespace {
 
+// the transform keeps keys apart (at most a complement of some bits): equal sort keys are equal keys
+inline bool xform_injective(const KeyXform &xf, int key_bytes)
+{
+	const uint64_t ones = key_bytes >= 8 ? ~0ull : ((1ull << (8 * key_bytes)) - 1ull);
+	return xf.A == ones && xf.sh == 0 && xf.B == 0 && xf.C == 0 && xf.D == 0;
+}
+
 // One capacity tier of the item sort over items [0, count) of the given table (count null: count_value
 // items); a long-run item is queued as qbase + its index. 1-byte keys take one LDS LSD pass instead.
 template <typename K, int THREADS, int KPT, int PB>
@@ -777,12 +848,18 @@ hipError_t launch_sort_t(const void *src, void *dst, const uint64_t *istart, con
 		const uint32_t grid = count ? full : std::min<uint32_t>(full, std::max<uint32_t>(1u, count_value));
 		hipLaunchKernelGGL((k_item_lsd<K, THREADS, KPT>), dim3(grid), dim3(THREADS), 0, s, static_cast<const K *>(src),
 			static_cast<K *>(dst), istart, ilen, nullptr, count, count_value, xf, reverse);
-	} else {
-		static const int occ = occupancy_of<THREADS, KPT>(k_item_sort<K, THREADS, KPT, PB>);
+	} else if (xform_injective(xf, static_cast<int>(sizeof(K)))) {
+		static const int occ = occupancy_of<THREADS, KPT>(k_item_sort<K, THREADS, KPT, PB, true>);
 		const uint32_t full = static_cast<uint32_t>(occ) * cus;
 		const uint32_t grid = count ? full : std::min<uint32_t>(full, std::max<uint32_t>(1u, count_value));
-		hipLaunchKernelGGL((k_item_sort<K, THREADS, KPT, PB>), dim3(grid), dim3(THREADS), 0, s, static_cast<const K *>(src),
-			static_cast<K *>(dst), istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase);
+		hipLaunchKernelGGL((k_item_sort<K, THREADS, KPT, PB, true>), dim3(grid), dim3(THREADS), 0, s,
+			static_cast<const K *>(src), static_cast<K *>(dst), istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase);
+	} else {
+		static const int occ = occupancy_of<THREADS, KPT>(k_item_sort<K, THREADS, KPT, PB, false>);
+		const uint32_t full = static_cast<uint32_t>(occ) * cus;
+		const uint32_t grid = count ? full : std::min<uint32_t>(full, std::max<uint32_t>(1u, count_value));
+		hipLaunchKernelGGL((k_item_sort<K, THREADS, KPT, PB, false>), dim3(grid), dim3(THREADS), 0, s,
+			static_cast<const K *>(src), static_cast<K *>(dst), istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase);
 	}
 	return hipGetLastError();
 }
@@ -861,15 +938,22 @@ hipError_t launch_seg_pass(SegPlan &pl, const void *in, void *out, void *buf, co
 	return hipGetLastError();
 }
 
-template <typename K, int THREADS, int KPT>
-hipError_t launch_scatter1(SegPlan &pl, const void *in, void *buf, const uint64_t *seg, const KeyXform &xf, int reverse,
-	hipStream_t s)
+template <typename K, int THREADS, int KPT, bool STABLE>
+hipError_t launch_scatter1_t(SegPlan &pl, const void *in, void *buf, const KeyXform &xf, int reverse, hipStream_t s)
 {
-	static int occ = occupancy_of<THREADS, KPT>(k_seg_scatter1<K, THREADS, KPT>);
+	static int occ = occupancy_of<THREADS, KPT>(k_seg_scatter1<K, THREADS, KPT, STABLE>);
 	const uint32_t grid = 8u * std::min<uint32_t>((pl.ntiles + 7) / 8, static_cast<uint32_t>(occ * device_cu_count()) / 8u);
-	hipLaunchKernelGGL((k_seg_scatter1<K, THREADS, KPT>), dim3(grid), dim3(THREADS), 0, s, static_cast<const K *>(in),
+	hipLaunchKernelGGL((k_seg_scatter1<K, THREADS, KPT, STABLE>), dim3(grid), dim3(THREADS), 0, s, static_cast<const K *>(in),
 		static_cast<K *>(buf), pl.tiles, pl.ntiles, pl.counts1, pl.flag, xf, reverse);
 	return hipGetLastError();
+}
+
+template <typename K, int THREADS, int KPT>
+hipError_t launch_scatter1(SegPlan &pl, const void *in, void *buf, const uint64_t *, const KeyXform &xf, int reverse,
+	hipStream_t s)
+{
+	if (xform_injective(xf, static_cast<int>(sizeof(K)))) return launch_scatter1_t<K, THREADS, KPT, false>(pl, in, buf, xf, reverse, s);
+	return launch_scatter1_t<K, THREADS, KPT, true>(pl, in, buf, xf, reverse, s);
 }
 
 template <typename K>
