@@ -120,18 +120,23 @@ int rsbd8_hip_sort_pairs(int keytype, unsigned flags, size_t count, void *worksp
 
 /* Segmented (batched) sort: nseg independent arrays in one CSR batch, each sorted stably with the
  * same flags (the reference runs one radixsortcopy per array, H:60468-60527; no counterpart for the
- * batch itself). A plan is made once from the host copy of the nseg+1 offsets: segments up to the
- * LDS capacity of small_cfg (0: 1024, 1: 4096, 2: 8192 keys; -1 automatic) are sorted in LDS by one
- * persistent kernel, longer ones by per-digit onesweep passes over all their tiles with the tile
- * configuration tile_cfg (-1 automatic). The plan owns its device tables and workspace. run issues
- * the kernels on a stream; capture records them into a hipGraph kept in the plan, and
- * launch_graph replays that graph with one launch (on the arrays given to capture). buffer holds
- * total elements; input, output and buffer must not alias. dev_offsets is the device copy of the
- * offsets. info receives {short segments, long segments, tiles, tile keys, LDS capacity, total
- * keys, plan bytes}. */
+ * batch itself). A plan is made once from the host copy of the nseg+1 offsets. Segments up to the
+ * LDS capacity of small_cfg (0: 1024, 1: 4096, 2: 8192 keys) are sorted in LDS, one block per
+ * segment (one wave per segment of at most 64 keys); longer ones are partitioned by their top digit
+ * in tiles of tile_cfg (threads x keys per thread: 256x8, 256x16, 512x8, 512x16, 256x24) and each
+ * run of buckets of at most that capacity is then sorted in LDS the same way; a segment with a
+ * larger bucket takes per-digit onesweep passes over its tiles instead. -1 selects the tuned choice
+ * for the key width and mode. The plan owns its device tables and workspace. run issues the kernels
+ * on a stream; capture records them into a hipGraph kept in the plan, and launch_graph replays that
+ * graph with one launch (on the arrays given to capture). buffer holds total elements; input,
+ * output and buffer must not alias. dev_offsets is the device copy of the same offsets. info
+ * receives {short segments, long segments, tiles, tile keys, LDS capacity, total keys, plan bytes}. */
 typedef struct rsbd8_hip_segplan rsbd8_hip_segplan;
 int rsbd8_hip_segmented_plan_create(int keytype, unsigned flags, size_t nseg, const uint64_t *host_offsets, int tile_cfg,
 	int small_cfg, rsbd8_hip_segplan **plan);
+/* The automatic (tuned) tile_cfg and small_cfg of a segmented plan for this key type and flags;
+ * host only, no GPU call. */
+int rsbd8_hip_segmented_default_config(int keytype, unsigned flags, int *tile_cfg, int *small_cfg);
 int rsbd8_hip_segmented_run(rsbd8_hip_segplan *plan, const void *input, void *output, void *buffer,
 	const uint64_t *dev_offsets, rsbd8_hip_stream stream);
 int rsbd8_hip_segmented_capture(rsbd8_hip_segplan *plan, const void *input, void *output, void *buffer,

@@ -95,6 +95,7 @@ def _load():
         "rsbd8_hip_segmented_capture": ([vp, vp, vp, vp, vp, vp], ci),
         "rsbd8_hip_segmented_launch_graph": ([vp, vp], ci),
         "rsbd8_hip_segmented_plan_info": ([vp, ctypes.POINTER(ctypes.c_uint64)], ci),
+        "rsbd8_hip_segmented_default_config": ([ci, cu, ctypes.POINTER(ci), ctypes.POINTER(ci)], ci),
         "rsbd8_hip_segmented_plan_destroy": ([vp], None),
         "rsbd8_hip_profile_begin": ([], ci),
         "rsbd8_hip_profile_end": ([ctypes.POINTER(ctypes.c_double), ctypes.POINTER(cu), ci], ci),
@@ -347,6 +348,14 @@ def sort_pairs(keys_in, keys_out, keys_buffer, payload_in, payload_out, payload_
 # ---- segmented batches --------------------------------------------------------------------
 
 SMALL_CAPS = (1024, 4096, 8192)
+
+
+def segmented_default_config(key_dtype, direction=sortingdirection.ascfwdorder, mode=sortingmode.native):
+    """The library's automatic (tile_cfg, small_cfg) for a segmented plan of this key type and mode."""
+    kt = keytype_of(key_dtype)
+    tc, sc = ctypes.c_int(), ctypes.c_int()
+    _check(_load().rsbd8_hip_segmented_default_config(kt, decode(kt, mode, direction), ctypes.byref(tc), ctypes.byref(sc)))
+    return tc.value, sc.value
 
 
 class SegmentedPlan:

@@ -491,13 +491,23 @@ int rsbd8_hip_segmented_plan_create(int keytype, unsigned flags, size_t nseg, co
 	*plan = nullptr;
 	const int kb = key_bytes_of(keytype);
 	if (!kb || !valid_flags(flags)) return fail(RSBD8_HIP_EINVAL, "bad key type or flags");
-	if (tile_cfg < 0) tile_cfg = kb >= 8 ? 1 : 3;
-	if (small_cfg < 0) small_cfg = kb >= 8 ? 1 : 2;
+	rsbd8hip::segplan_default_cfg(kb, flags, &tile_cfg, &small_cfg);
 	rsbd8hip::SegPlan *impl = nullptr;
 	const hipError_t e = segplan_create(kb, flags, nseg, host_offsets, tile_cfg, small_cfg, &impl);
 	if (e != hipSuccess) return fail_hip(e, "segmented_plan_create");
 	*plan = new (std::nothrow) rsbd8_hip_segplan{impl};
 	if (!*plan) { segplan_destroy(impl); return fail(RSBD8_HIP_ENOMEM, "plan allocation"); }
+	return RSBD8_HIP_OK;
+}
+
+int rsbd8_hip_segmented_default_config(int keytype, unsigned flags, int *tile_cfg, int *small_cfg)
+{
+	if (!tile_cfg || !small_cfg) return fail(RSBD8_HIP_EINVAL, "null argument");
+	const int kb = key_bytes_of(keytype);
+	if (!kb || !valid_flags(flags)) return fail(RSBD8_HIP_EINVAL, "bad key type or flags");
+	*tile_cfg = -1;
+	*small_cfg = -1;
+	rsbd8hip::segplan_default_cfg(kb, flags, tile_cfg, small_cfg);
 	return RSBD8_HIP_OK;
 }
 

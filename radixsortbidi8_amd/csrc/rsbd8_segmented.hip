@@ -1025,6 +1025,23 @@ hipError_t run_typed(SegPlan &pl, const void *in, void *out, void *buf, const ui
 } // namespace
 
 int tile_keys_of(int cfg) { return kSegTileCfgs[cfg].threads * kSegTileCfgs[cfg].kpt; }
+
+void segplan_default_cfg(int key_bytes, unsigned flags, int *tile_cfg, int *small_cfg)
+{
+	// winners of the sweep over (tile_cfg, small_cfg) on 65,536 segments with log-uniform lengths
+	// 2^4..2^20 and 2^28 keys in total, per key width and transform kind (the tuned table ships with
+	// the Python package)
+	const bool inj = xform_injective(make_xform(8u * static_cast<unsigned>(key_bytes), flags), key_bytes);
+	int tc = 3, sc = 1;
+	switch (key_bytes) {
+	case 1: tc = 1; sc = 1; break;
+	case 2: tc = 4; sc = 1; break;
+	case 4: tc = inj ? 3 : 2; sc = 1; break;
+	default: tc = inj ? 3 : 2; sc = 0; break;
+	}
+	if (*tile_cfg < 0) *tile_cfg = tc;
+	if (*small_cfg < 0) *small_cfg = sc;
+}
 int small_cap_of(int cfg) { return kSmallCaps[cfg]; }
 
 hipError_t segplan_create(int key_bytes, unsigned flags, uint64_t nseg, const uint64_t *host_off, int tile_cfg, int small_cfg,

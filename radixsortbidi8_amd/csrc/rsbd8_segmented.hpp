@@ -16,6 +16,8 @@ hipError_t segplan_capture(SegPlan *pl, const void *in, void *out, void *buf, co
 hipError_t segplan_launch(SegPlan *pl, hipStream_t s);
 void segplan_info(const SegPlan *pl, uint64_t *info);
 int tile_keys_of(int cfg);
+// the tuned automatic choice for tile_cfg / small_cfg arguments below zero
+void segplan_default_cfg(int key_bytes, unsigned flags, int *tile_cfg, int *small_cfg);
 int small_cap_of(int cfg);
 
 } // namespace rsbd8hip

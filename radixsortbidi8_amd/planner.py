@@ -456,6 +456,48 @@ def load_table(path=None):
         return json.load(f)
 
 
+# ---- segmented batches: tile configuration and LDS tier ------------------------------------------
+
+def segmented_batch(nseg: int, total_log2: int, seed: int = 0x5EED0005):
+    """CSR offsets of nseg segments with log-uniform lengths 2^4..2^20 scaled to about 2^total_log2
+    keys in total (the 65,536-array batch workload)."""
+    rng = np.random.default_rng(seed)
+    lens = np.floor(2.0 ** rng.uniform(4, 20, nseg))
+    lens = np.maximum(1, np.floor(lens * ((1 << total_log2) / lens.sum()))).astype(np.int64)
+    off = np.zeros(nseg + 1, np.uint64)
+    off[1:] = np.cumsum(lens)
+    return off
+
+
+def autotune_segmented(keys, offsets, direction=0, mode=64, reps: int = 3, log=None):
+    """Times the captured plan of one batch for every (tile configuration, LDS tier) pair and keeps
+    the fastest: returns ((tile_cfg, small_cfg), {"tile_cfg:small_cfg": ms})."""
+    torch = _torch()
+    from . import SegmentedPlan, SEGMENTED_TILE_CONFIGS, SMALL_CAPS
+    out, buf = torch.empty_like(keys), torch.empty_like(keys)
+    np_dt = {torch.int32: np.int32, torch.uint32: np.uint32, torch.int64: np.int64, torch.uint64: np.uint64,
+             torch.float32: np.float32, torch.float64: np.float64, torch.int16: np.int16, torch.uint16: np.uint16,
+             torch.int8: np.int8, torch.uint8: np.uint8}[keys.dtype]
+    times = {}
+    for sc in range(len(SMALL_CAPS)):
+        for tc in range(len(SEGMENTED_TILE_CONFIGS)):
+            plan = SegmentedPlan(np_dt, offsets, direction, mode, tile_cfg=tc, small_cfg=sc)
+            plan.capture(keys, out, buf)
+            plan.launch()
+            times[f"{tc}:{sc}"] = time_call(plan.launch, reps)
+            plan.close()
+            if log:
+                log(f"segmented {keys.dtype} tile_cfg {tc} small_cfg {sc}: {times[f'{tc}:{sc}']:.3f} ms")
+    best = min(times, key=times.get)
+    return tuple(int(v) for v in best.split(":")), times
+
+
+def load_segmented_table(path=None):
+    """{"<key bytes>": [tile_cfg, small_cfg]} from the segmented sweep (tuning/segmented_table.json)."""
+    with open(path or os.path.join(TUNING_DIR, "segmented_table.json")) as f:
+        return json.load(f)["best"]
+
+
 # ---- training data and evaluation ----------------------------------------------------------------
 
 def collect(requests, reps: int = 3, log=None):

@@ -5,8 +5,9 @@
                                                [--segmented [--small-cfg S] [--sweep]]
 
 Sorts 2^n seeded uniform keys (the benchmark's distribution) reps times after one warm-up sort,
-or (--segmented) sorts a batch of 65,536 int32 segments with log-uniform lengths totalling 2^n keys
-(plan tile configuration --tile-config, LDS tier --small-cfg; --sweep times every pair), then, with --copy, copies the same number of bytes device to device reps times: a plain copy of
+or (--segmented) sorts a batch of 65,536 segments with log-uniform lengths totalling 2^n keys (int32
+keys, or u64/f64 with --seg-dtype; plan tile configuration --tile-config, LDS tier --small-cfg; --sweep
+times every pair), then, with --copy, copies the same number of bytes device to device reps times: a plain copy of
 known size that calibrates the HBM byte counters (FETCH_SIZE, WRITE_SIZE) against the bytes it
 must move.
 """
@@ -25,6 +26,7 @@ def main():
     ap.add_argument("--segmented", action="store_true",
                     help="instead: 65,536 int32 segments, log-uniform lengths scaled to 2^n keys in total")
     ap.add_argument("--small-cfg", type=int, default=-1, help="segmented: LDS tier of the plan (-1 automatic)")
+    ap.add_argument("--seg-dtype", default="i32", choices=("i32", "u64", "f64"), help="segmented: key type")
     ap.add_argument("--sweep", action="store_true", help="segmented: time every tile configuration and LDS tier")
     args = ap.parse_args()
     import torch
@@ -69,14 +71,20 @@ def segmented(args, torch, r):
     off = np.zeros(nseg + 1, np.uint64)
     off[1:] = np.cumsum(lens)
     total = int(off[-1])
-    keys = torch.randint(-(1 << 31), (1 << 31) - 1, (total,), dtype=torch.int32, device="cuda",
-                         generator=torch.Generator(device="cuda").manual_seed(0x5EED0005))
+    gen = torch.Generator(device="cuda").manual_seed(0x5EED0005)
+    if args.seg_dtype in ("u64", "f64"):
+        keys = torch.randint(-(1 << 63), (1 << 63) - 1, (total,), dtype=torch.int64, device="cuda", generator=gen)
+        npdt = np.uint64 if args.seg_dtype == "u64" else np.float64
+        keys = keys.view(torch.uint64 if args.seg_dtype == "u64" else torch.float64)
+    else:
+        keys = torch.randint(-(1 << 31), (1 << 31) - 1, (total,), dtype=torch.int32, device="cuda", generator=gen)
+        npdt = np.int32
     out, buf = torch.empty_like(keys), torch.empty_like(keys)
     pairs = [(args.tile_config, args.small_cfg)]
     if args.sweep:
         pairs = [(tc, sc) for sc in range(len(r.SMALL_CAPS)) for tc in range(len(r.SEGMENTED_TILE_CONFIGS))]
     for tc, sc in pairs:
-        plan = r.SegmentedPlan(np.int32, off, tile_cfg=tc, small_cfg=sc)
+        plan = r.SegmentedPlan(npdt, off, tile_cfg=tc, small_cfg=sc)
         print(f"profile_driver: segmented plan tile_cfg {tc} small_cfg {sc} {plan.info()}", flush=True)
         plan.run(keys, out, buf)
         plan.capture(keys, out, buf)

@@ -128,3 +128,22 @@ def test_segmented_long_equal_runs(dtype):
             torch.cuda.synchronize()
             check(keys, off, d_out.cpu().numpy(), r.decode(kt, mode, d))
             plan.close()
+
+
+def test_segmented_autotune_small_batch():
+    """The sweep over (tile configuration, LDS tier) runs every pair on a batch and returns one of
+    them; the plan it picks sorts the batch like the oracle."""
+    from radixsortbidi8_amd import planner as pl
+    off = pl.segmented_batch(512, 18, seed=5)
+    rng = np.random.default_rng(8)
+    keys = rng.integers(-2**31, 2**31, int(off[-1]), dtype=np.int64).astype(np.int32)
+    d_in = torch.from_numpy(keys).cuda()
+    (tc, sc), times = pl.autotune_segmented(d_in, off, reps=1)
+    assert len(times) == len(r.SEGMENTED_TILE_CONFIGS) * len(r.SMALL_CAPS)
+    assert f"{tc}:{sc}" in times
+    out, buf = torch.empty_like(d_in), torch.empty_like(d_in)
+    plan = r.SegmentedPlan(np.int32, off, tile_cfg=tc, small_cfg=sc)
+    plan.run(d_in, out, buf)
+    torch.cuda.synchronize()
+    check(keys, off, out.cpu().numpy(), r.decode(r.I32))
+    plan.close()

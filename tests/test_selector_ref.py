@@ -35,3 +35,34 @@ def test_plan_list():
 
 def test_library_plan_order_matches():
     assert pl.library_plans() == pl.PLANS
+
+
+def test_segmented_table_covers_every_key_kind():
+    """The tuned segmented table names a valid (tile configuration, LDS tier) for each key width and
+    transform kind, and the sweep it came from timed every pair."""
+    import json
+    import os
+    from radixsortbidi8_amd import SEGMENTED_TILE_CONFIGS, SMALL_CAPS
+    path = os.path.join(os.path.dirname(__file__), "..", "radixsortbidi8_amd", "tuning", "segmented_table.json")
+    with open(path) as f:
+        t = json.load(f)
+    assert set(t["best"]) == {"1:1", "2:1", "4:0", "4:1", "8:0", "8:1"}
+    for tc, sc in t["best"].values():
+        assert 0 <= tc < len(SEGMENTED_TILE_CONFIGS) and 0 <= sc < len(SMALL_CAPS)
+    for name, times in t["times_ms"].items():
+        assert len(times) == len(SEGMENTED_TILE_CONFIGS) * len(SMALL_CAPS), name
+
+
+def test_segmented_automatic_choice_is_the_tuned_winner():
+    """The library's automatic segmented configuration equals the sweep's winner for every key
+    width and transform kind (integers: injective; floats: not)."""
+    import json
+    import os
+    import numpy as np
+    import radixsortbidi8_amd as r
+    path = os.path.join(os.path.dirname(__file__), "..", "radixsortbidi8_amd", "tuning", "segmented_table.json")
+    with open(path) as f:
+        best = json.load(f)["best"]
+    for dt, key in ((np.uint8, "1:1"), (np.int16, "2:1"), (np.int32, "4:1"), (np.uint32, "4:1"), (np.float32, "4:0"),
+                    (np.uint64, "8:1"), (np.int64, "8:1"), (np.float64, "8:0")):
+        assert list(r.segmented_default_config(dt)) == best[key], dt
