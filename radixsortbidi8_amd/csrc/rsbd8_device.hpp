@@ -192,6 +192,64 @@ __device__ __forceinline__ void block_rank(const uint32_t (&dig)[KPT], uint32_t 
 
 namespace rsbd8hip {
 
+// Claims one slot of s[d] per active lane and returns it: one atomic for the whole wave when all
+// active lanes share d (slots in lane order), else one per lane (slots in no particular order).
+__device__ __forceinline__ uint32_t wave_claim(uint32_t *s, uint32_t d)
+{
+	const uint64_t active = __ballot(1);
+	const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+	if (__ballot(d == d0) == active) {
+		const uint32_t below = lanes_below(active);
+		uint32_t base = 0;
+		if (below == 0) base = atomicAdd(&s[d0], static_cast<uint32_t>(__popcll(active)));
+		return __builtin_amdgcn_readfirstlane(base) + below;
+	}
+	return atomicAdd(&s[d], 1u);
+}
+
+// Block-wide digit grouping of the `valid` first keys (logical order l = wave*64*KPT + j*64 +
+// lane, as in block_rank): pos[j] is a place in the digit-sorted tile, inside its digit's run but
+// in no particular order within it (LDS atomics), and s_digoff[d] the digit's exclusive offset.
+// Keys at l >= valid get no place. s_cnt holds 256 words, s_wsum THREADS/64 words; every thread
+// must reach the barriers.
+template <int THREADS, int KPT>
+__device__ __forceinline__ void block_group(const uint32_t (&dig)[KPT], uint32_t valid, uint32_t (&pos)[KPT],
+	uint32_t *s_cnt, uint32_t *s_digoff, uint32_t *s_wsum)
+{
+	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+	for (uint32_t i = t; i < 256; i += THREADS) s_cnt[i] = 0;
+	__syncthreads();
+#pragma unroll
+	for (int j = 0; j < KPT; ++j) {
+		const uint32_t l = w * 64u * KPT + j * 64u + lane;
+		if (l < valid) wave_count(s_cnt, dig[j]);
+	}
+	__syncthreads();
+	if (t < 256) {
+		const uint32_t c = s_cnt[t];
+		const uint32_t incl = wave_inclusive_sum(c);
+		if (lane == 63) s_wsum[w] = incl;
+		s_digoff[t] = incl - c;
+	}
+	__syncthreads();
+	if (t < 256) {
+		uint32_t before = 0;
+		for (uint32_t v = 0; v < w; ++v) before += s_wsum[v];
+		s_digoff[t] += before;
+		s_cnt[t] = s_digoff[t];
+	}
+	__syncthreads();
+#pragma unroll
+	for (int j = 0; j < KPT; ++j) {
+		const uint32_t l = w * 64u * KPT + j * 64u + lane;
+		pos[j] = l < valid ? wave_claim(s_cnt, dig[j]) : 0xFFFFFFFFu;
+	}
+}
+
+} // namespace rsbd8hip
+
+namespace rsbd8hip {
+
 // Decoupled look-back for one digit: sums the published counts of the tiles before `tile` back to
 // the nearest inclusive prefix, reading W predecessors per round so that W memory latencies overlap.
 // `words` points at the look-back word of this digit for tile 0 (STRIDE words per tile). Every

@@ -648,8 +648,11 @@ __device__ __forceinline__ void seg_tile(const uint32_t *s_ts, const SegMap &m, 
 }
 
 // pp: the pass (inactive: exit); shift: its digit. With SEG, the tiles are those of the segment map
-// and num_tiles is an upper bound (the exact count is tile_start[256]).
-template <typename K, bool PAY, int THREADS, int KPT, bool SEG = false>
+// and num_tiles is an upper bound (the exact count is tile_start[256]). With GROUP (keys only) the
+// keys of a digit are not kept in order within the tile (LDS atomics instead of block ranking):
+// only for the MSD plan's partitions when the key transform is injective, where the per-bucket
+// sort that follows orders every bucket by the whole sort key and equal sort keys are equal keys.
+template <typename K, bool PAY, int THREADS, int KPT, bool SEG = false, bool GROUP = false>
 __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32_t shift, KeyXform xf, uint64_t n,
 	uint32_t num_tiles, const uint32_t *__restrict__ offsets, SegMap seg)
 {
@@ -719,10 +722,20 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 			const uint32_t l = w * 64u * KPT + j * 64u + lane;
 			dig[j] = l < valid ? digit_of<K>(keys[j], xf, shift) : 0xFFu;
 		}
-		block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total);
-		if (t < kRadix) s_glob[t] = offsets[static_cast<uint64_t>(tile) * kCountStride + t] - s_digoff[t];
+		if constexpr (GROUP) {
+			static_assert(!PAY, "grouping without order is for keys only");
+			(void)total;
+			block_group<THREADS, KPT>(dig, valid, pos, s_whist, s_digoff, s_wsum);
+			if (t < kRadix) s_glob[t] = offsets[static_cast<uint64_t>(tile) * kCountStride + t] - s_digoff[t];
 #pragma unroll
-		for (int j = 0; j < KPT; ++j) s_keys[pos[j]] = keys[j];
+			for (int j = 0; j < KPT; ++j)
+				if (pos[j] != 0xFFFFFFFFu) s_keys[pos[j]] = keys[j];
+		} else {
+			block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total);
+			if (t < kRadix) s_glob[t] = offsets[static_cast<uint64_t>(tile) * kCountStride + t] - s_digoff[t];
+#pragma unroll
+			for (int j = 0; j < KPT; ++j) s_keys[pos[j]] = keys[j];
+		}
 		uint32_t ppos[PAY ? KPT : 1];
 		if constexpr (PAY) {
 #pragma unroll
@@ -1149,22 +1162,30 @@ hipError_t launch_pass_cfg(int cfg, Control *ctrl, const uint32_t *bucket, uint6
 	}
 }
 
-template <typename K, bool PAY, int THREADS, int KPT, bool SEG = false>
+template <typename K, bool PAY, int THREADS, int KPT, bool SEG = false, bool GROUP = false>
 hipError_t launch_scatter(const PassPlan *pp, uint32_t shift, const KeyXform &xf, uint64_t n, uint32_t tiles,
 	const uint32_t *offsets, SegMap seg, hipStream_t stream)
 {
 	static int occ = -1;
 	if (occ < 0) {
 		int o = 0;
-		if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_scatter<K, PAY, THREADS, KPT, SEG>, THREADS, 0) != hipSuccess || o < 1) o = 1;
+		if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_scatter<K, PAY, THREADS, KPT, SEG, GROUP>, THREADS, 0) != hipSuccess ||
+			o < 1) o = 1;
 		occ = o;
 	}
 	const uint32_t per = (tiles + 7) / 8;
 	const uint32_t want = static_cast<uint32_t>(occ * devinfo().cus * 2 + 7) / 8;
 	const uint32_t grid = 8u * std::max<uint32_t>(1u, std::min<uint32_t>(per, want));
-	hipLaunchKernelGGL((k_scatter<K, PAY, THREADS, KPT, SEG>), dim3(grid), dim3(THREADS), 0, stream, pp, shift, xf, n, tiles,
-		offsets, seg);
+	hipLaunchKernelGGL((k_scatter<K, PAY, THREADS, KPT, SEG, GROUP>), dim3(grid), dim3(THREADS), 0, stream, pp, shift, xf, n,
+		tiles, offsets, seg);
 	return hipGetLastError();
+}
+
+// the transform only complements bits: equal sort keys are equal keys
+inline bool xform_injective(const KeyXform &xf, int key_bytes)
+{
+	const uint64_t ones = key_bytes >= 8 ? ~0ull : ((1ull << (8 * key_bytes)) - 1ull);
+	return xf.A == ones && xf.sh == 0 && xf.B == 0 && xf.C == 0 && xf.D == 0;
 }
 
 template <typename K, bool PAY>
@@ -1234,6 +1255,7 @@ hipError_t launch_hybrid(Control *ctrl, const uint32_t *bucket, uint32_t *counts
 	constexpr int P = sizeof(K);
 	constexpr uint32_t TILE = THREADS * KPT;
 	const int cus = devinfo().cus;
+	const bool inj = xform_injective(xf, P);
 	hipError_t e;
 	{
 		ProfScope ps(7, 0, ws, stream);
@@ -1245,7 +1267,9 @@ hipError_t launch_hybrid(Control *ctrl, const uint32_t *bucket, uint32_t *counts
 	}
 	{
 		ProfScope ps(5, 0, ws, stream);
-		e = launch_scatter<K, false, THREADS, KPT>(&ctrl->hyb[0], 8u * (P - 1), xf, n, tiles, counts, SegMap{}, stream);
+		e = inj ? launch_scatter<K, false, THREADS, KPT, false, true>(&ctrl->hyb[0], 8u * (P - 1), xf, n, tiles, counts, SegMap{},
+		                                                              stream)
+		        : launch_scatter<K, false, THREADS, KPT>(&ctrl->hyb[0], 8u * (P - 1), xf, n, tiles, counts, SegMap{}, stream);
 		if (e != hipSuccess) return e;
 	}
 	const uint32_t tiles2 = tiles + kRadix;  // bound: one partial tile per segment
@@ -1263,7 +1287,8 @@ hipError_t launch_hybrid(Control *ctrl, const uint32_t *bucket, uint32_t *counts
 	}
 	{
 		ProfScope ps(5, 1, ws, stream);
-		e = launch_scatter<K, false, THREADS, KPT, true>(&ctrl->hyb[1], 8u * (P - 2), xf, n, tiles2, counts, m, stream);
+		e = inj ? launch_scatter<K, false, THREADS, KPT, true, true>(&ctrl->hyb[1], 8u * (P - 2), xf, n, tiles2, counts, m, stream)
+		        : launch_scatter<K, false, THREADS, KPT, true>(&ctrl->hyb[1], 8u * (P - 2), xf, n, tiles2, counts, m, stream);
 		if (e != hipSuccess) return e;
 	}
 	ProfScope ps(6, 0, ws, stream);
