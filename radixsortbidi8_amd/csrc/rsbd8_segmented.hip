@@ -3,23 +3,27 @@
 // The reference sorts a batch by calling radixsortcopy once per array (Radixsortbidi8.hpp:60468);
 // here every segment of a CSR batch is sorted with the same mode/direction flags. Segments up to
 // the LDS capacity are items of their own; longer ones are partitioned by their top digit first:
+//   k_item_wave     items of at most 64 keys, one wave each: every lane ranks its key by counting
+//                   the keys that precede it in the stable order;
 //   k_item_sort     persistent blocks, one item (a key range sorted on its own) at a time, all in
-//                   LDS: partition by the top bits of the transformed key with LDS atomics, then
-//                   each key's exact position in its run by counting the run's keys that precede it
-//                   in the stable order (smaller key, or equal key at a lower position);
-//   k_item_lsd      items with a run too long for that: stable LDS LSD passes (block ranking);
+//                   LDS: partition by (key - min) >> sh with LDS atomics, then each key's exact
+//                   position in its partition by counting the partition's keys that precede it in
+//                   the stable order (smaller key, or equal key read earlier);
+//   k_item_lsd      items with a partition too long for that: stable LDS LSD passes;
 //   k_seg_count1    per-tile counts of the top digit over the tiles of the long segments;
 //   k_seg_scan1     one block per long segment: bucket sizes and starts, tile offsets, and the
 //                   segment's buckets packed into items of at most the LDS capacity (consecutive
 //                   buckets of one segment may share an item: they are already in top-digit
 //                   order, so a sort of the item by the whole key is exact); a bucket above the
 //                   capacity sends the segment to the LSD passes below instead;
-//   k_seg_scatter1  the stable partition of the long segments by the top digit;
+//   k_seg_scatter1  the partition of the long segments by the top digit;
 //   k_seg_hist, k_seg_plan, k_seg_pass, k_seg_copy
 //                   for those segments only: per-segment digit histograms, plan, onesweep passes
 //                   with the look-back confined to the segment, and the copy when every digit is
 //                   uniform;
 //   k_seg_finish    resets the counters and advances the look-back epoch.
+// When the key transform is injective (equal sort keys are equal keys) the order of ties cannot be
+// observed, and the partition and the item sort use cheaper forms that do not keep it.
 // A plan object owns the device tables (segment classes, tile map, items) and its workspace, and
 // can capture the whole sequence (short segments on a side stream in parallel with the long-segment
 // chain) into one hipGraph that is replayed with a single launch.
