@@ -94,9 +94,11 @@ def test_msd_long_runs_take_the_block_lsd_path(nb, msd):
             assert used
 
 
-@pytest.mark.parametrize("nb", [4, 8])
-def test_msd_oversized_bucket_falls_back(nb, msd):
-    """One bucket of 3000 keys with capacity 1024: the sort must switch back to the LSD plan."""
+@pytest.mark.parametrize("nb,kt,mode", [(4, r.U32, 64), (8, r.U64, 64), (4, r.F32, 64), (8, r.F64, 65)])
+def test_msd_oversized_bucket_falls_back(nb, kt, mode, msd):
+    """One bucket of 3000 keys with capacity 1024: the sort must switch back to the LSD plan (the
+    onesweep passes, from the intact input), for injective and non-injective transforms, with ±0.0
+    among the float keys so that tie order is observed."""
     rng = np.random.default_rng(77 + nb)
     n = 1 << 20
     bits = spread(nb, n, rng)
@@ -104,10 +106,12 @@ def test_msd_oversized_bucket_falls_back(nb, msd):
     idx = rng.choice(n, 3000, replace=False)
     bits[idx] = ((np.uint64(0x4242) << np.uint64(sh - 16)) |
                  rng.integers(0, 1 << (sh - 16), 3000, dtype=np.uint64)).astype(UINT[nb])
-    kt = r.U32 if nb == 4 else r.U64
+    zeros = rng.choice(n, 2000, replace=False)
+    bits[zeros[:1000]] = 0
+    bits[zeros[1000:]] = UINT[nb](1 << (sh - 1))
     for d in range(4):
-        got, used = run(bits, kt, 64, d)
-        exp, _ = oracle.radixsort(bits, r.decode(kt, 64, d))
+        got, used = run(bits, kt, mode, d)
+        exp, _ = oracle.radixsort(bits, r.decode(kt, mode, d))
         assert np.array_equal(got, exp), d
         assert not used
 
