@@ -287,12 +287,13 @@ namespace rsbd8hip {
 
 // Calls f(key) for every key of the physical range [lo, lo + len) of p, with the keys of a block
 // spread over its threads: the keys before the first 16-byte boundary one per thread, then 16-byte
-// vector loads (four in flight per thread), then the rest one per thread. The order in which a
+// vector loads (eight in flight per thread), then the rest one per thread. The order in which a
 // thread sees its keys is unspecified (counting only). p must be aligned to the key size.
 template <typename K, int THREADS, typename F>
 __device__ __forceinline__ void for_each_key(const K *__restrict__ p, uint64_t lo, uint32_t len, F &&f)
 {
 	constexpr uint32_t VEC = 16 / sizeof(K);
+	constexpr int INFLIGHT = 8;
 	const K *q = p + lo;
 	const uint32_t mis = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(q) & 15u) / static_cast<uint32_t>(sizeof(K));
 	uint32_t head = mis ? VEC - mis : 0u;
@@ -302,15 +303,15 @@ __device__ __forceinline__ void for_each_key(const K *__restrict__ p, uint64_t l
 	const uint32_t body = len - head;
 	const uint32_t nvec = body / VEC;
 	const uint4 *qv = reinterpret_cast<const uint4 *>(qa);
-	for (uint32_t v = threadIdx.x; v < nvec; v += THREADS * 4) {
-		uint4 u[4];
+	for (uint32_t v = threadIdx.x; v < nvec; v += THREADS * INFLIGHT) {
+		uint4 u[INFLIGHT];
 #pragma unroll
-		for (int jj = 0; jj < 4; ++jj) {
+		for (int jj = 0; jj < INFLIGHT; ++jj) {
 			const uint32_t idx = v + jj * THREADS;
 			if (idx < nvec) u[jj] = qv[idx];
 		}
 #pragma unroll
-		for (int jj = 0; jj < 4; ++jj) {
+		for (int jj = 0; jj < INFLIGHT; ++jj) {
 			if (v + jj * THREADS >= nvec) break;
 			K ks[VEC];
 			__builtin_memcpy(ks, &u[jj], 16);
