@@ -147,3 +147,35 @@ def test_segmented_autotune_small_batch():
     torch.cuda.synchronize()
     check(keys, off, out.cpu().numpy(), r.decode(r.I32))
     plan.close()
+
+
+@pytest.mark.parametrize("dtype", [np.uint32, np.int64, np.float32, np.float64, np.uint16, np.uint8])
+@pytest.mark.parametrize("small_cfg", [0, 1, 2])
+def test_segmented_tier_boundaries(dtype, small_cfg):
+    """Segment lengths at every boundary between the wave kernel (64 keys), the LDS tiers (1024,
+    4096, 8192 keys) and the partitioned long segments, in every direction, with duplicates and
+    (for floats) signed zeros so tie order is observed."""
+    lens = np.array([0, 1, 2, 3, 63, 64, 65, 127, 1023, 1024, 1025, 4095, 4096, 4097, 8191, 8192, 8193, 20000, 70000],
+                    np.int64)
+    off = np.zeros(lens.size + 1, np.uint64)
+    off[1:] = np.cumsum(lens)
+    n = int(off[-1])
+    nb = np.dtype(dtype).itemsize
+    ut = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[nb]
+    rng = np.random.default_rng(1000 + 10 * small_cfg + nb)
+    bits = rng.integers(0, np.iinfo(ut).max, n, dtype=ut, endpoint=True)
+    bits[::5] = bits[::5] & ut(0x0F)  # duplicates
+    if np.dtype(dtype).kind == "f":
+        z = rng.choice(n, n // 20, replace=False)
+        bits[z[: z.size // 2]] = 0
+        bits[z[z.size // 2:]] = ut(1 << (8 * nb - 1))
+    keys = bits.view(dtype)
+    d_in = torch.from_numpy(bits).cuda()
+    d_out, d_buf = torch.empty_like(d_in), torch.empty_like(d_in)
+    kt = r.keytype_of(keys.dtype)
+    for d in range(4):
+        plan = r.SegmentedPlan(keys.dtype, off, d, 64, small_cfg=small_cfg)
+        plan.run(d_in, d_out, d_buf)
+        torch.cuda.synchronize()
+        check(keys, off, d_out.cpu().numpy().view(dtype), r.decode(kt, 64, d))
+        plan.close()
