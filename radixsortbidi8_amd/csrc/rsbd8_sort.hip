@@ -1521,9 +1521,10 @@ int default_algo(int key_bytes, uint64_t n, bool keys_only)
 // measured on MI355X: 1024x12 tiles (long digit runs per tile) for 8-byte keys from 2^22 keys
 int default_cfg(int key_bytes, uint64_t n, bool payload)
 {
+	// the autotuned table's choices at these sizes (uniform keys)
 	if (n <= (1u << 16)) return 0;
-	if (default_algo(key_bytes, n, false) == kAlgoCountScan) return key_bytes >= 8 ? 3 : 1;
-	(void)payload;
+	if (!payload && default_algo(key_bytes, n, true) == kAlgoHybrid) return 3;  // MSD partitions: 1024x12
+	if (default_algo(key_bytes, n, false) == kAlgoCountScan) return key_bytes >= 8 ? 3 : (key_bytes == 2 ? 2 : 1);
 	return key_bytes >= 8 ? 4 : 2;
 }
 
