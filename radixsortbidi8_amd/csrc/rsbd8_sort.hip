@@ -1347,6 +1347,7 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 		if (lcfg < 0) algo = kAlgoCountScan;
 	}
 	const bool hybrid = lcfg >= 0;
+	const bool float_keys = (job.flags & kFltp) != 0;
 	const uint32_t tile_keys = static_cast<uint32_t>(kTileCfgs[cfg].threads * kTileCfgs[cfg].kpt);
 	const uint32_t tiles = static_cast<uint32_t>((n + tile_keys - 1) / tile_keys);
 	uint32_t *counts = counts_of(ws);
@@ -1365,9 +1366,19 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 		ProfScope ps(0, 0, ws, stream);
 		const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, 8u * cus));
 		if (hybrid) {
-			if constexpr (P >= 4)
-				hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS, true>), dim3(grid), dim3(256), 0, stream, static_cast<const K *>(job.in),
-					n, xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0, ctrl, 8u * (P - 1), nullptr);
+			// The MSD plan needs only the top two digits' histograms. Float keys, whose exponents
+			// cluster, usually fail its bucket estimate; for them the read counts every digit, so the
+			// LSD plan it falls back to needs no second read.
+			if constexpr (P >= 4) {
+				if (!float_keys)
+					hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS, true>), dim3(grid), dim3(256), 0, stream,
+						static_cast<const K *>(job.in), n, xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0, ctrl,
+						8u * (P - 1), nullptr);
+				else
+					hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS, false>), dim3(grid), dim3(256), 0, stream,
+						static_cast<const K *>(job.in), n, xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0, ctrl,
+						8u * (P - 1), nullptr);
+			}
 		} else if (count_tiles) {
 			hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS>), dim3(grid), dim3(256), 0, stream, static_cast<const K *>(job.in), n,
 				xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0, ctrl, 0u, nullptr);
@@ -1392,7 +1403,7 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	pa.hyb_cap = hybrid ? kLocalCaps[lcfg] : 0;
 	pa.tile2 = tile_keys;
 	pa.segtab = segtab_of(ws);
-	pa.partial = hybrid ? 1 : 0;
+	pa.partial = (hybrid && !float_keys) ? 1 : 0;
 	pa.gate = nullptr;
 
 	hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, stream, pa, ctrl, hist, bucket);
