@@ -1304,7 +1304,8 @@ hipError_t launch_hybrid_cfg(int cfg, Control *ctrl, const uint32_t *bucket, uin
 	case 1: return launch_hybrid<K, 256, 16>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
 	case 2: return launch_hybrid<K, 512, 8>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
 	case 3: return launch_hybrid<K, 1024, 12>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
-	default: return launch_hybrid<K, 256, 24>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
+	case 4: return launch_hybrid<K, 256, 24>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
+	default: return launch_hybrid<K, 1024, 16>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
 	}
 }
 
@@ -1350,6 +1351,11 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	const bool float_keys = (job.flags & kFltp) != 0;
 	const uint32_t tile_keys = static_cast<uint32_t>(kTileCfgs[cfg].threads * kTileCfgs[cfg].kpt);
 	const uint32_t tiles = static_cast<uint32_t>((n + tile_keys - 1) / tile_keys);
+	// the MSD plan's partitions of 8-byte keys take 1024x16 tiles where the table picks 1024x12
+	// (longer digit runs; keys only, so the payload staging that rules this tile out elsewhere is absent)
+	const int hcfg = (hybrid && P == 8 && cfg == 3) ? 5 : cfg;
+	const uint32_t htile_keys = hcfg == 5 ? 1024u * 16u : tile_keys;
+	const uint32_t htiles = static_cast<uint32_t>((n + htile_keys - 1) / htile_keys);
 	uint32_t *counts = counts_of(ws);
 	const bool fill = (P == 1) && !job.payload && job.fill8 &&
 		(job.flags & 7u) != (kAbs | kSign) && (job.flags & 7u) != (kSign | kFltp) && (job.flags & 7u) != (kAbs | kSign | kFltp);
@@ -1364,7 +1370,7 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	}
 	{
 		ProfScope ps(0, 0, ws, stream);
-		const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, 8u * cus));
+		const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(hybrid ? htiles : tiles, 8u * cus));
 		if (hybrid) {
 			// The MSD plan needs only the top two digits' histograms. Float keys, whose exponents
 			// cluster, usually fail its bucket estimate; for them the read counts every digit, so the
@@ -1372,11 +1378,11 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 			if constexpr (P >= 4) {
 				if (!float_keys)
 					hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS, true>), dim3(grid), dim3(256), 0, stream,
-						static_cast<const K *>(job.in), n, xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0, ctrl,
+						static_cast<const K *>(job.in), n, xf, hist, counts, htile_keys, htiles, (job.flags & kRev) ? 1 : 0, ctrl,
 						8u * (P - 1), nullptr);
 				else
 					hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS, false>), dim3(grid), dim3(256), 0, stream,
-						static_cast<const K *>(job.in), n, xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0, ctrl,
+						static_cast<const K *>(job.in), n, xf, hist, counts, htile_keys, htiles, (job.flags & kRev) ? 1 : 0, ctrl,
 						8u * (P - 1), nullptr);
 			}
 		} else if (count_tiles) {
@@ -1401,7 +1407,7 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	pa.counts0 = counts0 ? 1 : 0;
 	pa.hybrid = hybrid ? 1 : 0;
 	pa.hyb_cap = hybrid ? kLocalCaps[lcfg] : 0;
-	pa.tile2 = tile_keys;
+	pa.tile2 = htile_keys;
 	pa.segtab = segtab_of(ws);
 	pa.partial = (hybrid && !float_keys) ? 1 : 0;
 	pa.gate = nullptr;
@@ -1410,7 +1416,7 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	if ((e = hipGetLastError()) != hipSuccess) return e;
 	if constexpr (P >= 4) {
 		if (hybrid) {
-			e = launch_hybrid_cfg<K>(cfg, ctrl, bucket, counts, lookback, seg_tables(segtab_of(ws)), xf, n, tiles, lcfg, ws,
+			e = launch_hybrid_cfg<K>(hcfg, ctrl, bucket, counts, lookback, seg_tables(segtab_of(ws)), xf, n, htiles, lcfg, ws,
 				stream);
 			if (e != hipSuccess) return e;
 			// The LSD count + scan plan as the fallback: a full histogram read and a second plan,
