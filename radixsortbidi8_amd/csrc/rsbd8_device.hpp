@@ -94,16 +94,38 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m)
 		__builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
 }
 
-// adds 1 to s[d] for every active lane, with one atomic for the whole wave when all lanes share d
+// Adds 1 to digit d's count for every active lane, with one atomic for the whole wave when all
+// lanes share d. The counts live in C interleaved copies (s[d * C + c], 256 * C words): lane l adds
+// to copy l % C, so a digit that many lanes hold (skewed keys) takes C LDS words in C banks instead
+// of serialising on one. A digit's count is the sum of its C words (copies_sum).
+template <int C = 1>
 __device__ __forceinline__ void wave_count(uint32_t *s, uint32_t d)
 {
+	static_assert(C >= 1 && C <= 64 && (C & (C - 1)) == 0, "copies: a power of two up to 64");
 	const uint64_t active = __ballot(1);
 	const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
 	if (__ballot(d == d0) == active) {
-		if (lanes_below(active) == 0) atomicAdd(&s[d0], static_cast<uint32_t>(__popcll(active)));
+		if (lanes_below(active) == 0) atomicAdd(&s[d0 * C], static_cast<uint32_t>(__popcll(active)));
 	} else {
-		atomicAdd(&s[d], 1u);
+		atomicAdd(&s[d * C + (threadIdx.x & (C - 1))], 1u);
 	}
+}
+
+template <int C>
+__device__ __forceinline__ uint32_t copies_sum(const uint32_t *s, uint32_t d)
+{
+	uint32_t v = 0;
+#pragma unroll
+	for (int c = 0; c < C; ++c) v += s[d * C + c];
+	return v;
+}
+
+// Counter copies for a counting kernel that keeps `copied` 256-word histograms with copies and
+// `plain` ones without in LDS (1 KB per histogram and copy): as many copies as fit 19 KB, so that
+// eight 256-thread blocks still fit one CU's 160 KB, and at most 8.
+constexpr int count_copies(int copied, int plain = 0)
+{
+	return plain + copied * 8 <= 19 ? 8 : (plain + copied * 4 <= 19 ? 4 : (plain + copied * 2 <= 19 ? 2 : 1));
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
@@ -141,23 +163,53 @@ __device__ __forceinline__ void atomic_store_agent(uint64_t *p, uint64_t v)
 
 namespace rsbd8hip {
 
+// Lanes of a full wave (all 64 lanes active) that hold the same 8-bit digit d. Three ways, chosen
+// per wave by how many lanes share lane 0's digit:
+// - all of them: the whole wave;
+// - at least 16 (few distinct digits, so LDS atomics on one word would serialise): 8 ballots;
+// - fewer: every lane ORs its bit into the wave's LDS word for d, reads the word back (a wave's
+//   LDS operations execute in order, so every OR has landed), and the lowest peer clears it.
+// s_peer: the wave's 256 words, zero on entry and left zero.
+__device__ __forceinline__ uint64_t wave_peers(uint32_t d, uint64_t *s_peer)
+{
+	const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+	const uint64_t same = __ballot(d == d0);
+	if (same == ~0ull) return same;
+	if (__popcll(same) >= 16) return match_digit(d);
+	uint64_t *word = s_peer + d;
+	__hip_atomic_fetch_or(word, 1ull << (threadIdx.x & 63u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+	const uint64_t peers = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+	if (lanes_below(peers) == 0) __hip_atomic_store(word, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+	return peers;
+}
+
+// LDS bytes of block_rank's peer words (s_peer): 256 words of 8 bytes per wave
+constexpr size_t rank_peer_bytes(int threads) { return static_cast<size_t>(threads / 64) * 256u * 8u; }
+constexpr size_t max_bytes(size_t a, size_t b) { return a > b ? a : b; }
+
 // Block-wide stable rank of THREADS*KPT digits held in registers in logical order
 // l = wave*64*KPT + j*64 + lane (wave-major, then slot j, then lane). Returns each key's
 // position in the digit-sorted block and leaves, for threads t < 256, the digit's block total in
 // *total and its exclusive digit offset in s_digoff[t]. The three __syncthreads() inside must be
-// reached by every thread of the block. s_whist holds (THREADS/64)*256 words, s_wsum 4 words.
+// reached by every thread of the block. s_whist holds (THREADS/64)*256 words, s_wsum 4 words,
+// s_peer rank_peer_bytes(THREADS) bytes (8-byte aligned) that are free from entry to return: they
+// may share storage with the tile staging, which is written only after the ranking.
 template <int THREADS, int KPT>
 __device__ __forceinline__ void block_rank(const uint32_t (&dig)[KPT], uint32_t (&pos)[KPT], uint32_t *s_whist,
-	uint32_t *s_digoff, uint32_t *s_wsum, uint32_t *total)
+	uint32_t *s_digoff, uint32_t *s_wsum, uint32_t *total, uint64_t *s_peer)
 {
 	constexpr int NW = THREADS / 64;
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
-	for (int i = t; i < NW * 256; i += THREADS) s_whist[i] = 0;
+	for (int i = t; i < NW * 256; i += THREADS) {
+		s_whist[i] = 0;
+		s_peer[i] = 0;
+	}
 	__syncthreads();
 	uint32_t rank[KPT];
+	uint64_t *s_wpeer = s_peer + w * 256;
 #pragma unroll
 	for (int j = 0; j < KPT; ++j) {
-		const uint64_t peers = match_digit(dig[j]);
+		const uint64_t peers = wave_peers(dig[j], s_wpeer);
 		const uint32_t below = lanes_below(peers);
 		uint32_t *slot = &s_whist[w * 256 + dig[j]];
 		const uint32_t old = *slot;

@@ -347,7 +347,9 @@ __global__ __launch_bounds__(THREADS) void k_item_lsd(const K *__restrict__ src,
 	__shared__ uint32_t s_whist[NW * 256];
 	__shared__ uint32_t s_digoff[256];
 	__shared__ uint32_t s_wsum[4];
-	__shared__ K s_keys[CAP];
+	__shared__ __align__(16) unsigned char s_raw[max_bytes(CAP * sizeof(K), rank_peer_bytes(THREADS))];
+	K *s_keys = reinterpret_cast<K *>(s_raw);
+	uint64_t *s_peer = reinterpret_cast<uint64_t *>(s_raw);
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	const uint32_t queued = qcount ? *qcount : qvalue;
 	for (uint32_t qi = blockIdx.x; qi < queued; qi += gridDim.x) {
@@ -367,7 +369,7 @@ __global__ __launch_bounds__(THREADS) void k_item_lsd(const K *__restrict__ src,
 				const uint32_t l = w * 64u * KPT + j * 64u + lane;
 				dig[j] = l < len ? digit_of<K>(keys[j], xf, 8u * p) : 0xFFu;
 			}
-			block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total);
+			block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total, s_peer);
 #pragma unroll
 			for (int j = 0; j < KPT; ++j) s_keys[pos[j]] = keys[j];
 			__syncthreads();
@@ -392,16 +394,17 @@ __global__ __launch_bounds__(THREADS) void k_seg_count1(const K *__restrict__ in
 	uint32_t tile_keys, KeyXform xf, int reverse, uint32_t *__restrict__ counts)
 {
 	constexpr int P = sizeof(K);
-	__shared__ uint32_t s_t[256];
+	constexpr int C = count_copies(1);
+	__shared__ uint32_t s_t[256 * C];
 	const uint32_t g = blockIdx.x;
-	for (int i = threadIdx.x; i < 256; i += THREADS) s_t[i] = 0;
+	for (int i = threadIdx.x; i < 256 * C; i += THREADS) s_t[i] = 0;
 	const SegTile tl = tiles[g];
 	const uint32_t lo = tl.y * tile_keys, hi = min(tl.len, lo + tile_keys);
 	__syncthreads();
 	for_each_key<K, THREADS>(in, tl.sb + (reverse ? tl.len - hi : lo), hi - lo,
-		[&](K x) { wave_count(s_t, digit_of<K>(x, xf, 8u * (P - 1))); });
+		[&](K x) { wave_count<C>(s_t, digit_of<K>(x, xf, 8u * (P - 1))); });
 	__syncthreads();
-	for (int i = threadIdx.x; i < 256; i += THREADS) counts[static_cast<uint64_t>(g) * 256 + i] = s_t[i];
+	for (int i = threadIdx.x; i < 256; i += THREADS) counts[static_cast<uint64_t>(g) * 256 + i] = copies_sum<C>(s_t, i);
 }
 
 // One block per long segment, one thread per top digit: bucket sizes and starts, the segment's
@@ -501,7 +504,10 @@ __global__ __launch_bounds__(THREADS) void k_seg_scatter1(const K *__restrict__ 
 	__shared__ uint32_t s_digoff[256];
 	__shared__ uint32_t s_glob[256];
 	__shared__ uint32_t s_wsum[4];
-	__shared__ K s_keys[TILE];
+	__shared__ __align__(16) unsigned char s_raw[STABLE ? max_bytes(TILE * sizeof(K), rank_peer_bytes(THREADS)) : TILE * sizeof(K)];
+	K *s_keys = reinterpret_cast<K *>(s_raw);
+	uint64_t *s_peer = reinterpret_cast<uint64_t *>(s_raw);
+	(void)s_peer;
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	(void)lane;
 	const uint32_t per = (ntiles + 7) / 8;
@@ -537,7 +543,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_scatter1(const K *__restrict__ 
 			}
 			if constexpr (STABLE) {
 				uint32_t pos[KPT], total;
-				block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total);
+				block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total, s_peer);
 				if (t < 256) s_glob[t] = offsets[static_cast<uint64_t>(cur) * 256 + t] - s_digoff[t];
 #pragma unroll
 				for (int j = 0; j < KPT; ++j) s_keys[pos[j]] = keys[j];
@@ -597,22 +603,25 @@ __global__ __launch_bounds__(THREADS) void k_seg_hist(const K *__restrict__ in, 
 	const uint32_t *__restrict__ ftiles, const SegCtrl *ctrl, uint32_t tile_keys, KeyXform xf, uint32_t *__restrict__ hist)
 {
 	constexpr int P = sizeof(K);
-	__shared__ uint32_t s_h[P * 256];
+	constexpr int C = count_copies(P);
+	__shared__ uint32_t s_h[P * 256 * C];
 	const uint32_t nf = ctrl->n_ftiles;  // tiles of the segments flagged by k_seg_scan1
 	for (uint32_t q = blockIdx.x; q < nf; q += gridDim.x) {
 		const SegTile tl = tiles[ftiles[q]];
-		for (int i = threadIdx.x; i < P * 256; i += THREADS) s_h[i] = 0;
+		for (int i = threadIdx.x; i < P * 256 * C; i += THREADS) s_h[i] = 0;
 		__syncthreads();
 		const uint32_t lo = tl.y * tile_keys, hi = min(tl.len, lo + tile_keys);
 		for_each_key<K, THREADS>(in, tl.sb + lo, hi - lo, [&](K x) {
 			const auto k = sort_key<K>(x, xf);
 #pragma unroll
-			for (int p = 0; p < P; ++p) wave_count(s_h + p * 256, static_cast<uint32_t>(k >> (8 * p)) & 0xFFu);
+			for (int p = 0; p < P; ++p) wave_count<C>(s_h + p * 256 * C, static_cast<uint32_t>(k >> (8 * p)) & 0xFFu);
 		});
 		__syncthreads();
 		uint32_t *h = hist + static_cast<uint64_t>(tl.L) * P * 256;
-		for (int i = threadIdx.x; i < P * 256; i += THREADS)
-			if (s_h[i]) atomicAdd(&h[i], s_h[i]);
+		for (int i = threadIdx.x; i < P * 256; i += THREADS) {
+			const uint32_t v = copies_sum<C>(s_h, i);
+			if (v) atomicAdd(&h[i], v);
+		}
 		__syncthreads();
 	}
 }
@@ -685,7 +694,9 @@ __global__ __launch_bounds__(THREADS) void k_seg_pass(const K *__restrict__ in, 
 	__shared__ uint32_t s_glob[256];
 	__shared__ uint32_t s_wsum[4];
 	__shared__ uint32_t s_tile;
-	__shared__ K s_keys[TILE];
+	__shared__ __align__(16) unsigned char s_raw[max_bytes(TILE * sizeof(K), rank_peer_bytes(THREADS))];
+	K *s_keys = reinterpret_cast<K *>(s_raw);
+	uint64_t *s_peer = reinterpret_cast<uint64_t *>(s_raw);
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	const uint32_t shift = 8u * static_cast<uint32_t>(pass);
 	const uint32_t tag = ((ctrl->epoch * kMaxPasses + static_cast<uint32_t>(pass)) % 0x3FFFFFFFu) + 1u;
@@ -723,7 +734,7 @@ __global__ __launch_bounds__(THREADS) void k_seg_pass(const K *__restrict__ in, 
 				dig[j] = 0xFFu;
 			}
 		}
-		block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total);
+		block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total, s_peer);
 		if (t < 256) {
 			const uint32_t pub = total - ((t == 255) ? static_cast<uint32_t>(TILE) - valid : 0u);
 			uint64_t *lb = lookback + static_cast<uint64_t>(g) * 256;

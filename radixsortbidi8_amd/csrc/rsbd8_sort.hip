@@ -84,8 +84,9 @@ __global__ __launch_bounds__(THREADS) void k_hist(const K *__restrict__ in, uint
 {
 	if (presort_result(ctrl->check, reverse)) return;  // the input is already in order
 	constexpr int P = sizeof(K);
-	__shared__ uint32_t s_h[P * kRadix];
-	for (int i = threadIdx.x; i < P * kRadix; i += THREADS) s_h[i] = 0;
+	constexpr int C = count_copies(P);
+	__shared__ uint32_t s_h[P * kRadix * C];
+	for (int i = threadIdx.x; i < P * kRadix * C; i += THREADS) s_h[i] = 0;
 	__syncthreads();
 	// one contiguous, 64-key aligned range per block
 	const uint64_t chunk = ((n + gridDim.x - 1) / gridDim.x + 63) & ~63ull;
@@ -95,12 +96,14 @@ __global__ __launch_bounds__(THREADS) void k_hist(const K *__restrict__ in, uint
 		for_each_key<K, THREADS>(in, lo, len, [&](K x) {
 			const auto k = sort_key<K>(x, xf);
 #pragma unroll
-			for (int p = 0; p < P; ++p) wave_count(s_h + p * kRadix, static_cast<uint32_t>(k >> (8 * p)) & 0xFFu);
+			for (int p = 0; p < P; ++p) wave_count<C>(s_h + p * kRadix * C, static_cast<uint32_t>(k >> (8 * p)) & 0xFFu);
 		});
 	}
 	__syncthreads();
-	for (int i = threadIdx.x; i < P * kRadix; i += THREADS)
-		if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+	for (int i = threadIdx.x; i < P * kRadix; i += THREADS) {
+		const uint32_t v = copies_sum<C>(s_h, i);
+		if (v) atomicAdd(&hist[i], v);
+	}
 }
 
 
@@ -142,7 +145,9 @@ __global__ __launch_bounds__(THREADS) void k_lds_sort(const K *__restrict__ in, 
 	__shared__ uint32_t s_whist[NW * 256];
 	__shared__ uint32_t s_digoff[256];
 	__shared__ uint32_t s_wsum[4];
-	__shared__ K s_keys[CAP];
+	__shared__ __align__(16) unsigned char s_raw[max_bytes(CAP * sizeof(K), rank_peer_bytes(THREADS))];
+	K *s_keys = reinterpret_cast<K *>(s_raw);
+	uint64_t *s_peer = reinterpret_cast<uint64_t *>(s_raw);
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	K keys[KPT];
 #pragma unroll
@@ -157,7 +162,7 @@ __global__ __launch_bounds__(THREADS) void k_lds_sort(const K *__restrict__ in, 
 			const uint32_t l = w * 64u * KPT + j * 64u + lane;
 			dig[j] = l < n ? digit_of<K>(keys[j], xf, 8u * p) : 0xFFu;
 		}
-		block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total);
+		block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total, s_peer);
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) s_keys[pos[j]] = keys[j];
 		__syncthreads();
@@ -381,15 +386,20 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(Control *ctrl, const uint3
 	__shared__ uint32_t s_tile;
 	__shared__ uint8_t s_dig[PAY ? TILE : 1];
 	constexpr int kStage = (PAY && sizeof(K) < 4) ? 4 : static_cast<int>(sizeof(K));
-	__shared__ __align__(16) unsigned char s_raw[TILE * kStage];
+	// the tile staging, and before it the ranking's peer words
+	__shared__ __align__(16) unsigned char s_raw[max_bytes(TILE * kStage, rank_peer_bytes(THREADS))];
 	K *s_keys = reinterpret_cast<K *>(s_raw);
+	uint64_t *s_peer = reinterpret_cast<uint64_t *>(s_raw);
 
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	const uint32_t *pbucket = bucket + pass * kRadix;
 
 	for (;;) {
 		if (t == 0) s_tile = atomicAdd(&ctrl->tile_counter[pass], 1u);
-		for (int i = t; i < NW * kRadix; i += THREADS) s_whist[i] = 0;
+		for (int i = t; i < NW * kRadix; i += THREADS) {
+			s_whist[i] = 0;
+			s_peer[i] = 0;
+		}
 		__syncthreads();
 		const uint32_t tile = s_tile;
 		if (tile >= num_tiles) break;
@@ -418,7 +428,7 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(Control *ctrl, const uint3
 		// rank within the wave in logical order (j-major, lane-minor)
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) {
-			const uint64_t peers = match_digit(dig[j]);
+			const uint64_t peers = wave_peers(dig[j], s_peer + w * kRadix);
 			const uint32_t below = lanes_below(peers);
 			uint32_t *slot = &s_whist[w * kRadix + dig[j]];
 			const uint32_t old = *slot;
@@ -505,13 +515,21 @@ __global__ __launch_bounds__(THREADS) void k_hist_tiles(const K *__restrict__ in
 	constexpr int P = sizeof(K);
 	constexpr int P0 = PARTIAL ? P - 2 : 0;  // first digit counted
 	using W = typename KeyTraits<K>::W;
-	__shared__ uint32_t s_h[(P - P0) * kRadix];
-	__shared__ uint32_t s_t[kRadix];
+	// The top two digits (the skewed ones for floats and clustered integers) and the tile counts
+	// keep C counter copies, the digits below them one; digit p's histogram starts at s_h[hoff(p)].
+	constexpr int NLOW = P - 2 - P0 > 0 ? P - 2 - P0 : 0;
+	constexpr int NTOP = P - P0 - NLOW;
+	constexpr int C = count_copies(NTOP + 1, NLOW);
+	constexpr int HWORDS = (NLOW + NTOP * C) * kRadix;
+	auto hoff = [](int p) { return p - P0 < NLOW ? (p - P0) * kRadix : (NLOW + (p - P0 - NLOW) * C) * kRadix; };
+	auto hcop = [](int p) { return p - P0 < NLOW ? 1 : C; };
+	__shared__ uint32_t s_h[HWORDS];
+	__shared__ uint32_t s_t[kRadix * C];
 	__shared__ W s_or[THREADS / 64], s_nor[THREADS / 64];
 	W k_or = 0, k_nor = 0;
-	for (int i = threadIdx.x; i < (P - P0) * kRadix; i += THREADS) s_h[i] = 0;
+	for (int i = threadIdx.x; i < HWORDS; i += THREADS) s_h[i] = 0;
 	for (uint32_t tile = blockIdx.x; tile < num_tiles; tile += gridDim.x) {
-		for (int i = threadIdx.x; i < kRadix; i += THREADS) s_t[i] = 0;
+		for (int i = threadIdx.x; i < kRadix * C; i += THREADS) s_t[i] = 0;
 		__syncthreads();
 		const uint64_t base = static_cast<uint64_t>(tile) * tile_keys;
 		const uint32_t len = (n - base) < tile_keys ? static_cast<uint32_t>(n - base) : tile_keys;
@@ -520,18 +538,26 @@ __global__ __launch_bounds__(THREADS) void k_hist_tiles(const K *__restrict__ in
 		for_each_key<K, THREADS>(in, lo, len, [&](K x) {
 			const auto k = sort_key<K>(x, xf);
 #pragma unroll
-			for (int p = P0; p < P; ++p) wave_count(s_h + (p - P0) * kRadix, static_cast<uint32_t>(k >> (8 * p)) & 0xFFu);
-			if (counts) wave_count(s_t, static_cast<uint32_t>(k >> count_shift) & 0xFFu);
+			for (int p = P0; p < P; ++p) {
+				const uint32_t d = static_cast<uint32_t>(k >> (8 * p)) & 0xFFu;
+				if (p - P0 < NLOW) wave_count<1>(s_h + hoff(p), d);
+				else wave_count<C>(s_h + hoff(p), d);
+			}
+			if (counts) wave_count<C>(s_t, static_cast<uint32_t>(k >> count_shift) & 0xFFu);
 			if constexpr (PARTIAL) { k_or |= k; k_nor |= ~k; }
 		});
 		__syncthreads();
 		if (counts)
-			for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kCountStride + i] = s_t[i];
+			for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kCountStride + i] = copies_sum<C>(s_t, i);
 		__syncthreads();
 	}
 	__syncthreads();
-	for (int i = threadIdx.x; i < (P - P0) * kRadix; i += THREADS)
-		if (s_h[i]) atomicAdd(&hist[P0 * kRadix + i], s_h[i]);
+	for (int i = threadIdx.x; i < (P - P0) * kRadix; i += THREADS) {
+		const int p = P0 + (i >> 8);
+		const uint32_t d = static_cast<uint32_t>(i) & 0xFFu;
+		const uint32_t v = hcop(p) == 1 ? s_h[hoff(p) + d] : copies_sum<C>(s_h + hoff(p), d);
+		if (v) atomicAdd(&hist[P0 * kRadix + i], v);
+	}
 	if constexpr (PARTIAL) {
 #pragma unroll
 		for (int o = 32; o > 0; o >>= 1) {
@@ -559,16 +585,17 @@ __global__ __launch_bounds__(THREADS) void k_count(const Control *ctrl, int pass
 	if (!pp.active || (ctrl->counts_ready >> pass & 1u)) return;
 	const K *__restrict__ src = static_cast<const K *>(pp.src);
 	const uint32_t shift = 8u * static_cast<uint32_t>(pass);
-	__shared__ uint32_t s_t[kRadix];
+	constexpr int C = count_copies(1);
+	__shared__ uint32_t s_t[kRadix * C];
 	for (uint32_t tile = blockIdx.x; tile < num_tiles; tile += gridDim.x) {
-		for (int i = threadIdx.x; i < kRadix; i += THREADS) s_t[i] = 0;
+		for (int i = threadIdx.x; i < kRadix * C; i += THREADS) s_t[i] = 0;
 		__syncthreads();
 		const uint64_t base = static_cast<uint64_t>(tile) * tile_keys;
 		const uint32_t len = (n - base) < tile_keys ? static_cast<uint32_t>(n - base) : tile_keys;
 		const uint64_t lo = pp.reverse ? n - base - len : base;
-		for_each_key<K, THREADS>(src, lo, len, [&](K x) { wave_count(s_t, digit_of<K>(x, xf, shift)); });
+		for_each_key<K, THREADS>(src, lo, len, [&](K x) { wave_count<C>(s_t, digit_of<K>(x, xf, shift)); });
 		__syncthreads();
-		for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kCountStride + i] = s_t[i];
+		for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kCountStride + i] = copies_sum<C>(s_t, i);
 		__syncthreads();
 	}
 }
@@ -669,8 +696,10 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 	__shared__ uint32_t s_ts[SEG ? kRadix + 1 : 1];
 	__shared__ uint8_t s_dig[PAY ? TILE : 1];
 	constexpr int kStage = (PAY && sizeof(K) < 4) ? 4 : static_cast<int>(sizeof(K));
-	__shared__ __align__(16) unsigned char s_raw[TILE * kStage];
+	// the tile staging, and before it the ranking's peer words
+	__shared__ __align__(16) unsigned char s_raw[GROUP ? TILE * kStage : max_bytes(TILE * kStage, rank_peer_bytes(THREADS))];
 	K *s_keys = reinterpret_cast<K *>(s_raw);
+	uint64_t *s_peer = reinterpret_cast<uint64_t *>(s_raw);
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	const uint32_t per = (num_tiles + 7) / 8;
 	const uint32_t x = blockIdx.x & 7u, step = gridDim.x >> 3;
@@ -725,13 +754,14 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 		if constexpr (GROUP) {
 			static_assert(!PAY, "grouping without order is for keys only");
 			(void)total;
+			(void)s_peer;
 			block_group<THREADS, KPT>(dig, valid, pos, s_whist, s_digoff, s_wsum);
 			if (t < kRadix) s_glob[t] = offsets[static_cast<uint64_t>(tile) * kCountStride + t] - s_digoff[t];
 #pragma unroll
 			for (int j = 0; j < KPT; ++j)
 				if (pos[j] != 0xFFFFFFFFu) s_keys[pos[j]] = keys[j];
 		} else {
-			block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total);
+			block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total, s_peer);
 			if (t < kRadix) s_glob[t] = offsets[static_cast<uint64_t>(tile) * kCountStride + t] - s_digoff[t];
 #pragma unroll
 			for (int j = 0; j < KPT; ++j) s_keys[pos[j]] = keys[j];
@@ -789,20 +819,21 @@ __global__ __launch_bounds__(THREADS) void k_hyb_count(const PassPlan *ppp, uint
 	const PassPlan pp = *ppp;
 	if (!pp.active) return;
 	const K *__restrict__ src = static_cast<const K *>(pp.src);
+	constexpr int C = count_copies(1);
 	__shared__ uint32_t s_ts[kRadix + 1];
-	__shared__ uint32_t s_t[kRadix];
+	__shared__ uint32_t s_t[kRadix * C];
 	for (int i = threadIdx.x; i <= kRadix; i += THREADS) s_ts[i] = seg.tile_start[i];
 	__syncthreads();
 	const uint32_t num = s_ts[kRadix];
 	for (uint32_t tile = blockIdx.x; tile < num; tile += gridDim.x) {
-		for (int i = threadIdx.x; i < kRadix; i += THREADS) s_t[i] = 0;
+		for (int i = threadIdx.x; i < kRadix * C; i += THREADS) s_t[i] = 0;
 		__syncthreads();
 		uint64_t base;
 		uint32_t len;
 		seg_tile(s_ts, seg, tile_keys, tile, base, len);
-		for_each_key<K, THREADS>(src, base, len, [&](K x) { wave_count(s_t, digit_of<K>(x, xf, shift)); });
+		for_each_key<K, THREADS>(src, base, len, [&](K x) { wave_count<C>(s_t, digit_of<K>(x, xf, shift)); });
 		__syncthreads();
-		for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kCountStride + i] = s_t[i];
+		for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kCountStride + i] = copies_sum<C>(s_t, i);
 		__syncthreads();
 	}
 }
@@ -993,7 +1024,9 @@ __global__ __launch_bounds__(THREADS) void k_local_lsd(const Control *ctrl, KeyX
 	__shared__ uint32_t s_whist[NW * kRadix];
 	__shared__ uint32_t s_digoff[kRadix];
 	__shared__ uint32_t s_wsum[4];
-	__shared__ __align__(16) K s_raw[CAP];
+	__shared__ __align__(16) unsigned char s_buf[max_bytes(CAP * sizeof(K), rank_peer_bytes(THREADS))];
+	K *s_raw = reinterpret_cast<K *>(s_buf);
+	uint64_t *s_peer = reinterpret_cast<uint64_t *>(s_buf);
 	const PassPlan pp = ctrl->hyb[2];
 	if (!pp.active) return;
 	const uint32_t queued = ctrl->hyb_queue;
@@ -1016,7 +1049,7 @@ __global__ __launch_bounds__(THREADS) void k_local_lsd(const Control *ctrl, KeyX
 				const uint32_t l = w * 64u * KPT + j * 64u + lane;
 				dig[j] = l < len ? digit_of<K>(key[j], xf, 8u * p) : 0xFFu;
 			}
-			block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total);
+			block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total, s_peer);
 #pragma unroll
 			for (int j = 0; j < KPT; ++j) s_raw[pos[j]] = key[j];
 			__syncthreads();
@@ -1352,8 +1385,9 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	const uint32_t tile_keys = static_cast<uint32_t>(kTileCfgs[cfg].threads * kTileCfgs[cfg].kpt);
 	const uint32_t tiles = static_cast<uint32_t>((n + tile_keys - 1) / tile_keys);
 	// the MSD plan's partitions of 8-byte keys take 1024x16 tiles where the table picks 1024x12
-	// (longer digit runs; keys only, so the payload staging that rules this tile out elsewhere is absent)
-	const int hcfg = (hybrid && P == 8 && cfg == 3) ? 5 : cfg;
+	// (longer digit runs; keys only, so the payload staging that rules this tile out elsewhere is
+	// absent). Only the grouping form: the ranked form of that tile spills registers.
+	const int hcfg = (hybrid && P == 8 && cfg == 3 && xform_injective(xf, P)) ? 5 : cfg;
 	const uint32_t htile_keys = hcfg == 5 ? 1024u * 16u : tile_keys;
 	const uint32_t htiles = static_cast<uint32_t>((n + htile_keys - 1) / htile_keys);
 	uint32_t *counts = counts_of(ws);
@@ -1535,12 +1569,16 @@ int default_algo(int key_bytes, uint64_t n, bool keys_only)
 	return n >= (1u << 22) ? kAlgoCountScan : kAlgoOnesweep;
 }
 
-// measured on MI355X: 1024x12 tiles (long digit runs per tile) for 8-byte keys from 2^22 keys
+// measured on MI355X: 1024x12 tiles (long digit runs per tile) for 8-byte keys from 2^22 keys, and
+// for every sort with a payload from 2^26 keys
 int default_cfg(int key_bytes, uint64_t n, bool payload)
 {
 	// the autotuned table's choices at these sizes (uniform keys)
 	if (n <= (1u << 16)) return 0;
 	if (!payload && default_algo(key_bytes, n, true) == kAlgoHybrid) return 3;  // MSD partitions: 1024x12
+	// measured with a u32 payload: at 2^28 keys of 1, 2 or 4 bytes, 1024x12 tiles (longer digit
+	// runs per tile) take 0.74-0.92x the time of 256x16 tiles; at 2^24 every shape is within 6%
+	if (payload && n >= (1u << 26)) return 3;
 	if (default_algo(key_bytes, n, false) == kAlgoCountScan) return key_bytes >= 8 ? 3 : (key_bytes == 2 ? 2 : 1);
 	return key_bytes >= 8 ? 4 : 2;
 }
