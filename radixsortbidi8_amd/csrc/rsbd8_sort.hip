@@ -7,12 +7,14 @@
 //               LDS (wave-uniform digits are added with one atomic per wave);
 //   k_plan      one block: per-digit exclusive scans (bucket starts), uniform-digit detection
 //               (the pass-skip flag of H:11652) and the ping-pong buffer plan, all on device;
-//   k_onesweep  one launch per digit: persistent blocks take tiles in order, rank digits with
-//               wave64 ballot matching, publish per-tile digit counts and resolve the global
+//   k_onesweep  one launch per digit: persistent blocks take tiles in order, rank digits stably
+//               (LDS peer words, or ballots in skewed waves), publish per-tile digit counts and resolve the global
 //               offsets with a decoupled look-back, stage the tile in LDS in digit order and
 //               write coalesced runs; inactive (uniform-digit) passes exit at once;
 //   k_copy      copies when no pass runs or an in-place sort needs a staging copy;
 //   k_fill8     the 8-bit count fill of H:48769-48957 for bijective modes;
+//   k_hist16, k_scan16, k_fill16
+//               the same count fill for 16-bit keys (65536 counts) from 2^22 keys;
 //   k_finish    resets the tile counters and advances the look-back epoch.
 // The transform of every mode is fused into each load (rsbd8_device.hpp).
 #include <hip/hip_runtime.h>
@@ -1088,6 +1090,117 @@ __global__ __launch_bounds__(256) void k_fill8(const uint32_t *__restrict__ buck
 	}
 }
 
+// ---------------------------------------------------------------------------------------------
+// 16-bit count fill (keys only, bijective modes): as in the 8-bit fill, the sorted output follows
+// from the counts of the sort keys alone, here all 65536 of them. It replaces the two stable 8-bit
+// passes the reference runs for 16-bit keys (digit plan H:1848, passes H:31774-31982); the output
+// is the same sequence.
+
+constexpr uint32_t kFill16Half = 32768;    // sort keys per LDS histogram (128 KB)
+// smallest sort that takes the fill: measured on uniform keys, at 2^21 the fill and the two passes
+// take the same time, at 2^22 the fill takes 0.75x and at 2^28 0.23x
+constexpr uint64_t kFill16Min = 1u << 22;
+constexpr uint32_t kFill16Words = 65536 + 65536 + 65536 / 2;  // hist, starts, inverse (u16); then the chunk map
+
+// Block b counts the sort keys k with k >> 15 == b & 1 (one half of the key space) over chunk
+// b >> 1 of the input, in LDS, and adds its nonzero counts to hist (65536 words, zero on entry).
+// Every key is read by two blocks: reading twice costs less than counters that do not fit LDS.
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_hist16(const uint16_t *__restrict__ in, uint64_t n, KeyXform xf,
+	uint32_t *__restrict__ hist)
+{
+	__shared__ uint32_t s_h[kFill16Half];
+	for (uint32_t i = threadIdx.x; i < kFill16Half; i += THREADS) s_h[i] = 0;
+	__syncthreads();
+	const uint32_t half = blockIdx.x & 1u, chunks = gridDim.x >> 1, c = blockIdx.x >> 1;
+	const uint64_t per = ((n + chunks - 1) / chunks + 63) & ~63ull;
+	const uint64_t lo = static_cast<uint64_t>(c) * per;
+	if (lo < n) {
+		const uint32_t len = static_cast<uint32_t>((n - lo) < per ? n - lo : per);
+		for_each_key<uint16_t, THREADS>(in, lo, len, [&](uint16_t x) {
+			const uint32_t k = sort_key<uint16_t>(x, xf) & 0xFFFFu;
+			if ((k >> 15) == half) wave_count<1>(s_h, k & (kFill16Half - 1u));
+		});
+	}
+	__syncthreads();
+	for (uint32_t i = threadIdx.x; i < kFill16Half; i += THREADS)
+		if (s_h[i]) atomicAdd(&hist[half * kFill16Half + i], s_h[i]);
+}
+
+// One block: start[v] = the number of keys with a sort key below v, and inv[f(x)] = x (the
+// bijective transform's inverse).
+__global__ __launch_bounds__(1024) void k_scan16(const uint32_t *__restrict__ hist, uint32_t *__restrict__ start,
+	uint16_t *__restrict__ inv, KeyXform xf)
+{
+	__shared__ uint32_t s_w[16];
+	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+	uint32_t sum = 0;
+	for (uint32_t j = 0; j < 64; ++j) sum += hist[t * 64u + j];
+	const uint32_t incl = wave_inclusive_sum(sum);
+	if (lane == 63) s_w[w] = incl;
+	__syncthreads();
+	uint32_t run = incl - sum;
+	for (uint32_t v = 0; v < w; ++v) run += s_w[v];
+	for (uint32_t j = 0; j < 64; ++j) {
+		start[t * 64u + j] = run;
+		run += hist[t * 64u + j];
+	}
+	for (uint32_t x = t; x < 65536u; x += 1024u) inv[sort_key<uint16_t>(static_cast<uint16_t>(x), xf) & 0xFFFFu] = static_cast<uint16_t>(x);
+}
+
+constexpr uint32_t kFill16Chunk = 256 * 8;  // output positions per block and step of k_fill16
+
+// largest bucket v in [lo, hi) with start[v] <= i, given start[lo] <= i
+__device__ __forceinline__ uint32_t fill16_bucket(const uint32_t *__restrict__ start, uint32_t lo, uint32_t hi, uint64_t i)
+{
+	while (hi - lo > 1) {
+		const uint32_t mid = (lo + hi) >> 1;
+		if (start[mid] <= i) lo = mid;
+		else hi = mid;
+	}
+	return lo;
+}
+
+// map[c] = the bucket holding position c * kFill16Chunk for the chunks c < chunks, map[chunks] =
+// 65535: chunk c's positions lie in the buckets map[c] .. map[c + 1]. One thread per chunk, so the
+// searches' memory latencies overlap.
+__global__ __launch_bounds__(256) void k_map16(const uint32_t *__restrict__ start, uint32_t *__restrict__ map, uint32_t chunks)
+{
+	const uint32_t c = blockIdx.x * 256u + threadIdx.x;
+	if (c < chunks) map[c] = fill16_bucket(start, 0, 65536, static_cast<uint64_t>(c) * kFill16Chunk);
+	else if (c == chunks) map[c] = 65535;
+}
+
+// out[i] = inv[v] for the bucket v that holds position i (the last v with start[v] <= i), eight
+// consecutive positions per thread, searching only the buckets of the chunk.
+__global__ __launch_bounds__(256) void k_fill16(const uint32_t *__restrict__ start, const uint32_t *__restrict__ map,
+	const uint16_t *__restrict__ inv, uint16_t *__restrict__ out, uint64_t n)
+{
+	const uint64_t nchunks = (n + kFill16Chunk - 1) / kFill16Chunk;
+	const bool vec = (reinterpret_cast<uintptr_t>(out) & 15u) == 0;
+	for (uint64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+		const uint64_t i0 = ch * kFill16Chunk + threadIdx.x * 8ull;
+		if (i0 >= n) continue;
+		const uint32_t vlo = map[ch], vend = map[ch + 1] + 1;
+		uint32_t v = fill16_bucket(start, vlo, vend, i0);
+		uint16_t val[8];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			const uint64_t i = i0 + j;
+			if (v + 1 < vend && start[v + 1] <= i) v = fill16_bucket(start, v + 1, vend, i);
+			val[j] = inv[v];
+		}
+		if (vec && i0 + 8 <= n) {
+			uint4 u;
+			__builtin_memcpy(&u, val, 16);
+			*reinterpret_cast<uint4 *>(out + i0) = u;
+		} else {
+			for (int j = 0; j < 8; ++j)
+				if (i0 + j < n) out[i0 + j] = val[j];
+		}
+	}
+}
+
 __global__ void k_finish(Control *ctrl)
 {
 	if (threadIdx.x < kMaxPasses) {
@@ -1355,6 +1468,41 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	hipError_t e;
 
 	constexpr int HITEMS = 16 / sizeof(K) < 4 ? 4 : 16 / sizeof(K) * 2;
+	const bool bijective = (job.flags & 7u) != (kAbs | kSign) && (job.flags & 7u) != (kSign | kFltp) &&
+		(job.flags & 7u) != (kAbs | kSign | kFltp);
+	if constexpr (P == 2) {
+		if (!job.payload && job.fill8 && bijective && n >= kFill16Min && !job.algo) {
+			// 16-bit count fill. Its tables borrow the first tile records (look-back rows included),
+			// which are zeroed again afterwards, as a fresh workspace has them.
+			// tables plus the chunk map fit the records: 640 KB + 4 B per 2048 keys <= 1.5 B per key
+			static_assert(kFill16Words * 4 + (kFill16Min / kFill16Chunk + 2) * 4 <= (kFill16Min / kMinTile) * kRecordBytes,
+				"fill tables exceed the records");
+			uint32_t *h16 = reinterpret_cast<uint32_t *>(lookback_of(ws));
+			uint32_t *st16 = h16 + 65536;
+			uint16_t *inv16 = reinterpret_cast<uint16_t *>(st16 + 65536);
+			uint32_t *map16 = h16 + kFill16Words;
+			const uint32_t chunks = static_cast<uint32_t>((n + kFill16Chunk - 1) / kFill16Chunk);
+			const uint64_t used_words = kFill16Words + chunks + 1u;
+			ProfScope ps(0, 0, ws, stream);
+			hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, stream, reinterpret_cast<uint64_t *>(h16), uint64_t{65536 / 2});
+			if ((e = hipGetLastError()) != hipSuccess) return e;
+			const uint32_t hblocks = static_cast<uint32_t>(std::max(1, cus / 2)) * 2u;
+			hipLaunchKernelGGL(k_hist16<1024>, dim3(hblocks), dim3(1024), 0, stream, static_cast<const uint16_t *>(job.in), n, xf,
+				h16);
+			if ((e = hipGetLastError()) != hipSuccess) return e;
+			hipLaunchKernelGGL(k_scan16, dim3(1), dim3(1024), 0, stream, h16, st16, inv16, xf);
+			if ((e = hipGetLastError()) != hipSuccess) return e;
+			hipLaunchKernelGGL(k_map16, dim3((chunks + 1 + 255) / 256), dim3(256), 0, stream, st16, map16, chunks);
+			if ((e = hipGetLastError()) != hipSuccess) return e;
+			const uint32_t fgrid = std::max<uint32_t>(1u, std::min<uint32_t>(chunks, 8u * cus));
+			hipLaunchKernelGGL(k_fill16, dim3(fgrid), dim3(256), 0, stream, st16, map16, inv16, static_cast<uint16_t *>(job.out), n);
+			if ((e = hipGetLastError()) != hipSuccess) return e;
+			hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, stream, reinterpret_cast<uint64_t *>(h16), (used_words + 1) / 2);
+			if ((e = hipGetLastError()) != hipSuccess) return e;
+			hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, stream, ctrl);
+			return hipGetLastError();
+		}
+	}
 	const int cfg = job.cfg >= 0 ? job.cfg : default_cfg(P, n, job.payload);
 	const bool keys_only = !job.payload && !job.keys_final_any;
 	int algo = job.algo ? job.algo : default_algo(P, n, keys_only);
@@ -1391,8 +1539,7 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	const uint32_t htile_keys = hcfg == 5 ? 1024u * 16u : tile_keys;
 	const uint32_t htiles = static_cast<uint32_t>((n + htile_keys - 1) / htile_keys);
 	uint32_t *counts = counts_of(ws);
-	const bool fill = (P == 1) && !job.payload && job.fill8 &&
-		(job.flags & 7u) != (kAbs | kSign) && (job.flags & 7u) != (kSign | kFltp) && (job.flags & 7u) != (kAbs | kSign | kFltp);
+	const bool fill = (P == 1) && !job.payload && job.fill8 && bijective;
 	// the histogram read also counts, per tile, the digit of the first pass: digit 0 for the LSD
 	// count + scan plan, the top digit for the MSD plan
 	const bool counts0 = algo == kAlgoCountScan && !fill;
