@@ -899,7 +899,7 @@ __global__ void k_hyb_check(Control *ctrl)
 // destination may coincide.
 constexpr uint32_t kLocalRunMax = 96;
 
-template <typename K, int THREADS, int KPT, int PB>
+template <typename K, int THREADS, int KPT, int PB, bool INJ>
 __global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, const uint32_t *__restrict__ bstart,
 	const uint32_t *__restrict__ blen, uint32_t *__restrict__ queue)
 {
@@ -931,6 +931,9 @@ __global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, c
 		if (t == 0 && pp.src != pp.dst) dst[0] = src[0];
 		return;
 	}
+	// the top 16 bits of the sort key, which every key of the bucket shares (read before any store:
+	// source and destination may coincide)
+	const W pre = INJ ? (sort_key<K>(src[0], xf) >> (8 * P - 16)) << (8 * P - 16) : W(0);
 	K key[KPT];
 #pragma unroll
 	for (int j = 0; j < KPT; ++j) {
@@ -985,6 +988,7 @@ __global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, c
 	uint16_t *s_fin = reinterpret_cast<uint16_t *>(s_u);
 	uint32_t fin[KPT];
 	uint32_t at[KPT];
+	W xw[INJ ? KPT : 1];
 #pragma unroll
 	for (int j = 0; j < KPT; ++j) {
 		const uint32_t i = j * THREADS + t;
@@ -997,7 +1001,21 @@ __global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, c
 			for (uint32_t q = lo; q < hi; ++q) n_less += static_cast<uint32_t>(s_t[q] < x);
 			fin[j] = lo + n_less;
 			at[j] = static_cast<uint32_t>(x) & 0xFFFFu;
+			if constexpr (INJ) xw[j] = x;
 		}
+	}
+	if constexpr (INJ) {
+		// Equal sort keys are equal keys (the transform only complements bits), so each word gives
+		// back its raw key: the bucket's top 16 bits, the word's key bits, the transform undone.
+		// The keys then go straight to their final places, with no exchange through their owners.
+		__syncthreads();  // every run has been read: s_t takes the keys in final order
+		K *s_out = reinterpret_cast<K *>(s_t);
+#pragma unroll
+		for (int j = 0; j < KPT; ++j)
+			if (at[j] != 0xFFFFFFFFu) s_out[fin[j]] = static_cast<K>(((xw[j] >> 16) | pre) ^ static_cast<W>(xf.E));
+		__syncthreads();
+		for (uint32_t i = t; i < len; i += THREADS) dst[i] = s_out[i];
+		return;
 	}
 	__syncthreads();  // the cursors are dead: s_u now holds the final positions
 #pragma unroll
@@ -1368,8 +1386,12 @@ int local_cfg_for(uint64_t n)
 template <typename K, int THREADS, int KPT, int PB>
 hipError_t launch_local_cfg(Control *ctrl, const KeyXform &xf, const SegTables &T, hipStream_t stream)
 {
-	hipLaunchKernelGGL((k_local<K, THREADS, KPT, PB>), dim3(kHybBuckets), dim3(THREADS), 0, stream, ctrl, xf, T.bucket_start,
-		T.bucket_len, T.queue);
+	if (xform_injective(xf, sizeof(K)))
+		hipLaunchKernelGGL((k_local<K, THREADS, KPT, PB, true>), dim3(kHybBuckets), dim3(THREADS), 0, stream, ctrl, xf,
+			T.bucket_start, T.bucket_len, T.queue);
+	else
+		hipLaunchKernelGGL((k_local<K, THREADS, KPT, PB, false>), dim3(kHybBuckets), dim3(THREADS), 0, stream, ctrl, xf,
+			T.bucket_start, T.bucket_len, T.queue);
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess) return e;
 	const uint32_t grid = static_cast<uint32_t>(4 * devinfo().cus);
