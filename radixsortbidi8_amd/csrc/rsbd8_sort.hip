@@ -1100,20 +1100,38 @@ __global__ __launch_bounds__(THREADS) void k_local_lsd(const Control *ctrl, KeyX
 __global__ __launch_bounds__(256) void k_fill8(const uint32_t *__restrict__ bucket, KeyXform xf,
 	uint8_t *__restrict__ out, uint64_t n)
 {
-	__shared__ uint32_t s_b[kRadix];
+	__shared__ uint32_t s_b[kRadix + 1];
 	__shared__ uint8_t s_inv[kRadix];
 	s_b[threadIdx.x] = bucket[threadIdx.x];
+	if (threadIdx.x == 0) s_b[kRadix] = 0xFFFFFFFFu;  // past every position (n < 2^32)
 	// the bijective 8-bit transforms are invertible: bucket f(x) holds only the pattern x
 	s_inv[sort_key<uint8_t>(static_cast<uint8_t>(threadIdx.x), xf) & 0xFFu] = static_cast<uint8_t>(threadIdx.x);
 	__syncthreads();
+	// sixteen consecutive positions per thread, written with one 16-byte store
+	const bool vec = (reinterpret_cast<uintptr_t>(out) & 15u) == 0;
+	const uint64_t groups = (n + 15) / 16;
 	const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-	for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-		// last bucket whose start is <= i
+	for (uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < groups; g += stride) {
+		const uint64_t i0 = g * 16;
+		// last bucket whose start is <= i0, then onwards past the buckets that end at or before i
 		uint32_t lo = 0;
 #pragma unroll
 		for (uint32_t step = 128; step; step >>= 1)
-			if (s_b[lo + step] <= i) lo += step;
-		out[i] = s_inv[lo];
+			if (s_b[lo + step] <= i0) lo += step;
+		uint8_t v[16];
+#pragma unroll
+		for (int j = 0; j < 16; ++j) {
+			while (lo < kRadix - 1 && s_b[lo + 1] <= i0 + j) ++lo;
+			v[j] = s_inv[lo];
+		}
+		if (vec && i0 + 16 <= n) {
+			uint4 u;
+			__builtin_memcpy(&u, v, 16);
+			*reinterpret_cast<uint4 *>(out + i0) = u;
+		} else {
+			for (int j = 0; j < 16; ++j)
+				if (i0 + j < n) out[i0 + j] = v[j];
+		}
 	}
 }
 
