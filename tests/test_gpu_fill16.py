@@ -65,3 +65,21 @@ def test_fill16_in_place_and_workspace_reuse():
         assert np.array_equal(out.cpu().numpy().view(np.uint16), exp), mode
         # the fill runs no scatter pass; the absolute mode takes the LSD passes
         assert (prof["scatter"][1] == 0) == (mode == 64), (mode, prof)
+
+
+def test_fill16_noalloc_into_buffer_or_in_place():
+    """radixsortnoalloc: with movetobuffer the fill writes the buffer and leaves the input alone;
+    without it, the input itself."""
+    n = (1 << 22) + 31
+    rng = np.random.default_rng(17)
+    bits = rng.integers(0, 1 << 16, n, dtype=np.uint16)
+    ws = r.Workspace(r.U16, n)
+    for d in (0, 3):
+        exp, _ = oracle.radixsort(bits, r.decode(r.U16, 64, d))
+        src = torch.from_numpy(bits.view(np.int16)).cuda().view(torch.uint16)
+        buf = torch.empty_like(src)
+        r.radixsortnoalloc(src, buf, ws, True, d)
+        assert np.array_equal(buf.view(torch.int16).cpu().numpy().view(np.uint16), exp), d
+        assert np.array_equal(src.view(torch.int16).cpu().numpy().view(np.uint16), bits), d
+        r.radixsortnoalloc(src, buf, ws, False, d)
+        assert np.array_equal(src.view(torch.int16).cpu().numpy().view(np.uint16), exp), d
