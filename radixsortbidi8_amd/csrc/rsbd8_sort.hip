@@ -1520,9 +1520,11 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	const bool bijective = (job.flags & 7u) != (kAbs | kSign) && (job.flags & 7u) != (kSign | kFltp) &&
 		(job.flags & 7u) != (kAbs | kSign | kFltp);
 	if constexpr (P == 2) {
-		if (!job.payload && job.fill8 && bijective && n >= kFill16Min && !job.algo) {
-			// 16-bit count fill. Its tables borrow the first tile records (look-back rows included),
-			// which are zeroed again afterwards, as a fresh workspace has them.
+		if (!job.payload && job.fill8 && bijective && n >= kFill16Min && (!job.algo || job.algo == kAlgoHybrid)) {
+			// 16-bit count fill: the automatic plan, or algorithm 4 (the plan that is not a chain of
+			// LSD passes; forcing onesweep or count + scan keeps those passes). Its tables borrow
+			// the first tile records (look-back rows included), which are zeroed again afterwards,
+			// as a fresh workspace has them.
 			// tables plus the chunk map fit the records: 640 KB + 4 B per 2048 keys <= 1.5 B per key
 			static_assert(kFill16Words * 4 + (kFill16Min / kFill16Chunk + 2) * 4 <= (kFill16Min / kMinTile) * kRecordBytes,
 				"fill tables exceed the records");
