@@ -316,16 +316,19 @@ __device__ __forceinline__ uint32_t lookback_window(const uint64_t *words, int64
 		uint64_t v[W];
 #pragma unroll
 		for (int i = 0; i < W; ++i) v[i] = (prev - i >= first) ? atomic_load_agent(words + (prev - i) * STRIDE) : 0ull;
+		// consume the published words nearest first, up to the first inclusive one; stop at the
+		// first word not published yet (read again from there). Predicated, so the window stays in
+		// registers.
 		int used = 0;
-		bool done = false;
+		bool done = false, stop = false;
 #pragma unroll
 		for (int i = 0; i < W; ++i) {
-			if (done || used != i) break;
 			const uint64_t x = v[i];
-			if ((x >> 34) != tag || !(x & (3ull << 32))) break;  // not published yet: read again from here
-			excl += static_cast<uint32_t>(x);
-			++used;
-			if (x & kLbInclusive) done = true;
+			const bool ok = !stop && (x >> 34) == tag && (x & (3ull << 32)) != 0;
+			excl += ok ? static_cast<uint32_t>(x) : 0u;
+			used += ok ? 1 : 0;
+			done = done || (ok && (x & kLbInclusive) != 0);
+			stop = stop || !ok || done;
 		}
 		if (done) return excl;
 		prev -= used;
