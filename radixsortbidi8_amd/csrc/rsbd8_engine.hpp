@@ -104,8 +104,8 @@ constexpr uint32_t kLdsMax = 512 * 16;
 size_t sort_workspace_bytes(uint64_t n);
 hipError_t workspace_init(void *ws, size_t bytes, hipStream_t stream);
 hipError_t sort_launch(const SortJob &job, void *ws, size_t ws_bytes, hipStream_t stream);
-int default_cfg(int key_bytes, uint64_t n, bool payload);
-int default_algo(int key_bytes, uint64_t n, bool keys_only);
+int default_cfg(int key_bytes, uint64_t n, bool payload, bool float_keys);
+int default_algo(int key_bytes, uint64_t n, bool keys_only, bool float_keys);
 
 // Gather keys through records or pointers into a dense key array (indirection layouts).
 hipError_t gather_by_index(int key_bytes, uint64_t n, const void *base, uint64_t stride, uint64_t offset,

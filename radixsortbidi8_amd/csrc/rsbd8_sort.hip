@@ -1554,9 +1554,10 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 			return hipGetLastError();
 		}
 	}
-	const int cfg = job.cfg >= 0 ? job.cfg : default_cfg(P, n, job.payload);
+	const bool float_keys = (job.flags & kFltp) != 0;
+	const int cfg = job.cfg >= 0 ? job.cfg : default_cfg(P, n, job.payload, float_keys);
 	const bool keys_only = !job.payload && !job.keys_final_any;
-	int algo = job.algo ? job.algo : default_algo(P, n, keys_only);
+	int algo = job.algo ? job.algo : default_algo(P, n, keys_only, float_keys);
 	if (algo == kAlgoLds) {
 		if (!job.payload && n <= kLdsMax && !job.keys_final_any) {
 			// one block holds the whole array: every load happens before any store, so in place is fine
@@ -1571,7 +1572,7 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 				hipLaunchKernelGGL((k_lds_sort<K, 512, 16>), dim3(1), dim3(512), 0, stream, static_cast<const K *>(job.in), static_cast<K *>(job.out), nn, xf, rev);
 			return hipGetLastError();
 		}
-		algo = default_algo(P, n, false);
+		algo = default_algo(P, n, false, float_keys);
 	}
 	int lcfg = -1;
 	if (algo == kAlgoHybrid) {
@@ -1580,7 +1581,6 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 		if (lcfg < 0) algo = kAlgoCountScan;
 	}
 	const bool hybrid = lcfg >= 0;
-	const bool float_keys = (job.flags & kFltp) != 0;
 	const uint32_t tile_keys = static_cast<uint32_t>(kTileCfgs[cfg].threads * kTileCfgs[cfg].kpt);
 	const uint32_t tiles = static_cast<uint32_t>((n + tile_keys - 1) / tile_keys);
 	// the MSD plan's partitions take 1024x16 tiles where the table picks 1024x12 (longer digit runs;
@@ -1760,26 +1760,34 @@ hipError_t profile_end(double *ms, unsigned *launches, int nkinds)
 	return err;
 }
 
-// measured on MI355X (2^20..2^28 uniform keys): the count + scan path wins from about 2^22 keys,
-// the onesweep path below that (fewer launches); keys-only sorts of 4- and 8-byte keys from 2^26
-// keys use the two-level MSD plan
-int default_algo(int key_bytes, uint64_t n, bool keys_only)
+// measured on MI355X (uniform keys):
+// - keys-only sorts take the two-level MSD plan from 2^18 keys for 8-byte integer keys (2^20 u64:
+//   223 us against 296 for onesweep; 2^24: 427 against 1065 for count + scan) and from 2^22 for
+//   4-byte integer keys (2^24 u32: 301 us against 411); float keys, whose exponents cluster so that
+//   the plan's bucket estimate usually sends them to the LSD fallback, from 2^26 only;
+// - otherwise the count + scan path from 2^22 keys, the onesweep path below (fewer launches).
+int default_algo(int key_bytes, uint64_t n, bool keys_only, bool float_keys)
 {
-	if (keys_only && key_bytes >= 4 && n >= (1u << 26) && local_cfg_for(n) >= 0) return kAlgoHybrid;
+	if (keys_only && key_bytes >= 4 && local_cfg_for(n) >= 0) {
+		const uint64_t from = float_keys ? (1ull << 26) : (key_bytes >= 8 ? (1ull << 18) : (1ull << 22));
+		if (n >= from) return kAlgoHybrid;
+	}
 	return n >= (1u << 22) ? kAlgoCountScan : kAlgoOnesweep;
 }
 
 // measured on MI355X: 1024x12 tiles (long digit runs per tile) for 8-byte keys from 2^22 keys, and
 // for every sort with a payload from 2^26 keys
-int default_cfg(int key_bytes, uint64_t n, bool payload)
+int default_cfg(int key_bytes, uint64_t n, bool payload, bool float_keys)
 {
 	// the autotuned table's choices at these sizes (uniform keys)
 	if (n <= (1u << 16)) return 0;
-	if (!payload && default_algo(key_bytes, n, true) == kAlgoHybrid) return 3;  // MSD partitions: 1024x12
+	// MSD partitions: 512x8 tiles below 2^25 keys (2^24 u64: 427 us against 438 with 1024x12),
+	// 1024x12 from it (2^25 u64: 661 against 695)
+	if (!payload && default_algo(key_bytes, n, true, float_keys) == kAlgoHybrid) return n >= (1u << 25) ? 3 : 2;
 	// measured with a u32 payload: at 2^28 keys of 1, 2 or 4 bytes, 1024x12 tiles (longer digit
 	// runs per tile) take 0.74-0.92x the time of 256x16 tiles; at 2^24 every shape is within 6%
 	if (payload && n >= (1u << 26)) return 3;
-	if (default_algo(key_bytes, n, false) == kAlgoCountScan) return key_bytes >= 8 ? 3 : (key_bytes == 2 ? 2 : 1);
+	if (default_algo(key_bytes, n, false, float_keys) == kAlgoCountScan) return key_bytes >= 8 ? 3 : (key_bytes == 2 ? 2 : 1);
 	return key_bytes >= 8 ? 4 : 2;
 }
 
