@@ -1775,7 +1775,7 @@ int default_algo(int key_bytes, uint64_t n, bool keys_only, bool float_keys)
 	return n >= (1u << 22) ? kAlgoCountScan : kAlgoOnesweep;
 }
 
-// measured on MI355X: 1024x12 tiles (long digit runs per tile) for 8-byte keys from 2^22 keys, and
+// measured on MI355X: 1024x12 tiles (long digit runs per tile) for 8-byte keys from 2^25 keys, and
 // for every sort with a payload from 2^26 keys
 int default_cfg(int key_bytes, uint64_t n, bool payload, bool float_keys)
 {
@@ -1787,7 +1787,10 @@ int default_cfg(int key_bytes, uint64_t n, bool payload, bool float_keys)
 	// measured with a u32 payload: at 2^28 keys of 1, 2 or 4 bytes, 1024x12 tiles (longer digit
 	// runs per tile) take 0.74-0.92x the time of 256x16 tiles; at 2^24 every shape is within 6%
 	if (payload && n >= (1u << 26)) return 3;
-	if (default_algo(key_bytes, n, false, float_keys) == kAlgoCountScan) return key_bytes >= 8 ? 3 : (key_bytes == 2 ? 2 : 1);
+	// count + scan passes of 8-byte keys: 512x8 tiles below 2^25 keys (u64 keys only, 2^22 / 2^24:
+	// 431 / 953 us against 504 / 1065 with 1024x12; with a u32 payload at 2^24: 1.30 ms against 1.47)
+	if (default_algo(key_bytes, n, false, float_keys) == kAlgoCountScan)
+		return key_bytes >= 8 ? (n >= (1u << 25) ? 3 : 2) : (key_bytes == 2 ? 2 : 1);
 	return key_bytes >= 8 ? 4 : 2;
 }
 
