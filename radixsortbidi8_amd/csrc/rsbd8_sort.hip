@@ -1788,9 +1788,14 @@ int default_cfg(int key_bytes, uint64_t n, bool payload, bool float_keys)
 	// runs per tile) take 0.74-0.92x the time of 256x16 tiles; at 2^24 every shape is within 6%
 	if (payload && n >= (1u << 26)) return 3;
 	// count + scan passes of 8-byte keys: 512x8 tiles below 2^25 keys (u64 keys only, 2^22 / 2^24:
-	// 431 / 953 us against 504 / 1065 with 1024x12; with a u32 payload at 2^24: 1.30 ms against 1.47)
-	if (default_algo(key_bytes, n, false, float_keys) == kAlgoCountScan)
-		return key_bytes >= 8 ? (n >= (1u << 25) ? 3 : 2) : (key_bytes == 2 ? 2 : 1);
+	// 431 / 953 us against 504 / 1065 with 1024x12; with a u32 payload at 2^24: 1.30 ms against 1.47).
+	// 4-byte keys only (float keys below 2^26, say): 256x8 (f32 2^24: 422 us against 437 with
+	// 256x16); with a payload 256x16 (u32 2^24: 0.51 ms against 0.55 with 256x8).
+	if (default_algo(key_bytes, n, false, float_keys) == kAlgoCountScan) {
+		if (key_bytes >= 8) return n >= (1u << 25) ? 3 : 2;
+		if (key_bytes == 4) return payload ? 1 : 0;
+		return key_bytes == 2 ? 2 : 1;
+	}
 	return key_bytes >= 8 ? 4 : 2;
 }
 
