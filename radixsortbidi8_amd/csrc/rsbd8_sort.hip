@@ -1796,9 +1796,11 @@ int default_cfg(int key_bytes, uint64_t n, bool payload, bool float_keys)
 		if (key_bytes == 4) return payload ? 1 : 0;
 		return key_bytes == 2 ? 2 : 1;
 	}
-	// onesweep: 1024x12 tiles from 2^21 keys of up to 4 bytes (u32 / f32 / u16 at 2^21: 149 / 153 /
-	// 78 us against 164 / 166 / 87 with 512x8; at 2^19 512x8 is ahead)
-	if (key_bytes >= 8) return 4;
+	// onesweep: 8-byte keys 512x8 (f64 at 2^17 / 2^21: 147 / 327 us against 230 / 377 with 256x24;
+	// u64 with a u32 payload: 155 / 386 against 245 / 516); smaller keys 512x8, 1024x12 from 2^21
+	// (u32 / f32 / u16 at 2^21: 149 / 153 / 78 us against 164 / 166 / 87 with 512x8; at 2^19 512x8
+	// is ahead)
+	if (key_bytes >= 8) return 2;
 	return n >= (1u << 21) ? 3 : 2;
 }
 
