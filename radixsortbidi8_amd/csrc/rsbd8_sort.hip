@@ -1775,12 +1775,10 @@ int default_algo(int key_bytes, uint64_t n, bool keys_only, bool float_keys)
 	return n >= (1u << 22) ? kAlgoCountScan : kAlgoOnesweep;
 }
 
-// measured on MI355X: 1024x12 tiles (long digit runs per tile) for 8-byte keys from 2^25 keys, and
-// for every sort with a payload from 2^26 keys
+// Tile configuration of the automatic plan, measured on MI355X with uniform keys (times per sort).
 int default_cfg(int key_bytes, uint64_t n, bool payload, bool float_keys)
 {
-	// the autotuned table's choices at these sizes (uniform keys)
-	if (n <= (1u << 16)) return 0;
+	if (n <= (1u << 16)) return 0;  // 256x8: the autotuned table's choice up to 2^16 keys
 	// MSD partitions: 512x8 tiles below 2^25 keys (2^24 u64: 427 us against 438 with 1024x12),
 	// 1024x12 from it (2^25 u64: 661 against 695)
 	if (!payload && default_algo(key_bytes, n, true, float_keys) == kAlgoHybrid) return n >= (1u << 25) ? 3 : 2;
