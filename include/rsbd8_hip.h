@@ -162,7 +162,7 @@ int rsbd8_hip_set_algorithm(int algo);
 
 /* A sort plan chosen per call (no reference counterpart; the reference picks its plan from
  * compile-time thresholds, H:12948-13152): algo 0 automatic, 1 onesweep, 2 count + scan + scatter,
- * 3 a single block sorting everything in LDS (keys only, count <= 8192; otherwise automatic),
+ * 3 a single block sorting everything in LDS (keys only, count <= 16384; otherwise automatic),
  * 4 the two-level MSD plan (as for rsbd8_hip_set_algorithm); tile_cfg -1 automatic or 0..4 as for rsbd8_hip_set_tile_config; presort_check first tests
  * whether the input is already in order or strictly reversed (then the sort is one copy). */
 typedef struct rsbd8_hip_plan {
@@ -179,7 +179,7 @@ int rsbd8_hip_radixsortcopynoalloc_plan(int keytype, unsigned flags, size_t coun
 /* The candidate plans, indexed in the order of the selector network's outputs (0 <= index <
  * rsbd8_hip_plan_count()): onesweep, count + scan, then the two-level MSD plan, each over the tile
  * configurations, each without and with the presort check; the last is the single-block LDS sort
- * (count <= 8192). */
+ * (count <= 16384). */
 int rsbd8_hip_plan_count(void);
 int rsbd8_hip_plan_at(int index, rsbd8_hip_plan *plan);
 

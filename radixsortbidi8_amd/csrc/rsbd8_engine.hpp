@@ -99,7 +99,7 @@ struct SortJob {
 };
 
 constexpr int kAlgoAuto = 0, kAlgoOnesweep = 1, kAlgoCountScan = 2, kAlgoLds = 3, kAlgoHybrid = 4;
-constexpr uint32_t kLdsMax = 512 * 16;
+constexpr uint32_t kLdsMax = 1024 * 16;
 
 size_t sort_workspace_bytes(uint64_t n);
 hipError_t workspace_init(void *ws, size_t bytes, hipStream_t stream);

@@ -1568,8 +1568,10 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 				hipLaunchKernelGGL((k_lds_sort<K, 256, 4>), dim3(1), dim3(256), 0, stream, static_cast<const K *>(job.in), static_cast<K *>(job.out), nn, xf, rev);
 			else if (n <= 4096)
 				hipLaunchKernelGGL((k_lds_sort<K, 256, 16>), dim3(1), dim3(256), 0, stream, static_cast<const K *>(job.in), static_cast<K *>(job.out), nn, xf, rev);
-			else
+			else if (n <= 8192)
 				hipLaunchKernelGGL((k_lds_sort<K, 512, 16>), dim3(1), dim3(512), 0, stream, static_cast<const K *>(job.in), static_cast<K *>(job.out), nn, xf, rev);
+			else
+				hipLaunchKernelGGL((k_lds_sort<K, 1024, 16>), dim3(1), dim3(1024), 0, stream, static_cast<const K *>(job.in), static_cast<K *>(job.out), nn, xf, rev);
 			return hipGetLastError();
 		}
 		algo = default_algo(P, n, false, float_keys);
@@ -1765,9 +1767,13 @@ hipError_t profile_end(double *ms, unsigned *launches, int nkinds)
 //   223 us against 296 for onesweep; 2^24: 427 against 1065 for count + scan) and from 2^22 for
 //   4-byte integer keys (2^24 u32: 301 us against 411); float keys, whose exponents cluster so that
 //   the plan's bucket estimate usually sends them to the LSD fallback, from 2^26 only;
-// - otherwise the count + scan path from 2^22 keys, the onesweep path below (fewer launches).
+// - otherwise the count + scan path from 2^22 keys, the onesweep path below (fewer launches);
+// - keys-only sorts of up to kLdsMax keys: one block sorts them in LDS, one launch instead of a dozen
+//   (2^10 / 2^14 u32: 9 / 39 us against 49 / 66 for onesweep; u64: 16 / 85 against 87 / 124; u8 2^14:
+//   12 against 19 for the count fill).
 int default_algo(int key_bytes, uint64_t n, bool keys_only, bool float_keys)
 {
+	if (keys_only && n <= kLdsMax) return kAlgoLds;
 	if (keys_only && key_bytes >= 4 && local_cfg_for(n) >= 0) {
 		const uint64_t from = float_keys ? (1ull << 26) : (key_bytes >= 8 ? (1ull << 18) : (1ull << 22));
 		if (n >= from) return kAlgoHybrid;

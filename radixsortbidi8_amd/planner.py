@@ -3,7 +3,7 @@
 A *plan* is (algorithm, tile configuration, presort check):
   algorithm 1  onesweep: each scatter pass resolves its tile offsets by decoupled look-back
   algorithm 2  count + chained scan + scatter per pass
-  algorithm 3  one block sorts everything in LDS (keys only, n <= 8192)
+  algorithm 3  one block sorts everything in LDS (keys only, n <= 16384)
   algorithm 4  two MSD partitions by the top two digits, then one LDS sort per bucket (keys only,
                4/8-byte keys; narrower keys and skewed inputs end in algorithm 2)
   tile config  index into TILE_CONFIGS (threads x keys per thread of the scatter kernels)
@@ -35,7 +35,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TUNING_DIR = os.path.join(HERE, "tuning")
 NUM_FEATURES = 24
 HIDDEN = 32
-LDS_MAX = 8192
+LDS_MAX = 16384
 
 PLANS = [(a, c, chk) for a in (1, 2, 4) for c in range(len(TILE_CONFIGS)) for chk in (0, 1)] + [(3, -1, 0)]
 NUM_PLANS = len(PLANS)
