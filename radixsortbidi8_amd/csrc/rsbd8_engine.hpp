@@ -99,7 +99,8 @@ struct SortJob {
 };
 
 constexpr int kAlgoAuto = 0, kAlgoOnesweep = 1, kAlgoCountScan = 2, kAlgoLds = 3, kAlgoHybrid = 4;
-constexpr uint32_t kLdsMax = 1024 * 16;
+constexpr uint32_t kLdsMax = 1024 * 16;    // keys-only sorts one LDS block takes
+constexpr uint32_t kLdsPayMax = 512 * 16;  // the same with a u32 payload
 
 size_t sort_workspace_bytes(uint64_t n);
 hipError_t workspace_init(void *ws, size_t bytes, hipStream_t stream);

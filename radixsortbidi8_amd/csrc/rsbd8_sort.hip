@@ -136,10 +136,12 @@ __global__ __launch_bounds__(THREADS) void k_check(const K *__restrict__ in, uin
 	if (mine) atomicOr(&ctrl->check, mine);
 }
 
-// One block sorts the whole array in LDS (8-bit LSD passes with the block ranking), keys only.
-template <typename K, int THREADS, int KPT>
-__global__ __launch_bounds__(THREADS) void k_lds_sort(const K *__restrict__ in, K *__restrict__ out, uint32_t n,
-	KeyXform xf, int reverse)
+// One block sorts the whole array in LDS (8-bit LSD passes with the block ranking). With PAY a u32
+// payload (pin, or the source index when pin is null) moves with the keys. Every load happens
+// before any store, so in place is fine for keys and payload alike.
+template <typename K, int THREADS, int KPT, bool PAY>
+__global__ __launch_bounds__(THREADS) void k_lds_sort(const K *in, K *out, uint32_t n, KeyXform xf, int reverse,
+	const uint32_t *pin, uint32_t *pout)
 {
 	constexpr int CAP = THREADS * KPT;
 	constexpr int NW = THREADS / 64;
@@ -147,16 +149,22 @@ __global__ __launch_bounds__(THREADS) void k_lds_sort(const K *__restrict__ in, 
 	__shared__ uint32_t s_whist[NW * 256];
 	__shared__ uint32_t s_digoff[256];
 	__shared__ uint32_t s_wsum[4];
-	__shared__ __align__(16) unsigned char s_raw[max_bytes(CAP * sizeof(K), rank_peer_bytes(THREADS))];
+	// the keys' exchange, the payload's, and during the ranking the peer words
+	__shared__ __align__(16) unsigned char s_raw[max_bytes(max_bytes(CAP * sizeof(K), PAY ? CAP * 4 : 0), rank_peer_bytes(THREADS))];
 	K *s_keys = reinterpret_cast<K *>(s_raw);
+	uint32_t *s_pay = reinterpret_cast<uint32_t *>(s_raw);
 	uint64_t *s_peer = reinterpret_cast<uint64_t *>(s_raw);
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	K keys[KPT];
+	uint32_t pay[PAY ? KPT : 1];
 #pragma unroll
 	for (int j = 0; j < KPT; ++j) {
 		const uint32_t l = w * 64u * KPT + j * 64u + lane;
-		keys[j] = l < n ? in[reverse ? n - 1 - l : l] : K(0);
+		const uint32_t sidx = reverse ? n - 1 - l : l;
+		keys[j] = l < n ? in[sidx] : K(0);
+		if constexpr (PAY) pay[j] = l < n ? (pin ? pin[sidx] : sidx) : 0u;
 	}
+	__syncthreads();  // every load is done before any store (in place)
 	for (int p = 0; p < P; ++p) {
 		uint32_t dig[KPT], pos[KPT], total;
 #pragma unroll
@@ -171,11 +179,22 @@ __global__ __launch_bounds__(THREADS) void k_lds_sort(const K *__restrict__ in, 
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) keys[j] = s_keys[w * 64u * KPT + j * 64u + lane];
 		__syncthreads();
+		if constexpr (PAY) {
+#pragma unroll
+			for (int j = 0; j < KPT; ++j) s_pay[pos[j]] = pay[j];
+			__syncthreads();
+#pragma unroll
+			for (int j = 0; j < KPT; ++j) pay[j] = s_pay[w * 64u * KPT + j * 64u + lane];
+			__syncthreads();
+		}
 	}
 #pragma unroll
 	for (int j = 0; j < KPT; ++j) {
 		const uint32_t l = w * 64u * KPT + j * 64u + lane;
-		if (l < n) out[l] = keys[j];
+		if (l < n) {
+			out[l] = keys[j];
+			if constexpr (PAY) pout[l] = pay[j];
+		}
 	}
 }
 
@@ -1558,20 +1577,36 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	const int cfg = job.cfg >= 0 ? job.cfg : default_cfg(P, n, job.payload, float_keys);
 	const bool keys_only = !job.payload && !job.keys_final_any;
 	int algo = job.algo ? job.algo : default_algo(P, n, keys_only, float_keys);
+	// small sorts with a payload: one LDS block as well (up to kLdsPayMax keys)
+	if (!job.algo && job.payload && n <= kLdsPayMax) algo = kAlgoLds;
 	if (algo == kAlgoLds) {
+		// one block holds the whole array: every load happens before any store, so in place is fine
+		const int rev = (job.flags & kRev) ? 1 : 0;
+		const uint32_t nn = static_cast<uint32_t>(n);
+		const K *kin = static_cast<const K *>(job.in);
+		K *kout = static_cast<K *>(job.out);
 		if (!job.payload && n <= kLdsMax && !job.keys_final_any) {
-			// one block holds the whole array: every load happens before any store, so in place is fine
-			const int rev = (job.flags & kRev) ? 1 : 0;
-			const uint32_t nn = static_cast<uint32_t>(n);
 			ProfScope ps(1, 0, ws, stream);
 			if (n <= 1024)
-				hipLaunchKernelGGL((k_lds_sort<K, 256, 4>), dim3(1), dim3(256), 0, stream, static_cast<const K *>(job.in), static_cast<K *>(job.out), nn, xf, rev);
+				hipLaunchKernelGGL((k_lds_sort<K, 256, 4, false>), dim3(1), dim3(256), 0, stream, kin, kout, nn, xf, rev, nullptr, nullptr);
 			else if (n <= 4096)
-				hipLaunchKernelGGL((k_lds_sort<K, 256, 16>), dim3(1), dim3(256), 0, stream, static_cast<const K *>(job.in), static_cast<K *>(job.out), nn, xf, rev);
+				hipLaunchKernelGGL((k_lds_sort<K, 256, 16, false>), dim3(1), dim3(256), 0, stream, kin, kout, nn, xf, rev, nullptr, nullptr);
 			else if (n <= 8192)
-				hipLaunchKernelGGL((k_lds_sort<K, 512, 16>), dim3(1), dim3(512), 0, stream, static_cast<const K *>(job.in), static_cast<K *>(job.out), nn, xf, rev);
+				hipLaunchKernelGGL((k_lds_sort<K, 512, 16, false>), dim3(1), dim3(512), 0, stream, kin, kout, nn, xf, rev, nullptr, nullptr);
 			else
-				hipLaunchKernelGGL((k_lds_sort<K, 1024, 16>), dim3(1), dim3(1024), 0, stream, static_cast<const K *>(job.in), static_cast<K *>(job.out), nn, xf, rev);
+				hipLaunchKernelGGL((k_lds_sort<K, 1024, 16, false>), dim3(1), dim3(1024), 0, stream, kin, kout, nn, xf, rev, nullptr, nullptr);
+			return hipGetLastError();
+		}
+		if (job.payload && n <= kLdsPayMax) {
+			ProfScope ps(1, 0, ws, stream);
+			const uint32_t *pin = job.pin;
+			uint32_t *pout = job.pout;
+			if (n <= 1024)
+				hipLaunchKernelGGL((k_lds_sort<K, 256, 4, true>), dim3(1), dim3(256), 0, stream, kin, kout, nn, xf, rev, pin, pout);
+			else if (n <= 4096)
+				hipLaunchKernelGGL((k_lds_sort<K, 256, 16, true>), dim3(1), dim3(256), 0, stream, kin, kout, nn, xf, rev, pin, pout);
+			else
+				hipLaunchKernelGGL((k_lds_sort<K, 512, 16, true>), dim3(1), dim3(512), 0, stream, kin, kout, nn, xf, rev, pin, pout);
 			return hipGetLastError();
 		}
 		algo = default_algo(P, n, false, float_keys);

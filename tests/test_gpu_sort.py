@@ -329,3 +329,29 @@ def test_workspace_reuse_many_calls(algo):
         o, b = torch.empty_like(a), torch.empty_like(a)
         r.radixsortcopynoalloc(a, o, b, ws)
         assert np.array_equal(host(o), np.sort(bits))
+
+
+@pytest.mark.parametrize("nbytes", [1, 2, 4, 8])
+def test_small_pairs_one_lds_block(nbytes):
+    """Sorts with a payload of up to 8192 keys run as one LDS block in the automatic plan: every
+    shape boundary, every mode and direction, a given payload and the source index."""
+    rng = np.random.default_rng(300 + nbytes)
+    kt = KEYTYPES[nbytes][-1]
+    for n in (2, 3, 100, 1023, 1024, 1025, 4096, 4097, 8191, 8192):
+        bits = gen(nbytes, n, "dups", rng)
+        payload = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+        ws = r.Workspace(kt, n, r.LAYOUT_PAIRS)
+        k = dev(typed(bits, kt))
+        ko, kb = torch.empty_like(k), torch.empty_like(k)
+        p = dev(payload)
+        po, pb = torch.empty_like(p), torch.empty_like(p)
+        for mode in (64, 65, 66):
+            for d in range(4):
+                flags = r.decode(kt, mode, d, True)
+                exp_k, exp_p = oracle.radixsort(bits, flags, payload=payload)
+                r.sort_pairs(k, ko, kb, p, po, pb, ws, d, mode)
+                assert np.array_equal(host(ko).view(bits.dtype), exp_k), (n, mode, d)
+                assert np.array_equal(host(po), exp_p), (n, mode, d)
+                r.sort_pairs(k, ko, kb, None, po, pb, ws, d, mode)
+                _, perm = oracle.radixsort(bits, flags, want_payload=True)
+                assert np.array_equal(host(po), perm), (n, mode, d)
