@@ -164,8 +164,32 @@ __global__ __launch_bounds__(THREADS) void k_lds_sort(const K *in, K *out, uint3
 		keys[j] = l < n ? in[sidx] : K(0);
 		if constexpr (PAY) pay[j] = l < n ? (pin ? pin[sidx] : sidx) : 0u;
 	}
-	__syncthreads();  // every load is done before any store (in place)
+	// digits on which every sort key agrees need no pass (the pass skip of H:11652): the AND and
+	// the OR of the block's sort keys agree on them
+	using W = typename KeyTraits<K>::W;
+	__shared__ W s_and[NW], s_or[NW];
+	W k_and = ~W(0), k_or = 0;
+#pragma unroll
+	for (int j = 0; j < KPT; ++j) {
+		const uint32_t l = w * 64u * KPT + j * 64u + lane;
+		if (l < n) {
+			const W k = sort_key<K>(keys[j], xf);
+			k_and &= k;
+			k_or |= k;
+		}
+	}
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) {
+		k_and &= static_cast<W>(__shfl_xor(k_and, o, 64));
+		k_or |= static_cast<W>(__shfl_xor(k_or, o, 64));
+	}
+	if (lane == 0) { s_and[w] = k_and; s_or[w] = k_or; }
+	__syncthreads();  // also: every load is done before any store (in place)
+	W differ = 0;
+#pragma unroll
+	for (int v = 0; v < NW; ++v) differ |= s_and[v] ^ s_or[v];
 	for (int p = 0; p < P; ++p) {
+		if (((differ >> (8 * p)) & 0xFFu) == 0) continue;  // block-uniform
 		uint32_t dig[KPT], pos[KPT], total;
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) {
