@@ -11,7 +11,10 @@
  *     threads), and pzeroedindices becomes a device workspace of rsbd8_hip_workspace_size bytes
  *     that must be zero before its first use (rsbd8_hip_workspace_init) and is left ready for
  *     reuse by every sort.
- * Array arguments are device pointers (hipMalloc'd or otherwise GPU-accessible) unless stated.
+ * Array arguments are device pointers (hipMalloc'd, or managed memory) unless stated: every entry
+ * point that launches kernels on the caller's arrays checks them on the host first and returns
+ * RSBD8_HIP_EINVAL, before any launch, for an array that is host memory (pageable or pinned). Only
+ * rsbd8_hip_radixsort and rsbd8_hip_radixsortcopy take host arrays (they stage them).
  * All sorts are stream-ordered and asynchronous unless stated; calls on one workspace must not
  * overlap. Return value: 0 on success, a negative RSBD8_HIP_E* code otherwise
  * (rsbd8_hip_last_error gives the message).
@@ -38,7 +41,7 @@ enum {
 	RSBD8_HIP_ABS = 1, RSBD8_HIP_SIGN = 2, RSBD8_HIP_FLTP = 4, RSBD8_HIP_DESC = 8, RSBD8_HIP_REV = 16
 };
 
-/* raw enum values of rsbd8::sortingmode (H:118-156) and rsbd8::sortingdirection (H:158-171) */
+/* raw enum values of rsbd8::sortingmode (H:212-250) and rsbd8::sortingdirection (H:252-265) */
 enum {
 	RSBD8_HIP_MODE_NATIVE = 64, RSBD8_HIP_MODE_NATIVEABS = 65, RSBD8_HIP_MODE_NATIVETIEREDABS = 66,
 	RSBD8_HIP_MODE_FORCEUNSIGNED = 0, RSBD8_HIP_MODE_FORCESIGNED = 2, RSBD8_HIP_MODE_FORCEABSSIGNED = 3,

@@ -2,7 +2,7 @@
 //
 // Same namespace, template names, template parameters and argument lists as the reference
 // header's public surface (Radixsortbidi8.hpp = H):
-//   rsbd8::sortingmode / rsbd8::sortingdirection                         H:118-171
+//   rsbd8::sortingmode / rsbd8::sortingdirection                         H:212-265
 //   rsbd8::radixsort, radixsortcopy (allocating, return false on failure)  H:60397-60588
 //   rsbd8::radixsortnoalloc, radixsortcopynoalloc (caller-provided memory) H:60121-60395
 //   indirection: T** (H:60590-60981), member object / getter function
@@ -284,7 +284,12 @@ bool sortpointers(std::size_t count, V *const *input, V **output, Extract &&extr
 	bool secondlevel, std::ptrdiff_t off2, std::size_t index2)
 {
 	if (count < 2u) {
-		if (count == 1u && output != input) *output = *input;
+		if (count == 1u && output != input) {
+			// a device pointer array is never dereferenced on the host
+			if (isdevice(input) || isdevice(output))
+				return hipMemcpy(output, input, sizeof(V *), hipMemcpyDefault) == hipSuccess;
+			*output = *input;
+		}
 		return true;
 	}
 	if (deviceok && isdevice(input)) {

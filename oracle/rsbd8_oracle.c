@@ -9,7 +9,7 @@
  *                          independent Python table, plus the known-answer tests below.
  *   rsbd8o_sortkeys     -- equality with rsbd8o_sortkey element by element.
  *   rsbd8o_sortkey      -- brute force over all 2^8 and 2^16 patterns against the ordering
- *                          documented at H:130-155 and H:258-319 (IEEE order by float value for
+ *                          documented at H:224-248 and H:315-319 (IEEE order by float value for
  *                          forcefloatingp, |x| order for the absolute modes, the listed tiered and
  *                          inside-out sequences) and against numpy's float ordering.
  *   rsbd8o_radixsort    -- multiplatform/main.cpp:10-19, 182-215 known answers; equality with

@@ -13,7 +13,7 @@ are accepted by the allocating forms and staged through device memory).  The sor
 in the HIP library ``lib/librsbd8_hip.so`` (built from ``csrc/`` by ``build.py``); importing this
 package without that library raises.
 
-``sortingmode`` / ``sortingdirection`` carry the reference's enum values (H:118-171) and
+``sortingmode`` / ``sortingdirection`` carry the reference's enum values (H:212-265) and
 ``decode`` reproduces its compile-time mode decoding (H:59778-59904).
 """
 from __future__ import annotations
@@ -29,7 +29,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "librsbd8_hip.so")
 
 
 class sortingmode(enum.IntEnum):
-    """rsbd8::sortingmode (H:118-156)."""
+    """rsbd8::sortingmode (H:212-250)."""
     native = 64
     nativeabs = 65
     nativetieredabs = 66
@@ -46,7 +46,7 @@ class sortingmode(enum.IntEnum):
 
 
 class sortingdirection(enum.IntEnum):
-    """rsbd8::sortingdirection (H:158-171)."""
+    """rsbd8::sortingdirection (H:252-265)."""
     ascfwdorder = 0
     dscrevorder = 3
     dscfwdorder = 1
@@ -168,8 +168,18 @@ def _stream_ptr(stream=None):
     return ctypes.c_void_p(s.cuda_stream)
 
 
-def _ptr(t):
-    return None if t is None else ctypes.c_void_p(t.data_ptr())
+def _ptr(t, device: bool = True):
+    """Address of a tensor's data for the C ABI. The noalloc forms hand it to GPU kernels, so the
+    tensor must live on the GPU (device=True); a non-contiguous view would sort the wrong elements,
+    so it must be contiguous. The allocating forms also take host tensors (device=False)."""
+    if t is None:
+        return None
+    if device and not t.is_cuda:
+        raise Rsbd8Error("array must be a GPU tensor (the noalloc forms launch kernels on it); "
+                         "radixsort/radixsortcopy accept host arrays")
+    if not t.is_contiguous():
+        raise Rsbd8Error("array must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
 
 
 def _is_tensor(x):
@@ -276,8 +286,8 @@ def radixsortcopy(input, output, direction=sortingdirection.ascfwdorder, mode=so
     """rsbd8::radixsortcopy (H:60468-60588): torch GPU tensors or host numpy arrays."""
     if _is_tensor(input):
         kt, f = _flags(input.dtype, mode, direction)
-        return _host_call(_load().rsbd8_hip_radixsortcopy, kt, f, (_ptr(input), _ptr(output)), input.numel(),
-                          _stream_ptr(stream))
+        return _host_call(_load().rsbd8_hip_radixsortcopy, kt, f, (_ptr(input, False), _ptr(output, False)),
+                          input.numel(), _stream_ptr(stream))
     inp = np.ascontiguousarray(input)
     assert output.flags.c_contiguous and output.dtype == inp.dtype and output.size == inp.size
     kt, f = _flags(inp.dtype, mode, direction)
@@ -290,7 +300,8 @@ def radixsort(input, direction=sortingdirection.ascfwdorder, mode=sortingmode.na
     """rsbd8::radixsort (H:60397-60466), in place: torch GPU tensor or host numpy array."""
     if _is_tensor(input):
         kt, f = _flags(input.dtype, mode, direction)
-        return _host_call(_load().rsbd8_hip_radixsort, kt, f, (_ptr(input),), input.numel(), _stream_ptr(stream))
+        return _host_call(_load().rsbd8_hip_radixsort, kt, f, (_ptr(input, False),), input.numel(),
+                          _stream_ptr(stream))
     assert input.flags.c_contiguous
     kt, f = _flags(input.dtype, mode, direction)
     return _host_call(_load().rsbd8_hip_radixsort, kt, f, (input.ctypes.data_as(ctypes.c_void_p),), input.size,

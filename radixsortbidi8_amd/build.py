@@ -1,6 +1,11 @@
-"""Builds the gfx950 HIP library librsbd8_hip.so in-tree with hipcc (no torch involved)."""
+"""Builds the gfx950 HIP library librsbd8_hip.so in-tree with hipcc (no torch involved).
+
+Every source is compiled to its own object in parallel (no device code crosses a file, so no
+relocatable device code is needed), then the objects are linked into one shared library.
+"""
 from __future__ import annotations
 
+import concurrent.futures as cf
 import os
 import subprocess
 import sys
@@ -8,24 +13,38 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
+OBJDIR = os.path.join(LIBDIR, "obj")
 LIB = os.path.join(LIBDIR, "librsbd8_hip.so")
-SOURCES = ["rsbd8_sort.hip", "rsbd8_segmented.hip", "rsbd8_plan.hip", "rsbd8_capi.hip"]
+SOURCES = ["rsbd8_sort.hip", "rsbd8_segmented.hip", "rsbd8_plan.hip", "rsbd8_capi.hip", "rsbd8_wide.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wall",
 ]
 
 
 def sources():
-    return [os.path.join(CSRC, s) for s in SOURCES]
+    return [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+
+
+def _compile(src: str, verbose: bool) -> str:
+    obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
+    cmd = [HIPCC, *FLAGS, "-c", "-o", obj + ".tmp", src]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(obj + ".tmp", obj)
+    return obj
 
 
 def build(verbose: bool = False) -> str:
-    os.makedirs(LIBDIR, exist_ok=True)
+    os.makedirs(OBJDIR, exist_ok=True)
+    srcs = sources()
+    with cf.ThreadPoolExecutor(max_workers=min(len(srcs), 8)) as ex:
+        objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *sources()]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

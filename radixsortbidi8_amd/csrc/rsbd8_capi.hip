@@ -5,6 +5,7 @@
 #include <atomic>
 #include <cstdio>
 #include <cstring>
+#include <initializer_list>
 #include <new>
 #include <string>
 
@@ -189,6 +190,18 @@ bool is_device_memory(const void *p)
 	return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
 }
 
+// Every array a noalloc entry point hands to its kernels must be GPU memory (hipMalloc'd, or
+// managed): a host array would reach the kernels and fault the GPU. Checked on the host before any
+// launch; null pointers are left to the callers' own checks.
+int check_device_arrays(std::initializer_list<const void *> arrays, const char *what)
+{
+	for (const void *p : arrays)
+		if (p && !is_device_memory(p))
+			return fail(RSBD8_HIP_EINVAL, (std::string(what) + ": an array is not device memory (host arrays are accepted only by "
+				"rsbd8_hip_radixsort and rsbd8_hip_radixsortcopy)").c_str());
+	return RSBD8_HIP_OK;
+}
+
 int check_common(int keytype, unsigned flags, size_t count, size_t ws_bytes, int layout)
 {
 	if (!key_bytes_of(keytype)) return fail(RSBD8_HIP_EINVAL, "unknown key type");
@@ -288,6 +301,7 @@ int rsbd8_hip_radixsortcopynoalloc(int keytype, unsigned flags, size_t count, vo
 	if (!input || !output || !workspace) return fail(RSBD8_HIP_EINVAL, "null array");
 	if (input == output || (buffer && (buffer == input || buffer == output))) return fail(RSBD8_HIP_EINVAL, "arrays alias");
 	if (!buffer && needs_buffer(kb, flags) && kb != 1) return fail(RSBD8_HIP_EINVAL, "buffer required");
+	if ((r = check_device_arrays({workspace, input, output, buffer}, "radixsortcopynoalloc"))) return r;
 	SortJob job;
 	job.key_bytes = kb; job.flags = flags; job.n = count;
 	job.in = input; job.out = output; job.buf = buffer;
@@ -307,6 +321,7 @@ int rsbd8_hip_radixsortnoalloc(int keytype, unsigned flags, size_t count, void *
 	if (!input || !workspace) return fail(RSBD8_HIP_EINVAL, "null array");
 	if (buffer == input) return fail(RSBD8_HIP_EINVAL, "arrays alias");
 	if (!buffer && (needs_buffer(kb, flags) || movetobuffer)) return fail(RSBD8_HIP_EINVAL, "buffer required");
+	if ((r = check_device_arrays({workspace, input, buffer}, "radixsortnoalloc"))) return r;
 	SortJob job;
 	job.key_bytes = kb; job.flags = flags; job.n = count;
 	job.in = input;
@@ -330,6 +345,9 @@ int rsbd8_hip_sort_pairs(int keytype, unsigned flags, size_t count, void *worksp
 	if (keys_in == keys_out || keys_in == keys_buffer || keys_out == keys_buffer || payload_out == payload_buffer ||
 		(payload_in && (payload_in == payload_out || payload_in == payload_buffer)))
 		return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	if ((r = check_device_arrays({workspace, keys_in, keys_out, keys_buffer, payload_in, payload_out, payload_buffer},
+		"sort_pairs")))
+		return r;
 	SortJob job;
 	job.key_bytes = key_bytes_of(keytype); job.flags = flags; job.n = count;
 	job.in = keys_in; job.out = keys_out; job.buf = keys_buffer;
@@ -348,6 +366,7 @@ int rsbd8_hip_sort_indices(int keytype, unsigned flags, size_t count, void *work
 	if (!count) return RSBD8_HIP_OK;
 	if (!base || !idx_in || !idx_out || !workspace) return fail(RSBD8_HIP_EINVAL, "null array");
 	if (idx_in == idx_out) return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	if ((r = check_device_arrays({workspace, base, idx_in, idx_out}, "sort_indices"))) return r;
 	const int kb = key_bytes_of(keytype);
 	hipStream_t s = static_cast<hipStream_t>(stream);
 	IndirectWs w = carve_indirect(workspace, workspace_bytes, count, kb);
@@ -392,6 +411,7 @@ int rsbd8_hip_radixsortcopynoalloc_indirect(int keytype, unsigned flags, size_t 
 	if (!count) return RSBD8_HIP_OK;
 	if (!input || !output || !workspace) return fail(RSBD8_HIP_EINVAL, "null array");
 	if (static_cast<const void *>(input) == static_cast<const void *>(output)) return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	if ((r = check_device_arrays({workspace, input, output}, "radixsortcopynoalloc_indirect"))) return r;
 	return sort_pointers(keytype, flags, count, workspace, workspace_bytes, input, output, indirection1, second_level,
 		indirection2, index2, static_cast<hipStream_t>(stream));
 }
@@ -405,6 +425,7 @@ int rsbd8_hip_radixsortnoalloc_indirect(int keytype, unsigned flags, size_t coun
 	if (!count) return RSBD8_HIP_OK;
 	if (!input || !buffer || !workspace) return fail(RSBD8_HIP_EINVAL, "null array");
 	if (input == buffer) return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	if ((r = check_device_arrays({workspace, input, buffer}, "radixsortnoalloc_indirect"))) return r;
 	hipStream_t s = static_cast<hipStream_t>(stream);
 	// permute into buffer, then copy back unless the result is wanted in buffer
 	r = sort_pointers(keytype, flags, count, workspace, workspace_bytes, input, buffer, indirection1, second_level,
@@ -516,6 +537,7 @@ int rsbd8_hip_segmented_run(rsbd8_hip_segplan *plan, const void *input, void *ou
 {
 	if (!plan) return fail(RSBD8_HIP_EINVAL, "null plan");
 	if (input == output || (buffer && (buffer == input || buffer == output))) return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	if (int r = check_device_arrays({input, output, buffer, dev_offsets}, "segmented_run")) return r;
 	const hipError_t e = segplan_run(plan->impl, input, output, buffer, dev_offsets, static_cast<hipStream_t>(stream));
 	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "segmented_run");
 }
@@ -525,6 +547,7 @@ int rsbd8_hip_segmented_capture(rsbd8_hip_segplan *plan, const void *input, void
 {
 	if (!plan) return fail(RSBD8_HIP_EINVAL, "null plan");
 	if (input == output || (buffer && (buffer == input || buffer == output))) return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	if (int r = check_device_arrays({input, output, buffer, dev_offsets}, "segmented_capture")) return r;
 	const hipError_t e = segplan_capture(plan->impl, input, output, buffer, dev_offsets, static_cast<hipStream_t>(stream));
 	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "segmented_capture");
 }
@@ -585,6 +608,7 @@ int sort_with_plan(int keytype, unsigned flags, size_t count, void *workspace, s
 	if (!buffer && needs_buffer(kb, flags) && kb != 1) return fail(RSBD8_HIP_EINVAL, "buffer required");
 	if (plan.algo < 0 || plan.algo > 4 || plan.tile_cfg < -1 || plan.tile_cfg >= kNumTileCfgs)
 		return fail(RSBD8_HIP_EINVAL, "plan out of range");
+	if ((r = check_device_arrays({workspace, input, output, buffer}, what))) return r;
 	SortJob job;
 	job.key_bytes = kb; job.flags = flags; job.n = count;
 	job.in = input; job.out = output; job.buf = buffer;
@@ -633,6 +657,8 @@ int rsbd8_hip_select_plan(int keytype, unsigned flags, size_t count, const void 
 	const int kb = key_bytes_of(keytype);
 	if (!kb || !valid_flags(flags) || !params || !plan_index || (count && !input) || hidden <= 0 || hidden > 64)
 		return fail(RSBD8_HIP_EINVAL, "bad argument");
+	if (count)
+		if (int r = check_device_arrays({input, params}, "select_plan")) return r;
 	int *slot = selection_slot();
 	if (!slot) return fail(RSBD8_HIP_ENOMEM, "pinned selection slot");
 	*slot = -1;
@@ -669,6 +695,8 @@ int rsbd8_hip_features(int keytype, unsigned flags, size_t count, const void *in
 {
 	const int kb = key_bytes_of(keytype);
 	if (!kb || !valid_flags(flags) || !features || (count && !input)) return fail(RSBD8_HIP_EINVAL, "bad argument");
+	if (count)
+		if (int r = check_device_arrays({input, features}, "features")) return r;
 	const hipError_t e = features_launch(kb, flags, count, input, features, static_cast<hipStream_t>(stream));
 	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "features");
 }

@@ -185,9 +185,15 @@ __global__ __launch_bounds__(THREADS) void k_lds_sort(const K *in, K *out, uint3
 	}
 	if (lane == 0) { s_and[w] = k_and; s_or[w] = k_or; }
 	__syncthreads();  // also: every load is done before any store (in place)
-	W differ = 0;
+	// block-wide AND and OR first: a digit can be constant inside every wave's chunk and still
+	// differ between chunks (waves with no key contribute the neutral ~0 and 0)
+	W and_all = ~W(0), or_all = 0;
 #pragma unroll
-	for (int v = 0; v < NW; ++v) differ |= s_and[v] ^ s_or[v];
+	for (int v = 0; v < NW; ++v) {
+		and_all &= s_and[v];
+		or_all |= s_or[v];
+	}
+	const W differ = and_all ^ or_all;
 	for (int p = 0; p < P; ++p) {
 		if (((differ >> (8 * p)) & 0xFFu) == 0) continue;  // block-uniform
 		uint32_t dig[KPT], pos[KPT], total;

@@ -355,3 +355,27 @@ def test_small_pairs_one_lds_block(nbytes):
                 r.sort_pairs(k, ko, kb, None, po, pb, ws, d, mode)
                 _, perm = oracle.radixsort(bits, flags, want_payload=True)
                 assert np.array_equal(host(po), perm), (n, mode, d)
+
+
+def test_noalloc_rejects_host_arrays_on_device():
+    """With a real device: a device workspace with a host input, or a device input with a host output,
+    returns EINVAL before any launch; the Python forms refuse CPU tensors."""
+    import ctypes
+    L = r.lib()
+    n = 4096
+    ws = r.Workspace(r.U32, n)
+    d = dev(np.arange(n, dtype=np.uint32))
+    h = np.zeros(n, np.uint32)
+    hp = h.ctypes.data_as(ctypes.c_void_p)
+    dp = ctypes.c_void_p(d.data_ptr())
+    o = torch.empty_like(d)
+    op = ctypes.c_void_p(o.data_ptr())
+    assert L.rsbd8_hip_radixsortcopynoalloc(r.U32, 0, n, ws.ptr(), ws.nbytes, hp, op, None, None) == -1
+    assert L.rsbd8_hip_radixsortcopynoalloc(r.U32, 0, n, ws.ptr(), ws.nbytes, dp, hp, None, None) == -1
+    assert L.rsbd8_hip_radixsortnoalloc(r.U32, 0, n, ws.ptr(), ws.nbytes, hp, op, 0, None) == -1
+    assert not h.any()
+    with pytest.raises(r.Rsbd8Error):
+        r.radixsortcopynoalloc(d.cpu(), o, torch.empty_like(o), ws)
+    # the same call with device arrays works
+    r.radixsortcopynoalloc(d, o, torch.empty_like(o), ws)
+    assert np.array_equal(host(o), np.arange(n, dtype=np.uint32))

@@ -22,8 +22,8 @@ KAT = json.load(open(os.path.join(HERE, "golden", "kat_reference.json")))
 UINT = {8: np.uint8, 16: np.uint16, 32: np.uint32, 64: np.uint64}
 SINT = {8: np.int8, 16: np.int16, 32: np.int32, 64: np.int64}
 FLOAT = {16: np.float16, 32: np.float32, 64: np.float64}
-MODES = [0, 1, 2, 3, 4, 5, 6, 7]          # explicit sortingmode values, H:130-155
-DIRECTIONS = [0, 1, 2, 3]                  # sortingdirection values, H:162-168
+MODES = [0, 1, 2, 3, 4, 5, 6, 7]          # explicit sortingmode values, H:224-248
+DIRECTIONS = [0, 1, 2, 3]                  # sortingdirection values, H:256-263
 
 
 def _bits_of(case, key="input"):
@@ -67,7 +67,7 @@ def test_documented_mode_orders(case):
 
 
 def test_decode_matches_documented_defaults():
-    """Mode decoding against the enum documentation (H:118-171), not against the code."""
+    """Mode decoding against the enum documentation (H:212-265), not against the code."""
     named = {"forceunsigned": 0, "forcesigned": SIGN, "forceabssigned": ABS | SIGN,
              "forcefloatingp": SIGN | FLTP, "forceabsfloatingp": ABS | SIGN | FLTP,
              "specialfloatingp": FLTP, "forcetieredabsfloatingp": ABS | FLTP,
@@ -79,18 +79,18 @@ def test_decode_matches_documented_defaults():
         for kt in range(10):
             assert oracle.decode(kt, v, 0) == named[name], (name, kt)
     for kt in (oracle.U8, oracle.U16, oracle.U32, oracle.U64):
-        # "no distinct effect when used on an unsigned integer input type" (H:124, H:127)
+        # "no distinct effect when used on an unsigned integer input type" (H:218, H:221)
         for native in (64, 65, 66):
             assert oracle.decode(kt, native, 0) == named["forceunsigned"]
     for kt in (oracle.I8, oracle.I16, oracle.I32, oracle.I64):
-        assert oracle.decode(kt, 64, 0) == named["forcesigned"]            # H:135-136
-        assert oracle.decode(kt, 65, 0) == named["forceabssigned"]         # H:122-123
-        assert oracle.decode(kt, 66, 0) == named["forcetieredabssigned"]   # H:125-126
+        assert oracle.decode(kt, 64, 0) == named["forcesigned"]            # H:229-230
+        assert oracle.decode(kt, 65, 0) == named["forceabssigned"]         # H:216-217
+        assert oracle.decode(kt, 66, 0) == named["forcetieredabssigned"]   # H:219-220
     for kt in (oracle.F32, oracle.F64):
-        assert oracle.decode(kt, 64, 0) == named["forcefloatingp"]         # H:139-140
+        assert oracle.decode(kt, 64, 0) == named["forcefloatingp"]         # H:233-234
         assert oracle.decode(kt, 65, 0) == named["forceabsfloatingp"]
         assert oracle.decode(kt, 66, 0) == named["forcetieredabsfloatingp"]
-    # direction bits (H:162-168); isrevorder is always honoured with indirection
+    # direction bits (H:253-263); isrevorder is always honoured with indirection
     for d in DIRECTIONS:
         f = oracle.decode(oracle.U32, 64, d, True)
         assert bool(f & DESC) == bool(d & 1) and bool(f & REV) == bool(d & 2)
