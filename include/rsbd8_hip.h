@@ -73,6 +73,19 @@ int rsbd8_hip_workspace_init(void *workspace, size_t workspace_bytes, rsbd8_hip_
 int rsbd8_hip_radixsortcopynoalloc(int keytype, unsigned flags, size_t count, void *workspace,
 	size_t workspace_bytes, const void *input, void *output, void *buffer, rsbd8_hip_stream stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): rsbd8_hip_radixsortcopynoalloc
+ * without the workspace argument. Its workspace is a file-scope device array of the library, one
+ * per GPU, sized for RSBD8_HIP_STATIC_MAX_COUNT keys (about 26 MB, held by every GPU the library
+ * runs on until the process ends); a larger count returns RSBD8_HIP_EINVAL. The call allocates
+ * nothing (the first call on a GPU looks the array's address up once). input, output and buffer
+ * are the caller's device arrays as in rsbd8_hip_radixsortcopynoalloc. Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams, other host threads) race
+ * and must be serialized by the caller. */
+#define RSBD8_HIP_STATIC_MAX_COUNT (1u << 24)
+int rsbd8_hip_radixsortcopynoalloc_static(int keytype, unsigned flags, size_t count, const void *input, void *output,
+	void *buffer, rsbd8_hip_stream stream);
+
 /* rsbd8::radixsortnoalloc without indirection: multi-part H:60155-60184, single-part
  * H:60244-60395. The result lands in input, or in buffer when movetobuffer is set; the other
  * array holds garbage afterwards (H:188). buffer may be NULL for 8-bit keys in the bijective

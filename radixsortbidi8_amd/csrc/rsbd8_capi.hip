@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <initializer_list>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -219,6 +220,45 @@ bool needs_buffer(int kb, unsigned flags)
 		m == (RSBD8_HIP_ABS | RSBD8_HIP_SIGN | RSBD8_HIP_FLTP);
 }
 
+// ---------------------------------------------------------------------------------------------
+// The workspace of rsbd8_hip_radixsortcopynoalloc_static: a file-scope device array, one copy per
+// GPU, zero when the code object loads and held until the process ends. Host code never passes
+// the array's name to a kernel (on the host it is only a stand-in for the symbol): it uses the
+// device address hipGetSymbolAddress returns, looked up once per device with that device current.
+
+constexpr uint64_t kStaticMaxCount = RSBD8_HIP_STATIC_MAX_COUNT;
+constexpr size_t kStaticWorkspaceBytes = workspace_bytes_for(kStaticMaxCount);
+static_assert(kStaticWorkspaceBytes % 256 == 0, "the static workspace keeps 256-byte alignment");
+
+static __device__ __align__(256) unsigned char scratch[kStaticWorkspaceBytes];
+
+constexpr int kMaxStaticDevices = 64;
+std::mutex g_static_mu;
+void *g_static_ws[kMaxStaticDevices];
+
+// the device address of scratch on the GPU that runs stream (the current device for the null stream)
+int static_workspace(hipStream_t stream, void **ptr)
+{
+	int cur = 0;
+	hipError_t e = hipGetDevice(&cur);
+	if (e != hipSuccess) return fail_hip(e, "hipGetDevice");
+	int dev = cur;
+	if (stream && (e = hipStreamGetDevice(stream, &dev)) != hipSuccess) return fail_hip(e, "hipStreamGetDevice");
+	if (dev < 0 || dev >= kMaxStaticDevices) return fail(RSBD8_HIP_EINVAL, "device index beyond the static workspace table");
+	std::lock_guard<std::mutex> lock(g_static_mu);
+	if (!g_static_ws[dev]) {
+		if (dev != cur && (e = hipSetDevice(dev)) != hipSuccess) return fail_hip(e, "hipSetDevice");
+		void *p = nullptr;
+		e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (dev != cur) (void)hipSetDevice(cur);
+		if (e != hipSuccess) return fail_hip(e, "hipGetSymbolAddress(scratch)");
+		if (!p) return fail(RSBD8_HIP_EDEVICE, "hipGetSymbolAddress(scratch) returned no address");
+		g_static_ws[dev] = p;
+	}
+	*ptr = g_static_ws[dev];
+	return RSBD8_HIP_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -309,6 +349,36 @@ int rsbd8_hip_radixsortcopynoalloc(int keytype, unsigned flags, size_t count, vo
 	job.algo = g_algo.load();
 	const hipError_t e = sort_launch(job, workspace, workspace_bytes, static_cast<hipStream_t>(stream));
 	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "radixsortcopynoalloc");
+}
+
+int rsbd8_hip_radixsortcopynoalloc_static(int keytype, unsigned flags, size_t count, const void *input, void *output,
+	void *buffer, rsbd8_hip_stream stream)
+{
+	const int kb = key_bytes_of(keytype);
+	if (!kb) return fail(RSBD8_HIP_EINVAL, "unknown key type");
+	if (!valid_flags(flags)) return fail(RSBD8_HIP_EINVAL, "unknown flag bits");
+	if (count > kStaticMaxCount)
+		return fail(RSBD8_HIP_EINVAL, "radixsortcopynoalloc_static: count above RSBD8_HIP_STATIC_MAX_COUNT (2^24 keys), the "
+			"capacity of the static workspace; use rsbd8_hip_radixsortcopynoalloc with a workspace argument");
+	if (!count) return RSBD8_HIP_OK;
+	if (!input || !output) return fail(RSBD8_HIP_EINVAL, "null array");
+	if (input == output || (buffer && (buffer == input || buffer == output))) return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	if (!buffer && needs_buffer(kb, flags) && kb != 1) return fail(RSBD8_HIP_EINVAL, "buffer required");
+	int r = check_device_arrays({input, output, buffer}, "radixsortcopynoalloc_static");
+	if (r) return r;
+	const hipStream_t s = static_cast<hipStream_t>(stream);
+	void *ws = nullptr;
+	if ((r = static_workspace(s, &ws))) return r;
+	// the previous call on this GPU left the array in whatever state it ended in
+	hipError_t e = workspace_reset(ws, s);
+	if (e != hipSuccess) return fail_hip(e, "radixsortcopynoalloc_static");
+	SortJob job;
+	job.key_bytes = kb; job.flags = flags; job.n = count;
+	job.in = input; job.out = output; job.buf = buffer;
+	job.cfg = g_tile_cfg.load();
+	job.algo = g_algo.load();
+	e = sort_launch(job, ws, kStaticWorkspaceBytes, s);
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "radixsortcopynoalloc_static");
 }
 
 int rsbd8_hip_radixsortnoalloc(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,

@@ -102,8 +102,19 @@ constexpr int kAlgoAuto = 0, kAlgoOnesweep = 1, kAlgoCountScan = 2, kAlgoLds = 3
 constexpr uint32_t kLdsMax = 1024 * 16;    // keys-only sorts one LDS block takes
 constexpr uint32_t kLdsPayMax = 512 * 16;  // the same with a u32 payload
 
+// bytes of a keys-only workspace for n keys: the header, then one tile record per kMinTile keys,
+// one more, and one partial tile per top-digit segment of the MSD plan
+constexpr size_t workspace_bytes_for(uint64_t n)
+{
+	return kHeaderBytes + static_cast<size_t>((n + kMinTile - 1) / kMinTile + 1 + kRadix) * kRecordBytes;
+}
+
 size_t sort_workspace_bytes(uint64_t n);
 hipError_t workspace_init(void *ws, size_t bytes, hipStream_t stream);
+// Brings a workspace that another sort used (or left half-done after an error) to the state every
+// sort expects: histograms zero, tile and scan counters and flags zero, a fresh look-back epoch.
+// One small kernel (no memset node), so a caller may capture it.
+hipError_t workspace_reset(void *ws, hipStream_t stream);
 hipError_t sort_launch(const SortJob &job, void *ws, size_t ws_bytes, hipStream_t stream);
 int default_cfg(int key_bytes, uint64_t n, bool payload, bool float_keys);
 int default_algo(int key_bytes, uint64_t n, bool keys_only, bool float_keys);

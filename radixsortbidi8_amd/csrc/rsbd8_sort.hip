@@ -42,9 +42,6 @@ inline uint32_t *counts_of(void *ws)
 	return reinterpret_cast<uint32_t *>(static_cast<char *>(ws) + kHeaderBytes + kRadix * sizeof(uint64_t));
 }
 inline uint64_t max_tiles(uint64_t n) { return (n + kMinTile - 1) / kMinTile + 1; }
-// tile records: the tiles of a pass, plus one partial tile per top-digit segment when the tiles of
-// the segmented MSD pass are aligned to the segments
-inline uint64_t record_count(uint64_t n) { return max_tiles(n) + kRadix; }
 
 __device__ __forceinline__ uint32_t pass_tag(uint32_t epoch, int slot)
 {
@@ -1295,7 +1292,7 @@ __global__ __launch_bounds__(256) void k_fill16(const uint32_t *__restrict__ sta
 	}
 }
 
-__global__ void k_finish(Control *ctrl)
+__device__ __forceinline__ void reset_control(Control *ctrl)
 {
 	if (threadIdx.x < kMaxPasses) {
 		ctrl->tile_counter[threadIdx.x] = 0;
@@ -1311,6 +1308,15 @@ __global__ void k_finish(Control *ctrl)
 		ctrl->key_or = 0;
 		ctrl->key_nor = 0;
 	}
+}
+
+__global__ void k_finish(Control *ctrl) { reset_control(ctrl); }
+
+// the histograms (which k_plan leaves zeroed) and the control block, whatever state they are in
+__global__ __launch_bounds__(256) void k_reset(Control *ctrl, uint32_t *hist)
+{
+	for (uint32_t i = threadIdx.x; i < kMaxPasses * kRadix; i += blockDim.x) hist[i] = 0;
+	reset_control(ctrl);
 }
 
 __global__ void k_zero(uint64_t *p, uint64_t words)
@@ -1875,8 +1881,16 @@ int default_cfg(int key_bytes, uint64_t n, bool payload, bool float_keys)
 
 size_t sort_workspace_bytes(uint64_t n)
 {
-	// the header, then one record (look-back row + count row) per tile
-	return kHeaderBytes + static_cast<size_t>(record_count(n)) * kRecordBytes;
+	// the header, then one record (look-back row + count row) per tile: max_tiles(n), plus one
+	// partial tile per top-digit segment when the tiles of the segmented MSD pass are aligned to
+	// the segments
+	return workspace_bytes_for(n);
+}
+
+hipError_t workspace_reset(void *ws, hipStream_t stream)
+{
+	hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, stream, ctrl_of(ws), hist_of(ws));
+	return hipGetLastError();
 }
 
 hipError_t workspace_init(void *ws, size_t bytes, hipStream_t stream)
