@@ -1,10 +1,19 @@
 // kat_main.cpp -- the reference's portable known-answer tests (multiplatform/main.cpp:10-304) and
 // the documented direction example (Radixsortbidi8.hpp:265-274), written against the drop-in
-// header include/rsbd8_hip.hpp, plus device-array runs of the noalloc forms. Exits non-zero on the
-// first mismatch and prints which check failed.
+// header include/rsbd8_hip.hpp, plus:
+//   - the compile-time offsets of main.cpp:235-238 (static_assert on getoffsetof);
+//   - every noalloc form, with and without indirection, on device arrays and allocatearray
+//     workspaces (T**, member object, getter, type+offset; copy, in place, to buffer);
+//   - indexed first-level and second-level indirection (one and two index arguments), getters
+//     returning a pointer or a reference, with and without isindexed2, in all four directions.
+// Expected orders come from std::stable_sort over the keys read on the host (ties in source order,
+// or reversed for the rev directions). Exits non-zero on the first mismatch and prints which check
+// failed.
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <numeric>
 #include <vector>
 
 #include "../../include/rsbd8_hip.hpp"
@@ -17,16 +26,27 @@ class Testmeclass {
 public:
 	std::uint64_t co;
 	std::int64_t sco;
-	std::uint64_t get() const noexcept { return co; }
-	std::uint64_t getwparam(int testfor8in) const noexcept { return testfor8in == 8 ? co : 0; }
-	std::int64_t sget() const noexcept { return sco; }
-	Testmeclass(std::uint64_t input) noexcept : co{input}, sco{static_cast<std::int64_t>(input) - 1} {}
+	constexpr std::uint64_t get() const noexcept { return co; }
+	constexpr std::uint64_t getwparam(int testfor8in) const noexcept { return testfor8in == 8 ? co : 0; }
+	constexpr std::int64_t sget() const noexcept { return sco; }
+	constexpr std::uint64_t bget() noexcept { return co; }
+	constexpr Testmeclass(std::uint64_t input) noexcept : co{input}, sco{static_cast<std::int64_t>(input) - 1} {}
 };
 #pragma pack(pop)
 
 struct myclass {
 	int keyorder;
 	char const *name;
+};
+
+// a record for the indexed forms: four 32-bit keys (first-level index), a pointer to 16-bit values
+// (second-level index), and getters onto both
+struct Rec {
+	std::uint32_t k[4];
+	std::uint16_t const *vals;
+	std::uint16_t const *second() const noexcept { return vals; }
+	std::uint32_t const &third() const noexcept { return k[3]; }
+	std::uint16_t const *offsetvals(int add) const noexcept { return vals + add; }
 };
 
 static int failures;
@@ -37,6 +57,212 @@ static int failures;
 			++failures;                                                                \
 		}                                                                              \
 	} while (0)
+
+// the compile-time offsets of main.cpp:233-238
+static std::size_t constexpr sizecontainer{sizeof(Testmeclass)};
+static_assert(25 == sizecontainer);
+static std::size_t constexpr offsetco{rsbd8::getoffsetof<&Testmeclass::co>};
+static_assert(9 == offsetco);
+static std::size_t constexpr offsetsco{rsbd8::getoffsetof<&Testmeclass::sco>};
+static_assert(17 == offsetsco);
+static_assert(0 == rsbd8::getoffsetof<&myclass::keyorder> && 8 == rsbd8::getoffsetof<&myclass::name>);
+static_assert(16 == rsbd8::getoffsetof<&Rec::vals>);
+static_assert(5 == rsbd8::getoffsetof<&rsbd8::memberobjectgenerator<std::uint32_t, 5>::object>);
+static_assert(0 == rsbd8::getoffsetof<&rsbd8::memberobjectgenerator<double, 0>::object>);
+
+// the stable order of keys (ascending, or descending), ties in source order or reversed
+template<typename K>
+std::vector<std::size_t> stableorder(std::vector<K> const &keys, bool desc, bool rev)
+{
+	std::vector<std::size_t> idx(keys.size());
+	std::iota(idx.begin(), idx.end(), std::size_t{});
+	if (rev) std::reverse(idx.begin(), idx.end());
+	std::stable_sort(idx.begin(), idx.end(), [&](std::size_t a, std::size_t b) { return desc ? keys[b] < keys[a] : keys[a] < keys[b]; });
+	return idx;
+}
+
+template<typename T>
+T *todevice(std::vector<T> const &h)
+{
+	T *d{};
+	if (hipMalloc(reinterpret_cast<void **>(&d), h.size() * sizeof(T) + 8) != hipSuccess) return nullptr;
+	if (hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+	return d;
+}
+
+template<typename T>
+std::vector<T> tohost(T const *d, std::size_t n)
+{
+	std::vector<T> h(n);
+	if (hipMemcpy(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) h.clear();
+	return h;
+}
+
+static std::uint64_t rng_state{0x9E3779B97F4A7C15u};
+static std::uint64_t next()
+{
+	rng_state ^= rng_state << 13;
+	rng_state ^= rng_state >> 7;
+	rng_state ^= rng_state << 17;
+	return rng_state;
+}
+
+// pointer results as indices into a record array
+template<typename V>
+std::vector<std::size_t> indices(std::vector<V *> const &ptrs, V *base)
+{
+	std::vector<std::size_t> r(ptrs.size());
+	for (std::size_t i{}; i != ptrs.size(); ++i) r[i] = static_cast<std::size_t>(ptrs[i] - base);
+	return r;
+}
+
+// every indirection form of one direction over n records, host and device arrays
+template<rsbd8::sortingdirection dir>
+void indirection_forms(std::size_t n)
+{
+	constexpr bool desc{rsbd8::helper::isdescsort<dir>}, rev{rsbd8::helper::isrevorderwithindirection<dir>};
+	// records on the host, and the same records (with device value arrays) on the device
+	std::vector<std::uint16_t> vals(n * 3);
+	for (auto &v : vals) v = static_cast<std::uint16_t>(next() % 97);  // many ties
+	std::uint16_t *dvals{todevice(vals)};
+	std::vector<Rec> hrec(n), drec_h(n);
+	for (std::size_t i{}; i != n; ++i) {
+		for (auto &k : hrec[i].k) k = static_cast<std::uint32_t>(next() % 1000u);
+		hrec[i].vals = vals.data() + 3 * i;
+		drec_h[i] = hrec[i];
+		drec_h[i].vals = dvals + 3 * i;
+	}
+	Rec *drec{todevice(drec_h)};
+	std::vector<Rec *> hin(n), din_h(n);
+	for (std::size_t i{}; i != n; ++i) {
+		hin[i] = hrec.data() + (i * 7919u) % n;
+		din_h[i] = drec + (i * 7919u) % n;
+	}
+	Rec **din{todevice(din_h)};
+	Rec **dout{}, **dbuf{};
+	CHECK(hipMalloc(reinterpret_cast<void **>(&dout), n * sizeof(Rec *)) == hipSuccess);
+	CHECK(hipMalloc(reinterpret_cast<void **>(&dbuf), n * sizeof(Rec *)) == hipSuccess);
+	auto [alloc, size] = rsbd8::allocatearray<false, false, false, std::uint32_t, true>(n);
+	CHECK(alloc.pbuffer != nullptr);
+	rsbd8::buffermemorywrapper guard{alloc.pbuffer, size};
+	void *z{alloc.pzeroedindices};
+	// the source order of the pointer arrays, as record indices
+	std::vector<std::size_t> src(n);
+	for (std::size_t i{}; i != n; ++i) src[i] = (i * 7919u) % n;
+	auto expect{[&](auto keyof) {
+		using K = decltype(keyof(std::size_t{}));
+		std::vector<K> keys(n);
+		for (std::size_t i{}; i != n; ++i) keys[i] = keyof(src[i]);
+		std::vector<std::size_t> order{stableorder(keys, desc, rev)};
+		for (auto &o : order) o = src[o];
+		return order;
+	}};
+	auto dres{[&](Rec **d) { return indices(tohost(d, n), drec); }};
+	std::vector<Rec *> hout(n), hbuf(n);
+
+	// first level, indexed: key = k[2] through the member k[0] displaced by 2 elements (H:2755-2758)
+	{
+		auto const want{expect([&](std::size_t r) { return hrec[r].k[2]; })};
+		CHECK((rsbd8::radixsortcopy<std::uint32_t, 0, dir>(n, hin.data(), hout.data(), 0, 2)) && indices(hout, hrec.data()) == want);
+		rsbd8::radixsortcopynoalloc<std::uint32_t, 0, dir>(n, 0u, nullptr, z, din, dout, dbuf, 2);
+		CHECK(rsbd8::lasterror() == 0 && dres(dout) == want);
+		CHECK(hipMemcpy(dbuf, din, n * sizeof(Rec *), hipMemcpyDeviceToDevice) == hipSuccess);
+		rsbd8::radixsortnoalloc<std::uint32_t, 0, dir>(n, 0u, nullptr, z, dbuf, dout, false, 2);
+		CHECK(rsbd8::lasterror() == 0 && dres(dbuf) == want);
+		rsbd8::radixsortnoalloc<std::uint32_t, 0, dir>(n, 0u, nullptr, z, din, dout, true, 2);
+		CHECK(rsbd8::lasterror() == 0 && dres(dout) == want);
+	}
+	// type+offset with a byte offset: k[1] at +4
+	{
+		auto const want{expect([&](std::size_t r) { return hrec[r].k[1]; })};
+		CHECK((rsbd8::radixsortcopy<std::uint32_t, 4, dir>(n, hin.data(), hout.data())) && indices(hout, hrec.data()) == want);
+		rsbd8::radixsortcopynoalloc<std::uint32_t, 4, dir>(n, 0u, nullptr, z, din, dout, dbuf);
+		CHECK(rsbd8::lasterror() == 0 && dres(dout) == want);
+		hbuf = hin;
+		CHECK((rsbd8::radixsort<std::uint32_t, 4, dir>(n, hbuf.data())) && indices(hbuf, hrec.data()) == want);
+	}
+	// second level through the pointer member, no index: vals[0]; indexed: vals[2]; and both
+	// indices: the pointer member of the record displaced by one pointer... (records are not
+	// contiguous pointers, so the two-index form is taken through a type+offset at the pointer)
+	{
+		auto const want0{expect([&](std::size_t r) { return vals[3 * r]; })};
+		CHECK((rsbd8::radixsortcopy<&Rec::vals, dir>(n, hin.data(), hout.data())) && indices(hout, hrec.data()) == want0);
+		rsbd8::radixsortcopynoalloc<&Rec::vals, dir>(n, 0u, nullptr, z, din, dout, dbuf);
+		CHECK(rsbd8::lasterror() == 0 && dres(dout) == want0);
+		auto const want2{expect([&](std::size_t r) { return vals[3 * r + 2]; })};
+		CHECK((rsbd8::radixsortcopy<&Rec::vals, dir, rsbd8::sortingmode::native, 0, true>(n, hin.data(), hout.data(), 0, 2)) &&
+			indices(hout, hrec.data()) == want2);
+		rsbd8::radixsortcopynoalloc<&Rec::vals, dir, rsbd8::sortingmode::native, 0, true>(n, 0u, nullptr, z, din, dout, dbuf, 2);
+		CHECK(rsbd8::lasterror() == 0 && dres(dout) == want2);
+		// indirection2 in bytes: vals + 2 bytes, index 1 -> vals[2]
+		rsbd8::radixsortcopynoalloc<&Rec::vals, dir, rsbd8::sortingmode::native, 2, true>(n, 0u, nullptr, z, din, dout, dbuf, 1);
+		CHECK(rsbd8::lasterror() == 0 && dres(dout) == want2);
+		CHECK(hipMemcpy(dbuf, din, n * sizeof(Rec *), hipMemcpyDeviceToDevice) == hipSuccess);
+		rsbd8::radixsortnoalloc<&Rec::vals, dir, rsbd8::sortingmode::native, 0, true>(n, 0u, nullptr, z, dbuf, dout, false, 2);
+		CHECK(rsbd8::lasterror() == 0 && dres(dbuf) == want2);
+		// type+offset of a pointer type with two indices: the first displaces the record by whole
+		// pointers (16 + 0 * 8: the vals member), the second indexes the values
+		CHECK((rsbd8::radixsortcopy<std::uint16_t *, 16, dir, rsbd8::sortingmode::native, 0, true>(n, hin.data(), hout.data(), 0, 0, 1)) &&
+			indices(hout, hrec.data()) == expect([&](std::size_t r) { return vals[3 * r + 1]; }));
+		rsbd8::radixsortcopynoalloc<std::uint16_t *, 8, dir, rsbd8::sortingmode::native, 0, true>(n, 0u, nullptr, z, din, dout, dbuf, 1, 1);
+		CHECK(rsbd8::lasterror() == 0 && dres(dout) == expect([&](std::size_t r) { return vals[3 * r + 1]; }));
+	}
+	// getters (host arrays only): a pointer with isindexed2 (the first argument is the index), a
+	// pointer with an argument for the getter, and a reference (read as a pointer to its object)
+	{
+		CHECK((rsbd8::radixsortcopy<&Rec::second, dir, rsbd8::sortingmode::native, 0, true>(n, hin.data(), hout.data(), 0, 1)) &&
+			indices(hout, hrec.data()) == expect([&](std::size_t r) { return vals[3 * r + 1]; }));
+		CHECK((rsbd8::radixsortcopy<&Rec::offsetvals, dir, rsbd8::sortingmode::native, 0, true>(n, hin.data(), hout.data(), 0, 1, 1)) &&
+			indices(hout, hrec.data()) == expect([&](std::size_t r) { return vals[3 * r + 2]; }));
+		CHECK((rsbd8::radixsortcopy<&Rec::third, dir>(n, hin.data(), hout.data())) &&
+			indices(hout, hrec.data()) == expect([&](std::size_t r) { return hrec[r].k[3]; }));
+		rsbd8::radixsortcopynoalloc<&Rec::third, dir, rsbd8::sortingmode::native, -4>(n, 0u, nullptr, z, hin.data(), hout.data(), hbuf.data());
+		CHECK(rsbd8::lasterror() == 0 && indices(hout, hrec.data()) == expect([&](std::size_t r) { return hrec[r].k[2]; }));
+		hbuf = hin;
+		rsbd8::radixsortnoalloc<&Rec::second, dir, rsbd8::sortingmode::native, 0, true>(n, 0u, nullptr, z, hbuf.data(), hout.data(), false, 2);
+		CHECK(rsbd8::lasterror() == 0 && indices(hbuf, hrec.data()) == expect([&](std::size_t r) { return vals[3 * r + 2]; }));
+		// a getter cannot run on device memory
+		rsbd8::radixsortcopynoalloc<&Rec::second, dir, rsbd8::sortingmode::native, 0, true>(n, 0u, nullptr, z, din, dout, dbuf, 1);
+		CHECK(rsbd8::lasterror() == RSBD8_HIP_EINVAL);
+	}
+	// T**: key = p[index], with the pointers into the device value array
+	{
+		std::vector<std::uint16_t const *> hp(n);
+		std::vector<std::uint16_t *> dp_h(n);
+		for (std::size_t i{}; i != n; ++i) {
+			hp[i] = vals.data() + 3 * src[i];
+			dp_h[i] = dvals + 3 * src[i];
+		}
+		std::vector<std::uint16_t const *> hpo(n);
+		auto const want1{expect([&](std::size_t r) { return vals[3 * r + 1]; })};
+		CHECK(rsbd8::radixsortcopy<dir>(n, hp.data(), hpo.data(), 0, 1));
+		std::vector<std::size_t> got(n);
+		for (std::size_t i{}; i != n; ++i) got[i] = static_cast<std::size_t>(hpo[i] - vals.data()) / 3;
+		CHECK(got == want1);
+		std::uint16_t **dp{todevice(dp_h)};
+		std::uint16_t **dpo{}, **dpb{};
+		CHECK(hipMalloc(reinterpret_cast<void **>(&dpo), n * sizeof(void *)) == hipSuccess);
+		CHECK(hipMalloc(reinterpret_cast<void **>(&dpb), n * sizeof(void *)) == hipSuccess);
+		rsbd8::radixsortcopynoalloc<dir>(n, 0u, nullptr, z, dp, dpo, dpb, 1);
+		CHECK(rsbd8::lasterror() == 0);
+		auto const res{tohost(dpo, n)};
+		for (std::size_t i{}; i != n; ++i) got[i] = static_cast<std::size_t>(res[i] - dvals) / 3;
+		CHECK(got == want1);
+		rsbd8::radixsortnoalloc<dir>(n, 0u, nullptr, z, dp, dpb, true);
+		CHECK(rsbd8::lasterror() == 0);
+		auto const res0{tohost(dpb, n)};
+		for (std::size_t i{}; i != n; ++i) got[i] = static_cast<std::size_t>(res0[i] - dvals) / 3;
+		CHECK(got == expect([&](std::size_t r) { return vals[3 * r]; }));
+		static_cast<void>(hipFree(dp));
+		static_cast<void>(hipFree(dpo));
+		static_cast<void>(hipFree(dpb));
+	}
+	static_cast<void>(hipFree(dvals));
+	static_cast<void>(hipFree(drec));
+	static_cast<void>(hipFree(din));
+	static_cast<void>(hipFree(dout));
+	static_cast<void>(hipFree(dbuf));
+}
 
 int main()
 {
@@ -76,15 +302,12 @@ int main()
 		CHECK(*outim[0] == (1u << 31 | 18) && *outim[1] == (1u << 31 | 2) && *outim[2] == (1u << 31 | 2) && *outim[3] == 0 &&
 			*outim[4] == 3 && *outim[5] == 3 && *outim[6] == 8);
 	}
+	Testmeclass cin[]{8, 0, 6, 4, 0, 2, 6};
+	// ascending by co with ties in source order: 0(1) 0(4) 2(5) 4(3) 6(2) 6(6) 8(0)
+	int const want[7]{1, 4, 5, 3, 2, 6, 0};
 	{ // main.cpp:217-304, getter, member and type+offset indirection on a packed 25-byte class
-		static_assert(25 == sizeof(Testmeclass));
-		CHECK(9 == rsbd8::getoffsetof<&Testmeclass::co>);
-		CHECK(17 == rsbd8::getoffsetof<&Testmeclass::sco>);
-		Testmeclass cin[]{8, 0, 6, 4, 0, 2, 6};
 		Testmeclass const *fin[7]{cin, cin + 1, cin + 2, cin + 3, cin + 4, cin + 5, cin + 6};
 		Testmeclass const *fout[7];
-		// ascending by co with ties in source order: 0(1) 0(4) 2(5) 4(3) 6(2) 6(6) 8(0)
-		int const want[7]{1, 4, 5, 3, 2, 6, 0};
 		auto same = [&] {
 			for (int i = 0; i < 7; ++i)
 				if (fout[i] != fin[want[i]]) return false;
@@ -97,6 +320,16 @@ int main()
 		CHECK(rsbd8::radixsortcopy<&Testmeclass::sget>(7, fin, fout) && same());
 		CHECK(rsbd8::radixsortcopy<&Testmeclass::sco>(7, fin, fout) && same());
 		CHECK((rsbd8::radixsortcopy<std::int64_t, 17>(7, fin, fout)) && same());
+		Testmeclass *yin[7]{cin, cin + 1, cin + 2, cin + 3, cin + 4, cin + 5, cin + 6};
+		Testmeclass *yout[7];
+		auto ysame = [&] {
+			for (int i = 0; i < 7; ++i)
+				if (yout[i] != yin[want[i]]) return false;
+			return true;
+		};
+		// main.cpp:286-304: non-const getters work on non-const records
+		CHECK(rsbd8::radixsortcopy<&Testmeclass::bget>(7, yin, yout) && ysame());
+		CHECK(rsbd8::radixsortcopy<&Testmeclass::getwparam>(7, yin, yout, 0, 8) && ysame());
 	}
 	{ // Radixsortbidi8.hpp:265-274: dscfwdorder keeps tie order, dscrevorder reverses it
 		myclass collA[]{{1, "first"}, {1, "second"}, {-5, "third"}, {2, "fourth"}};
@@ -110,11 +343,7 @@ int main()
 	{ // device arrays through the noalloc forms with an allocatearray workspace
 		std::size_t const n{1u << 20};
 		std::vector<std::uint64_t> h(n);
-		std::uint64_t x{0x9E3779B97F4A7C15u};
-		for (auto &v : h) {
-			x ^= x << 13; x ^= x >> 7; x ^= x << 17;
-			v = x;
-		}
+		for (auto &v : h) v = next();
 		auto [alloc, size] = rsbd8::allocatearray<false, false, false, std::int64_t, false>(n);
 		CHECK(alloc.pbuffer != nullptr);
 		rsbd8::buffermemorywrapper guard{alloc.pbuffer, size};
@@ -135,8 +364,54 @@ int main()
 		ok = true;
 		for (std::size_t i = 1; i < n; ++i) ok &= got[i - 1] <= got[i];
 		CHECK(ok);
+		// a host array handed to a noalloc form is rejected before any launch
+		std::vector<std::int64_t> hostarr(n);
+		rsbd8::radixsortcopynoalloc(n, 0u, nullptr, alloc.pzeroedindices, reinterpret_cast<std::int64_t const *>(h.data()),
+			hostarr.data(), alloc.pbuffer);
+		CHECK(rsbd8::lasterror() == RSBD8_HIP_EINVAL);
 		static_cast<void>(hipFree(din));
 		static_cast<void>(hipFree(dout));
+	}
+	{ // the packed class on the device: member, type+offset, in place and to buffer, single pointer
+		std::vector<Testmeclass> hc(cin, cin + 7);
+		Testmeclass *dc{todevice(hc)};
+		std::vector<Testmeclass *> dptr(7);
+		for (int i = 0; i < 7; ++i) dptr[i] = dc + i;
+		Testmeclass **din{todevice(dptr)}, **dout{}, **dbuf{};
+		CHECK(hipMalloc(reinterpret_cast<void **>(&dout), 7 * sizeof(void *)) == hipSuccess);
+		CHECK(hipMalloc(reinterpret_cast<void **>(&dbuf), 7 * sizeof(void *)) == hipSuccess);
+		auto [alloc, size] = rsbd8::allocatearray<false, false, false, std::uint64_t, true>(7);
+		CHECK(alloc.pbuffer != nullptr);
+		rsbd8::buffermemorywrapper guard{alloc.pbuffer, size};
+		auto dsame = [&](Testmeclass **d) {
+			auto const r{tohost(d, 7)};
+			for (int i = 0; i < 7; ++i)
+				if (r[i] != dc + want[i]) return false;
+			return true;
+		};
+		rsbd8::radixsortcopynoalloc<&Testmeclass::co>(7, 0u, nullptr, alloc.pzeroedindices, din, dout, dbuf);
+		CHECK(rsbd8::lasterror() == 0 && dsame(dout));
+		rsbd8::radixsortcopynoalloc<std::int64_t, 17>(7, 0u, nullptr, alloc.pzeroedindices, din, dout, dbuf);
+		CHECK(rsbd8::lasterror() == 0 && dsame(dout));
+		rsbd8::radixsortnoalloc<&Testmeclass::sco>(7, 0u, nullptr, alloc.pzeroedindices, din, dbuf, true);
+		CHECK(rsbd8::lasterror() == 0 && dsame(dbuf));
+		rsbd8::radixsortnoalloc<std::uint64_t, 9>(7, 0u, nullptr, alloc.pzeroedindices, din, dbuf);
+		CHECK(rsbd8::lasterror() == 0 && dsame(din));
+		CHECK(rsbd8::radixsortcopy<&Testmeclass::co>(7, din, dout) && dsame(dout));
+		CHECK((rsbd8::radixsort<std::uint64_t, 9>(7, din)) && dsame(din));
+		// count 1 with device pointer arrays: the element is copied on the device
+		rsbd8::radixsortcopynoalloc<&Testmeclass::co>(1, 0u, nullptr, alloc.pzeroedindices, din + 3, dout, dbuf);
+		CHECK(rsbd8::lasterror() == 0 && tohost(dout, 1)[0] == dc + want[3]);
+		static_cast<void>(hipFree(dc));
+		static_cast<void>(hipFree(din));
+		static_cast<void>(hipFree(dout));
+		static_cast<void>(hipFree(dbuf));
+	}
+	for (std::size_t n : {std::size_t{1000}, std::size_t{70001}}) {
+		indirection_forms<rsbd8::sortingdirection::ascfwdorder>(n);
+		indirection_forms<rsbd8::sortingdirection::dscfwdorder>(n);
+		indirection_forms<rsbd8::sortingdirection::ascrevorder>(n);
+		indirection_forms<rsbd8::sortingdirection::dscrevorder>(n);
 	}
 	std::printf("%s (%d failures)\n", failures ? "FAILED" : "ok", failures);
 	return failures ? 1 : 0;
