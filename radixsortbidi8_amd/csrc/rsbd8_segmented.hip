@@ -48,7 +48,9 @@ struct SegCtrl {
 	uint32_t q_small;    // queued short segments (long runs)
 	uint32_t q_sub;      // queued items of long segments (long runs)
 	uint32_t n_sub;      // items made from the buckets of the long segments
-	uint32_t pad[3];
+	uint32_t n_big;      // buckets above the item capacity, up to the largest tier: items of their own
+	uint32_t q_big;      // queued big items (long runs)
+	uint32_t pad[1];
 };
 
 // per long segment: sel holds, for pass p, bits 4p..4p+1 = source (0 input, 1 output, 2 buffer) and
@@ -407,22 +409,25 @@ __global__ __launch_bounds__(THREADS) void k_seg_count1(const K *__restrict__ in
 	for (int i = threadIdx.x; i < 256; i += THREADS) counts[static_cast<uint64_t>(g) * 256 + i] = copies_sum<C>(s_t, i);
 }
 
-// One wave per long segment (four segments per block), four consecutive top digits per lane:
+// One wave per long segment (sixteen segments per block), four consecutive top digits per lane:
 // bucket sizes and starts, the segment's tile counts turned into output offsets, and its buckets
-// packed greedily into items of at most cap keys. A bucket above cap flags the segment for the LSD
-// passes (and skips everything else). The items of a block's segments take their slots with one
-// atomic per block: one per segment serialised on the item counter. Measured on the 14,278 long
-// segments of 65,536 log-uniform arrays: 175 us with a block and an atomic per segment, 59 us with
-// four segments per block, 37 us with sixteen.
+// packed greedily into items of at most cap keys. A bucket above cap but within bigcap (the
+// largest LDS tier) is an item of its own in the big-item table, sorted by that tier; a bucket
+// above bigcap flags the segment for the LSD passes (and skips everything else). The items of a
+// block's segments take their slots with one atomic per table and block: one per segment
+// serialised on the item counter. Measured on the 14,278 long segments of 65,536 log-uniform
+// arrays: 175 us with a block and an atomic per segment, 59 us with four segments per block, 37 us
+// with sixteen.
 constexpr uint32_t kScan1Segs = 16;  // segments (waves) per block
 __global__ __launch_bounds__(64 * kScan1Segs) void k_seg_scan1(const uint64_t *__restrict__ seg,
 	const uint32_t *__restrict__ large_ids, uint32_t nlarge, const uint32_t *__restrict__ tile_first,
-	uint32_t *__restrict__ counts, uint32_t cap, uint32_t *__restrict__ flag, uint32_t *__restrict__ ftiles, SegCtrl *ctrl,
-	uint64_t *__restrict__ istart, uint32_t *__restrict__ ilen)
+	uint32_t *__restrict__ counts, uint32_t cap, uint32_t bigcap, uint32_t *__restrict__ flag, uint32_t *__restrict__ ftiles,
+	SegCtrl *ctrl, uint64_t *__restrict__ istart, uint32_t *__restrict__ ilen, uint64_t *__restrict__ bstart,
+	uint32_t *__restrict__ blen)
 {
 	__shared__ uint32_t s_pre[kScan1Segs][257];
 	__shared__ uint16_t s_next[kScan1Segs][256];
-	__shared__ uint32_t s_items[kScan1Segs];
+	__shared__ uint32_t s_items[kScan1Segs], s_bigs[kScan1Segs];
 	const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
 	const uint32_t L = blockIdx.x * kScan1Segs + w;
 	const bool have = L < nlarge;  // wave-uniform
@@ -440,7 +445,7 @@ __global__ __launch_bounds__(64 * kScan1Segs) void k_seg_scan1(const uint64_t *_
 			const uint4 c = reinterpret_cast<const uint4 *>(counts + static_cast<uint64_t>(tl) * 256)[lane];
 			tot.x += c.x; tot.y += c.y; tot.z += c.z; tot.w += c.w;
 		}
-		if (__ballot(tot.x > cap || tot.y > cap || tot.z > cap || tot.w > cap)) {
+		if (__ballot(tot.x > bigcap || tot.y > bigcap || tot.z > bigcap || tot.w > bigcap)) {
 			// the segment's tiles join the list of the LSD passes as one run in tile order, so a
 			// tile's look-back only meets tiles of its segment that were queued before it
 			uint32_t at = 0;
@@ -475,7 +480,8 @@ __global__ __launch_bounds__(64 * kScan1Segs) void k_seg_scan1(const uint64_t *_
 	__syncthreads();
 	// The buckets are packed greedily, in digit order, into items of at most cap keys. An item that
 	// starts at bucket b ends before the first bucket e past the largest prefix within cap:
-	// next[b] = max{e : pre[e] - pre[b] <= cap} (> b, as every bucket fits on its own).
+	// next[b] = max{e : pre[e] - pre[b] <= cap}. A bucket above cap (at most bigcap here) is an item
+	// of its own: next[b] = b + 1; the greedy packing of its predecessors stops before it.
 	if (scan) {
 #pragma unroll
 		for (uint32_t i = 0; i < 4; ++i) {
@@ -485,36 +491,55 @@ __global__ __launch_bounds__(64 * kScan1Segs) void k_seg_scan1(const uint64_t *_
 #pragma unroll
 			for (uint32_t step = 256; step; step >>= 1)
 				if (e + step <= 256u && pre[e + step] - base <= cap) e += step;
-			next[b] = static_cast<uint16_t>(e);
+			next[b] = static_cast<uint16_t>(e > b ? e : b + 1u);
 		}
 	}
 	__syncthreads();
 	if (lane == 0) {
-		uint32_t nitems = 0;
+		uint32_t nitems = 0, nbig = 0;
 		if (scan) {
 			const uint32_t total = pre[256];
-			for (uint32_t b = 0; b < 256u && pre[b] < total; b = next[b]) ++nitems;
+			for (uint32_t b = 0; b < 256u && pre[b] < total; b = next[b]) {
+				const bool big = pre[b + 1] - pre[b] > cap;
+				nbig += big ? 1u : 0u;
+				nitems += big ? 0u : 1u;
+			}
 		}
 		s_items[w] = nitems;
+		s_bigs[w] = nbig;
 	}
 	__syncthreads();
 	if (threadIdx.x == 0) {
-		uint32_t sum = 0;
-		for (uint32_t v = 0; v < kScan1Segs; ++v) sum += s_items[v];
-		uint32_t at = sum ? atomicAdd(&ctrl->n_sub, sum) : 0u;
+		uint32_t sum = 0, bsum = 0;
 		for (uint32_t v = 0; v < kScan1Segs; ++v) {
-			const uint32_t c = s_items[v];
+			sum += s_items[v];
+			bsum += s_bigs[v];
+		}
+		uint32_t at = sum ? atomicAdd(&ctrl->n_sub, sum) : 0u;
+		uint32_t bat = bsum ? atomicAdd(&ctrl->n_big, bsum) : 0u;
+		for (uint32_t v = 0; v < kScan1Segs; ++v) {
+			const uint32_t c = s_items[v], bc = s_bigs[v];
 			s_items[v] = at;
+			s_bigs[v] = bat;
 			at += c;
+			bat += bc;
 		}
 	}
 	__syncthreads();
 	if (lane != 0 || !scan) return;
 	const uint32_t total = pre[256];
-	uint32_t it = s_items[w];
-	for (uint32_t b = 0; b < 256u && pre[b] < total; b = next[b], ++it) {
-		istart[it] = sb + pre[b];
-		ilen[it] = (next[b] < 256u ? pre[next[b]] : total) - pre[b];
+	uint32_t it = s_items[w], bit = s_bigs[w];
+	for (uint32_t b = 0; b < 256u && pre[b] < total; b = next[b]) {
+		const uint32_t len = (next[b] < 256u ? pre[next[b]] : total) - pre[b];
+		if (len > cap) {
+			bstart[bit] = sb + pre[b];
+			blen[bit] = len;
+			++bit;
+		} else {
+			istart[it] = sb + pre[b];
+			ilen[it] = len;
+			++it;
+		}
 	}
 }
 
@@ -818,6 +843,8 @@ __global__ void k_seg_finish(SegCtrl *ctrl, uint32_t *flag, uint32_t nlarge)
 		ctrl->q_small = 0;
 		ctrl->q_sub = 0;
 		ctrl->n_sub = 0;
+		ctrl->n_big = 0;
+		ctrl->q_big = 0;
 		ctrl->epoch = ctrl->epoch + 1u;
 	}
 }
@@ -862,8 +889,9 @@ struct SegPlan {
 	char *mem = nullptr;  // one device allocation for every table and the workspace
 	size_t mem_bytes = 0;
 	SegCtrl *ctrl = nullptr;
-	uint64_t *small_start = nullptr, *sub_start = nullptr;
-	uint32_t *small_len = nullptr, *sub_len = nullptr, *q_small = nullptr, *q_sub = nullptr;
+	uint64_t *small_start = nullptr, *sub_start = nullptr, *big_start = nullptr;
+	uint32_t *small_len = nullptr, *sub_len = nullptr, *q_small = nullptr, *q_sub = nullptr, *big_len = nullptr,
+		*q_big = nullptr;
 	uint32_t *large_ids = nullptr, *tile_first = nullptr, *counts1 = nullptr, *flag = nullptr, *ftiles = nullptr;
 	uint32_t *hist = nullptr, *bucket = nullptr;
 	SegTile *tiles = nullptr;
@@ -1018,9 +1046,10 @@ hipError_t launch_large(SegPlan &pl, const void *in, void *out, void *buf, const
 	hipLaunchKernelGGL((k_seg_count1<K, 256>), dim3(pl.ntiles), dim3(256), 0, s, static_cast<const K *>(in), pl.tiles,
 		pl.tile_keys, xf, rev, pl.counts1);
 	if ((e = hipGetLastError()) != hipSuccess) return e;
+	constexpr uint32_t kBigCap = static_cast<uint32_t>(kSmallCaps[kNumSmallCaps - 1]);
 	hipLaunchKernelGGL(k_seg_scan1, dim3((pl.nlarge + kScan1Segs - 1) / kScan1Segs), dim3(64 * kScan1Segs), 0, s, seg,
-		pl.large_ids, pl.nlarge, pl.tile_first, pl.counts1,
-		pl.small_cap, pl.flag, pl.ftiles, pl.ctrl, pl.sub_start, pl.sub_len);
+		pl.large_ids, pl.nlarge, pl.tile_first, pl.counts1, pl.small_cap, kBigCap, pl.flag, pl.ftiles, pl.ctrl, pl.sub_start,
+		pl.sub_len, pl.big_start, pl.big_len);
 	if ((e = hipGetLastError()) != hipSuccess) return e;
 	switch (pl.tile_cfg) {
 	case 0: e = launch_scatter1<K, 256, 8>(pl, in, buf, seg, xf, rev, s); break;
@@ -1034,6 +1063,13 @@ hipError_t launch_large(SegPlan &pl, const void *in, void *out, void *buf, const
 		&pl.ctrl->q_sub, 0, s)) != hipSuccess) return e;
 	if ((e = launch_lsd_cfg<K>(pl.small_cfg, buf, out, pl.sub_start, pl.sub_len, pl.q_sub, &pl.ctrl->q_sub, xf, 0, s)) !=
 		hipSuccess) return e;
+	if (pl.small_cfg < kNumSmallCaps - 1) {
+		// buckets above the item capacity: single-bucket items on the largest tier
+		if ((e = launch_sort_cfg<K>(kNumSmallCaps - 1, buf, out, pl.big_start, pl.big_len, &pl.ctrl->n_big, 0, xf, 0, pl.q_big,
+			&pl.ctrl->q_big, 0, s)) != hipSuccess) return e;
+		if ((e = launch_lsd_cfg<K>(kNumSmallCaps - 1, buf, out, pl.big_start, pl.big_len, pl.q_big, &pl.ctrl->q_big, xf, 0, s)) !=
+			hipSuccess) return e;
+	}
 	// segments with a bucket above the LDS capacity: the LSD passes (every launch skips the others)
 	hipLaunchKernelGGL((k_seg_hist<K, 256>), dim3(std::min<uint32_t>(pl.ntiles, 8u * cus)), dim3(256), 0, s,
 		static_cast<const K *>(in), pl.tiles, pl.ftiles, pl.ctrl, pl.tile_keys, xf, pl.hist);
@@ -1158,6 +1194,11 @@ hipError_t segplan_create(int key_bytes, unsigned flags, uint64_t nseg, const ui
 	const size_t o_substart = off; off += up256(nsub * 8 + 8);
 	const size_t o_sublen = off; off += up256(nsub * 4 + 4);
 	const size_t o_qsub = off; off += up256(nsub * 4 + 4);
+	// big items: buckets above the item capacity (none when the capacity is the largest tier)
+	const size_t nbig = small_cfg < kNumSmallCaps - 1 ? nsub : 0;
+	const size_t o_bigstart = off; off += up256(nbig * 8 + 8);
+	const size_t o_biglen = off; off += up256(nbig * 4 + 4);
+	const size_t o_qbig = off; off += up256(nbig * 4 + 4);
 	const size_t o_sel = off; off += up256(large.size() * sizeof(SegSel) + 8);
 	const size_t o_hist = off; off += up256(large.size() * P * 256 * 4 + 4);
 	const size_t o_bucket = off; off += up256(large.size() * P * 256 * 4 + 4);
@@ -1178,6 +1219,9 @@ hipError_t segplan_create(int key_bytes, unsigned flags, uint64_t nseg, const ui
 	pl->sub_start = reinterpret_cast<uint64_t *>(pl->mem + o_substart);
 	pl->sub_len = reinterpret_cast<uint32_t *>(pl->mem + o_sublen);
 	pl->q_sub = reinterpret_cast<uint32_t *>(pl->mem + o_qsub);
+	pl->big_start = reinterpret_cast<uint64_t *>(pl->mem + o_bigstart);
+	pl->big_len = reinterpret_cast<uint32_t *>(pl->mem + o_biglen);
+	pl->q_big = reinterpret_cast<uint32_t *>(pl->mem + o_qbig);
 	pl->sel = reinterpret_cast<SegSel *>(pl->mem + o_sel);
 	pl->hist = reinterpret_cast<uint32_t *>(pl->mem + o_hist);
 	pl->bucket = reinterpret_cast<uint32_t *>(pl->mem + o_bucket);

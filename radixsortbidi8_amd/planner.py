@@ -458,12 +458,15 @@ def load_table(path=None):
 
 # ---- segmented batches: tile configuration and LDS tier ------------------------------------------
 
-def segmented_batch(nseg: int, total_log2: int, seed: int = 0x5EED0005):
-    """CSR offsets of nseg segments with log-uniform lengths 2^4..2^20 scaled to about 2^total_log2
-    keys in total (the 65,536-array batch workload)."""
+def segmented_batch(nseg: int, total_log2=None, seed: int = 0x5EED0005):
+    """CSR offsets of nseg segments with lengths floor(2^U(4, 20)) (the 65,536-array batch workload:
+    about 6.2e9 keys for 65,536 segments), or those lengths scaled to about 2^total_log2 keys in
+    total when total_log2 is given."""
     rng = np.random.default_rng(seed)
     lens = np.floor(2.0 ** rng.uniform(4, 20, nseg))
-    lens = np.maximum(1, np.floor(lens * ((1 << total_log2) / lens.sum()))).astype(np.int64)
+    if total_log2 is not None:
+        lens = np.maximum(1, np.floor(lens * ((1 << total_log2) / lens.sum())))
+    lens = lens.astype(np.int64)
     off = np.zeros(nseg + 1, np.uint64)
     off[1:] = np.cumsum(lens)
     return off

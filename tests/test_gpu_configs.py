@@ -187,6 +187,64 @@ def test_c4_segmented_65536_arrays_2p28_total():
     plan.close()
 
 
+def test_c4_segmented_65536_arrays_stated_lengths():
+    """C4 as stated: 65,536 int32 arrays with lengths floor(2^U(4, 20)) (seed 0x5EED0005), about
+    6.2e9 keys, generated on the device. Every segment's order and three multiset checksums on the
+    GPU (in chunks of segments), 64 sampled segments and the longest bit-exact against the oracle,
+    and the captured graph's result equal to the plain run's."""
+    nseg = 65536
+    off = pl.segmented_batch(nseg, None, 0x5EED0005)
+    lens = np.diff(off).astype(np.int64)
+    total = int(off[-1])
+    say(f"C4 stated: {nseg} segments, {total} keys, longest {int(lens.max())}")
+    gen = torch.Generator(device="cuda").manual_seed(0x5EED0005 + 1)
+    d_in = torch.randint(-2**31, 2**31 - 1, (total,), dtype=torch.int32, device="cuda", generator=gen)
+    out, buf = torch.empty_like(d_in), torch.empty_like(d_in)
+    plan = r.SegmentedPlan(np.int32, off)
+    say(f"C4 stated: plan {plan.info()}")
+    plan.run(d_in, out, buf)
+    torch.cuda.synchronize()
+    say("C4 stated: sorted; checking segment order and checksums")
+    first = torch.zeros(total, dtype=torch.bool, device="cuda")
+    first[torch.from_numpy(off[:-1].astype(np.int64)).cuda()] = True
+    assert bool(((out[1:] >= out[:-1]) | first[1:]).all().item())
+    del first
+    # per-segment multiset checksums, over chunks of whole segments of at most 2^28 keys
+    lo = 0
+    while lo < nseg:
+        hi = lo
+        while hi < nseg and int(off[hi + 1] - off[lo]) <= (1 << 28):
+            hi += 1
+        a, b = int(off[lo]), int(off[hi])
+        seg_id = torch.repeat_interleave(torch.arange(hi - lo, device="cuda"), torch.from_numpy(lens[lo:hi]).cuda())
+        for x, y in zip(segment_checksums(d_in[a:b], seg_id, hi - lo), segment_checksums(out[a:b], seg_id, hi - lo)):
+            assert torch.equal(x, y), (lo, hi)
+        del seg_id
+        lo = hi
+    flags = r.decode(r.I32)
+    sample = list(np.random.default_rng(3).choice(nseg, 64, replace=False)) + [int(np.argmax(lens))]
+    for s in sample:
+        a, b = int(off[s]), int(off[s + 1])
+        exp, _ = oracle.radixsort(d_in[a:b].cpu().numpy(), flags)
+        assert np.array_equal(out[a:b].cpu().numpy(), exp), s
+    say("C4 stated: graph replay")
+    out2 = torch.empty_like(d_in)
+    plan.capture(d_in, out2, buf)
+    plan.launch()
+    torch.cuda.synchronize()
+    assert torch.equal(out, out2)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    stream = torch.cuda.current_stream()
+    ev[0].record(stream)
+    for _ in range(3):
+        plan.launch(stream)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / 3
+    say(f"C4 stated: {total} keys in {ms:.2f} ms per graph launch, {total / ms / 1e6:.1f} GKeys/s")
+    plan.close()
+
+
 def test_c5_selector_stream_4096_requests():
     table = pl.load_table()
     sel = pl.Selector.load(pl.TUNING_DIR + "/selector_model.npz")
