@@ -33,7 +33,15 @@ typedef void *rsbd8_hip_stream; /* a hipStream_t; NULL is the null stream */
 enum rsbd8_hip_keytype {
 	RSBD8_HIP_U8 = 0, RSBD8_HIP_U16 = 1, RSBD8_HIP_U32 = 2, RSBD8_HIP_U64 = 3,
 	RSBD8_HIP_I8 = 4, RSBD8_HIP_I16 = 5, RSBD8_HIP_I32 = 6, RSBD8_HIP_I64 = 7,
-	RSBD8_HIP_F32 = 8, RSBD8_HIP_F64 = 9
+	RSBD8_HIP_F32 = 8, RSBD8_HIP_F64 = 9,
+	/* Keys wider than 64 bits (H:1061-1804, H:2186-2404), little-endian records: the 80-bit
+	 * extended float (64-bit mantissa, then 16 bits of sign and exponent) in its packed 10-byte
+	 * layout (longdoubletest80), its 12-byte layout (longdoubletest96) and its 16-byte layout
+	 * (longdoubletest128, x86-64 long double), whose padding bytes are moved but never compared;
+	 * and 128-bit integers (test128, unsigned __int128 / __int128), low word first. They decode as
+	 * long double (signed, floating point) and as the 128-bit integers. Taken by the noalloc,
+	 * allocating, sort_pairs and _indirect entry points. */
+	RSBD8_HIP_LD80 = 10, RSBD8_HIP_LD96 = 11, RSBD8_HIP_LD128 = 12, RSBD8_HIP_U128 = 13, RSBD8_HIP_I128 = 14
 };
 
 /* decoded flags: isabsvalue, issignmode, isfltpmode, isdescsort, isrevorder */

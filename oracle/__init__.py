@@ -69,8 +69,41 @@ def lib() -> ctypes.CDLL:
         L.rsbd8o_sortkeys.restype = None
         L.rsbd8o_radixsort_mt.argtypes = [c_u, c_u, c_size, c_vp, c_vp, c_u]
         L.rsbd8o_radixsort_mt.restype = ctypes.c_int
+        L.rsbd8o_sortkey_wide.argtypes = [ctypes.c_int, c_u, c_vp, c_vp, c_vp]
+        L.rsbd8o_sortkey_wide.restype = None
+        L.rsbd8o_radixsort_wide.argtypes = [ctypes.c_int, c_u, c_size, c_vp, c_size, c_vp]
+        L.rsbd8o_radixsort_wide.restype = ctypes.c_int
         _lib = L
     return _lib
+
+
+# keys wider than 64 bits (H:1061-1804, H:2186-2404): the 80-bit extended float in 10-, 12- and
+# 16-byte records, and 128-bit integers; bytes per record and value bits
+LD80, LD96, LD128, U128, I128 = 10, 11, 12, 13, 14
+WIDE_LAYOUT = {LD80: (10, 80), LD96: (12, 80), LD128: (16, 80), U128: (16, 128), I128: (16, 128)}
+
+
+def sortkey_wide(keytype: int, flags: int, recs: np.ndarray):
+    """The 128-bit sort keys (as Python ints) of the records in recs (uint8, n * record bytes)."""
+    rb = WIDE_LAYOUT[keytype][0]
+    b = np.ascontiguousarray(recs, dtype=np.uint8).reshape(-1, rb)
+    lo, hi = ctypes.c_uint64(), ctypes.c_uint64()
+    out = []
+    for row in b:
+        lib().rsbd8o_sortkey_wide(keytype, flags, row.ctypes.data_as(ctypes.c_void_p), ctypes.byref(lo), ctypes.byref(hi))
+        out.append((hi.value << 64) | lo.value)
+    return out
+
+
+def radixsort_wide(keytype: int, flags: int, recs: np.ndarray) -> np.ndarray:
+    """The stable permutation of the records in recs (uint8, n * record bytes) by their sort keys."""
+    rb = WIDE_LAYOUT[keytype][0]
+    b = np.ascontiguousarray(recs, dtype=np.uint8).ravel()
+    n = b.size // rb
+    perm = np.empty(n, np.uint32)
+    assert lib().rsbd8o_radixsort_wide(keytype, flags, n, b.ctypes.data_as(ctypes.c_void_p), rb,
+                                       perm.ctypes.data_as(ctypes.c_void_p)) == 0
+    return perm
 
 
 def keytype_of(dtype) -> int:

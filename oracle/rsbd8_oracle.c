@@ -35,6 +35,10 @@ unsigned rsbd8o_decode(int keytype, unsigned mode, unsigned direction, int indir
 	int issigned = keytype >= RSBD8O_I8;           /* std::is_signed_v<T>: true for floats too */
 	int isfloat = keytype >= RSBD8O_F32;           /* std::is_floating_point_v<T> */
 	unsigned abs, sign, fltp, flags;
+	if (keytype >= RSBD8O_LD80) {                   /* the 80-bit layouts are long double (H:2849-2856) */
+		isfloat = keytype <= RSBD8O_LD128;
+		issigned = isfloat || keytype == RSBD8O_I128;
+	}
 	if (mode >= 64u) {                              /* native, nativeabs, nativetieredabs */
 		abs = mode >= 65u && issigned;          /* H:59779-59780 */
 		sign = mode <= 65u && issigned;         /* H:59809-59811 */
@@ -452,4 +456,107 @@ int rsbd8o_radixsort_mt(unsigned w, unsigned flags, size_t n, const void *in, vo
 done:
 	free(jobs); free(tid); free(counts); free(offsets); free(a); free(b);
 	return executed;
+}
+
+/* ---- keys wider than 64 bits: H:1061-1804 (layouts), H:7479-7530 and H:9047-9150 (filters) ---- */
+
+typedef unsigned __int128 u128o;
+
+static int wide_layout(int keytype, unsigned *rec_bytes, unsigned *value_bits)
+{
+	switch (keytype) {
+	case RSBD8O_LD80: *rec_bytes = 10u; *value_bits = 80u; return 1;
+	case RSBD8O_LD96: *rec_bytes = 12u; *value_bits = 80u; return 1;
+	case RSBD8O_LD128: *rec_bytes = 16u; *value_bits = 80u; return 1;
+	case RSBD8O_U128: case RSBD8O_I128: *rec_bytes = 16u; *value_bits = 128u; return 1;
+	default: return 0;
+	}
+}
+
+/* the closed forms of rsbd8o_sortkey (Appendix-A table of the narrow keys) for W = 80 or 128 */
+static u128o wide_key(unsigned w, unsigned flags, u128o x)
+{
+	const u128o ones = w >= 128u ? ~(u128o)0 : (((u128o)1 << w) - 1u), msb = (u128o)1 << (w - 1u);
+	u128o s, smask, m, k;
+	x &= ones;
+	s = x >> (w - 1u);
+	smask = ((u128o)0 - s) & ones;
+	m = x & ~msb;
+	switch (flags & 7u) {
+	case 0u: k = x; break;
+	case RSBD8O_SIGN: k = x ^ msb; break;
+	case RSBD8O_ABS | RSBD8O_SIGN: k = ((x ^ smask) - smask) & ones; break;
+	case RSBD8O_ABS: k = ((x << 1) ^ smask) & ones; break;
+	case RSBD8O_SIGN | RSBD8O_FLTP: k = (((m ^ smask) - smask) & ones) ^ msb; break;
+	case RSBD8O_ABS | RSBD8O_SIGN | RSBD8O_FLTP: k = m; break;
+	case RSBD8O_FLTP: k = m ^ smask; break;
+	default: k = (((x << 1) | s) & ones) ^ 1u; break;
+	}
+	return (flags & RSBD8O_DESC) ? k ^ ones : k;
+}
+
+static u128o wide_value(const uint8_t *p, unsigned value_bits)
+{
+	uint64_t lo, hi = 0;
+	memcpy(&lo, p, 8);
+	memcpy(&hi, p + 8, value_bits == 80u ? 2u : 8u);
+	return ((u128o)hi << 64) | lo;
+}
+
+void rsbd8o_sortkey_wide(int keytype, unsigned flags, const void *rec, uint64_t *lo, uint64_t *hi)
+{
+	unsigned rb, vb;
+	u128o k;
+	if (!wide_layout(keytype, &rb, &vb)) { *lo = *hi = 0; return; }
+	k = wide_key(vb, flags, wide_value((const uint8_t *)rec, vb));
+	*lo = (uint64_t)k;
+	*hi = (uint64_t)(k >> 64);
+}
+
+struct wsctx {
+	const u128o *key;
+	int rev;
+};
+
+/* i before j in the stable order: smaller key, or an equal key earlier in the logical order */
+static int wide_before(const struct wsctx *c, uint32_t i, uint32_t j)
+{
+	if (c->key[i] != c->key[j]) return c->key[i] < c->key[j];
+	return c->rev ? i > j : i < j;
+}
+
+static void wide_msort(const struct wsctx *c, uint32_t *v, uint32_t *tmp, size_t n)
+{
+	size_t h, a, b, o;
+	if (n < 2u) return;
+	h = n / 2u;
+	wide_msort(c, v, tmp, h);
+	wide_msort(c, v + h, tmp, n - h);
+	a = 0; b = h; o = 0;
+	while (a < h && b < n) tmp[o++] = wide_before(c, v[b], v[a]) ? v[b++] : v[a++];
+	while (a < h) tmp[o++] = v[a++];
+	while (b < n) tmp[o++] = v[b++];
+	memcpy(v, tmp, n * sizeof(uint32_t));
+}
+
+int rsbd8o_radixsort_wide(int keytype, unsigned flags, size_t n, const void *in, size_t stride, uint32_t *perm)
+{
+	unsigned rb, vb;
+	u128o *key;
+	uint32_t *tmp;
+	struct wsctx c;
+	if (!wide_layout(keytype, &rb, &vb) || stride < rb || n > 0xFFFFFFFFu) return -1;
+	key = (u128o *)malloc((n ? n : 1) * sizeof(u128o));
+	tmp = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+	if (!key || !tmp) { free(key); free(tmp); return -1; }
+	for (size_t i = 0; i < n; ++i) {
+		key[i] = wide_key(vb, flags, wide_value((const uint8_t *)in + i * stride, vb));
+		perm[i] = (uint32_t)i;
+	}
+	c.key = key;
+	c.rev = (flags & RSBD8O_REV) != 0;
+	wide_msort(&c, perm, tmp, n);
+	free(key);
+	free(tmp);
+	return 0;
 }

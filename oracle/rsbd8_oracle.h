@@ -34,7 +34,10 @@ enum {
 enum {
 	RSBD8O_U8 = 0, RSBD8O_U16 = 1, RSBD8O_U32 = 2, RSBD8O_U64 = 3,
 	RSBD8O_I8 = 4, RSBD8O_I16 = 5, RSBD8O_I32 = 6, RSBD8O_I64 = 7,
-	RSBD8O_F32 = 8, RSBD8O_F64 = 9
+	RSBD8O_F32 = 8, RSBD8O_F64 = 9,
+	/* wider than 64 bits: the 80-bit extended float in 10-, 12- and 16-byte records
+	 * (longdoubletest80/96/128, H:1061-1804) and 128-bit integers (test128, H:2186-2404) */
+	RSBD8O_LD80 = 10, RSBD8O_LD96 = 11, RSBD8O_LD128 = 12, RSBD8O_U128 = 13, RSBD8O_I128 = 14
 };
 
 /* mode decoding of H:59778-59904 for a plain key type code (enums pass their unsigned
@@ -76,6 +79,19 @@ int rsbd8o_sort_segmented(unsigned width_bits, unsigned flags, size_t nseg, cons
 /* 8-bit-digit histograms (digit d of pass p = (sortkey >> 8p) & 255, descending applied as a
  * complement), counts[p*256 + d]; used to check a device histogram. */
 void rsbd8o_histogram8(unsigned width_bits, unsigned flags, size_t n, const void *in, uint64_t *counts);
+
+/* Keys wider than 64 bits. The value of a record is its first 80 bits (the 80-bit layouts: 64-bit
+ * mantissa, then sign and exponent in the next 16 bits; padding beyond them is ignored, as the
+ * reference's filters read only signexponent's low 16 bits, H:7479-7530) or its 128 bits (low
+ * word first, H:9047-9150), and its sort key the closed form of rsbd8o_sortkey with W = 80 or 128
+ * (sign at bit W-1). rsbd8o_sortkey_wide writes the key's low and high 64-bit words, descending
+ * applied as a complement of the W bits. */
+void rsbd8o_sortkey_wide(int keytype, unsigned flags, const void *rec, uint64_t *lo, uint64_t *hi);
+
+/* Stable sort of n wide-key records (stride bytes apart) by their sort keys, ties in source order
+ * (reversed under RSBD8O_REV): perm[i] = source index of the i-th record of the result. A merge
+ * sort on the 128-bit keys, independent of the device's two-pass method. */
+int rsbd8o_radixsort_wide(int keytype, unsigned flags, size_t n, const void *in, size_t stride, uint32_t *perm);
 
 /* Multithreaded variant of rsbd8o_radixsort (pthreads; counting and scatter sliced across
  * threads with per-thread offsets, as the reference's split counting H:28919 plus slice

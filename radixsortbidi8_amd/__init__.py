@@ -56,6 +56,11 @@ class sortingdirection(enum.IntEnum):
 ABS, SIGN, FLTP, DESC, REV = 1, 2, 4, 8, 16
 LAYOUT_KEYS, LAYOUT_PAIRS, LAYOUT_INDIRECT = 0, 1, 2
 U8, U16, U32, U64, I8, I16, I32, I64, F32, F64 = range(10)
+# keys wider than 64 bits (H:1061-1804, H:2186-2404): the 80-bit extended float in its 10-, 12- and
+# 16-byte records (longdoubletest80/96/128; the 16-byte one is x86-64's long double, numpy's
+# longdouble) and 128-bit integers, passed as byte records
+LD80, LD96, LD128, U128, I128 = 10, 11, 12, 13, 14
+WIDE_RECORD_BYTES = {LD80: 10, LD96: 12, LD128: 16, U128: 16, I128: 16}
 TILE_CONFIGS = ((256, 8), (256, 16), (512, 8), (1024, 12), (256, 24))
 SEGMENTED_TILE_CONFIGS = ((256, 8), (256, 16), (512, 8), (512, 16), (256, 24))
 
@@ -254,21 +259,31 @@ def _flags(dtype, mode, direction, indirect=False, keytype=None):
     return kt, decode(kt, mode, direction, indirect)
 
 
+def _count(a, keytype: int) -> int:
+    """Number of keys in an array: its elements, or for the wide key types its bytes over the
+    record size (wide records travel as uint8 tensors or arrays, WIDE_RECORD_BYTES each)."""
+    if keytype in WIDE_RECORD_BYTES:
+        nbytes = a.numel() * a.element_size() if _is_tensor(a) else a.nbytes
+        return nbytes // WIDE_RECORD_BYTES[keytype]
+    return a.numel() if _is_tensor(a) else a.size
+
+
 # ---- the four templates without indirection ----------------------------------------------
 
 def radixsortcopynoalloc(input, output, buffer, workspace: Workspace,
-                         direction=sortingdirection.ascfwdorder, mode=sortingmode.native, stream=None):
-    """rsbd8::radixsortcopynoalloc (H:60121-60153; 8-bit keys H:60186-60242, buffer may be None)."""
-    kt, f = _flags(input.dtype, mode, direction)
-    _check(_load().rsbd8_hip_radixsortcopynoalloc(kt, f, input.numel(), workspace.ptr(), workspace.nbytes,
+                         direction=sortingdirection.ascfwdorder, mode=sortingmode.native, stream=None, keytype=None):
+    """rsbd8::radixsortcopynoalloc (H:60121-60153; 8-bit keys H:60186-60242, buffer may be None).
+    keytype overrides the key type of input's dtype (the wide types: uint8 records)."""
+    kt, f = _flags(input.dtype, mode, direction, keytype=keytype)
+    _check(_load().rsbd8_hip_radixsortcopynoalloc(kt, f, _count(input, kt), workspace.ptr(), workspace.nbytes,
                                                    _ptr(input), _ptr(output), _ptr(buffer), _stream_ptr(stream)))
 
 
 def radixsortnoalloc(input, buffer, workspace: Workspace, movetobuffer: bool = False,
-                     direction=sortingdirection.ascfwdorder, mode=sortingmode.native, stream=None):
+                     direction=sortingdirection.ascfwdorder, mode=sortingmode.native, stream=None, keytype=None):
     """rsbd8::radixsortnoalloc (H:60155-60184; 8-bit keys H:60244-60395)."""
-    kt, f = _flags(input.dtype, mode, direction)
-    _check(_load().rsbd8_hip_radixsortnoalloc(kt, f, input.numel(), workspace.ptr(), workspace.nbytes,
+    kt, f = _flags(input.dtype, mode, direction, keytype=keytype)
+    _check(_load().rsbd8_hip_radixsortnoalloc(kt, f, _count(input, kt), workspace.ptr(), workspace.nbytes,
                                                _ptr(input), _ptr(buffer), int(bool(movetobuffer)),
                                                _stream_ptr(stream)))
 
@@ -282,29 +297,30 @@ def _host_call(fn, kt, f, arrays, count, stream):
 
 
 def radixsortcopy(input, output, direction=sortingdirection.ascfwdorder, mode=sortingmode.native,
-                  stream=None) -> bool:
+                  stream=None, keytype=None) -> bool:
     """rsbd8::radixsortcopy (H:60468-60588): torch GPU tensors or host numpy arrays."""
     if _is_tensor(input):
-        kt, f = _flags(input.dtype, mode, direction)
+        kt, f = _flags(input.dtype, mode, direction, keytype=keytype)
         return _host_call(_load().rsbd8_hip_radixsortcopy, kt, f, (_ptr(input, False), _ptr(output, False)),
-                          input.numel(), _stream_ptr(stream))
+                          _count(input, kt), _stream_ptr(stream))
     inp = np.ascontiguousarray(input)
     assert output.flags.c_contiguous and output.dtype == inp.dtype and output.size == inp.size
-    kt, f = _flags(inp.dtype, mode, direction)
+    kt, f = _flags(inp.dtype, mode, direction, keytype=keytype)
     return _host_call(_load().rsbd8_hip_radixsortcopy, kt, f,
-                      (inp.ctypes.data_as(ctypes.c_void_p), output.ctypes.data_as(ctypes.c_void_p)), inp.size,
+                      (inp.ctypes.data_as(ctypes.c_void_p), output.ctypes.data_as(ctypes.c_void_p)), _count(inp, kt),
                       None if stream is None else _stream_ptr(stream))
 
 
-def radixsort(input, direction=sortingdirection.ascfwdorder, mode=sortingmode.native, stream=None) -> bool:
+def radixsort(input, direction=sortingdirection.ascfwdorder, mode=sortingmode.native, stream=None,
+              keytype=None) -> bool:
     """rsbd8::radixsort (H:60397-60466), in place: torch GPU tensor or host numpy array."""
     if _is_tensor(input):
-        kt, f = _flags(input.dtype, mode, direction)
-        return _host_call(_load().rsbd8_hip_radixsort, kt, f, (_ptr(input, False),), input.numel(),
+        kt, f = _flags(input.dtype, mode, direction, keytype=keytype)
+        return _host_call(_load().rsbd8_hip_radixsort, kt, f, (_ptr(input, False),), _count(input, kt),
                           _stream_ptr(stream))
     assert input.flags.c_contiguous
-    kt, f = _flags(input.dtype, mode, direction)
-    return _host_call(_load().rsbd8_hip_radixsort, kt, f, (input.ctypes.data_as(ctypes.c_void_p),), input.size,
+    kt, f = _flags(input.dtype, mode, direction, keytype=keytype)
+    return _host_call(_load().rsbd8_hip_radixsort, kt, f, (input.ctypes.data_as(ctypes.c_void_p),), _count(input, kt),
                       None if stream is None else _stream_ptr(stream))
 
 
@@ -315,8 +331,9 @@ def radixsortcopynoalloc_pointers(input_ptrs, output_ptrs, workspace: Workspace,
                                   mode=sortingmode.native, second_level: bool = False,
                                   indirection2: int = 0, index2: int = 0, stream=None):
     """T**/type+offset/member indirection (H:60590-60627, H:61360-61393): input_ptrs is an int64
-    tensor of device addresses; output_ptrs receives them sorted by the key they point at."""
-    kt = keytype_of(key_dtype)
+    tensor of device addresses; output_ptrs receives them sorted by the key they point at. key_dtype
+    is a dtype or a key type code (the wide types)."""
+    kt = key_dtype if isinstance(key_dtype, int) else keytype_of(key_dtype)
     f = decode(kt, mode, direction, True)
     _check(_load().rsbd8_hip_radixsortcopynoalloc_indirect(
         kt, f, input_ptrs.numel(), workspace.ptr(), workspace.nbytes, _ptr(input_ptrs), _ptr(output_ptrs),
@@ -328,7 +345,7 @@ def radixsortnoalloc_pointers(input_ptrs, buffer_ptrs, workspace: Workspace, key
                               mode=sortingmode.native, second_level: bool = False, indirection2: int = 0,
                               index2: int = 0, stream=None):
     """In-place pointer sort (H:61017-61045, H:61395-61427)."""
-    kt = keytype_of(key_dtype)
+    kt = key_dtype if isinstance(key_dtype, int) else keytype_of(key_dtype)
     f = decode(kt, mode, direction, True)
     _check(_load().rsbd8_hip_radixsortnoalloc_indirect(
         kt, f, input_ptrs.numel(), workspace.ptr(), workspace.nbytes, _ptr(input_ptrs), _ptr(buffer_ptrs),
@@ -347,11 +364,12 @@ def sort_indices(records, stride: int, key_offset: int, idx_in, idx_out, workspa
 
 
 def sort_pairs(keys_in, keys_out, keys_buffer, payload_in, payload_out, payload_buffer, workspace: Workspace,
-               direction=sortingdirection.ascfwdorder, mode=sortingmode.native, indirect: bool = True, stream=None):
+               direction=sortingdirection.ascfwdorder, mode=sortingmode.native, indirect: bool = True, stream=None,
+               keytype=None):
     """Stable key sort moving a u32 payload along (payload_in None: the source index)."""
-    kt = keytype_of(keys_in.dtype)
+    kt = keytype_of(keys_in.dtype) if keytype is None else keytype
     f = decode(kt, mode, direction, indirect)
-    _check(_load().rsbd8_hip_sort_pairs(kt, f, keys_in.numel(), workspace.ptr(), workspace.nbytes, _ptr(keys_in),
+    _check(_load().rsbd8_hip_sort_pairs(kt, f, _count(keys_in, kt), workspace.ptr(), workspace.nbytes, _ptr(keys_in),
                                          _ptr(keys_out), _ptr(keys_buffer), _ptr(payload_in), _ptr(payload_out),
                                          _ptr(payload_buffer), _stream_ptr(stream)))
 

@@ -157,6 +157,20 @@ int key_bytes_of(int keytype)
 	}
 }
 
+// keys wider than 64 bits: bytes per record (0 for the narrow types) and value bits
+int wide_bytes_of(int keytype)
+{
+	switch (keytype) {
+	case RSBD8_HIP_LD80: return 10;
+	case RSBD8_HIP_LD96: return 12;
+	case RSBD8_HIP_LD128: case RSBD8_HIP_U128: case RSBD8_HIP_I128: return 16;
+	default: return 0;
+	}
+}
+int wide_bits_of(int keytype) { return keytype == RSBD8_HIP_U128 || keytype == RSBD8_HIP_I128 ? 128 : 80; }
+// bytes per key record, narrow or wide
+int record_bytes_of(int keytype) { return key_bytes_of(keytype) ? key_bytes_of(keytype) : wide_bytes_of(keytype); }
+
 inline size_t align_up(size_t v) { return (v + 255) & ~static_cast<size_t>(255); }
 
 struct IndirectWs {
@@ -205,7 +219,7 @@ int check_device_arrays(std::initializer_list<const void *> arrays, const char *
 
 int check_common(int keytype, unsigned flags, size_t count, size_t ws_bytes, int layout)
 {
-	if (!key_bytes_of(keytype)) return fail(RSBD8_HIP_EINVAL, "unknown key type");
+	if (!record_bytes_of(keytype)) return fail(RSBD8_HIP_EINVAL, "unknown key type");
 	if (!valid_flags(flags)) return fail(RSBD8_HIP_EINVAL, "unknown flag bits");
 	if (count > 0xFFFFFFFFull) return fail(RSBD8_HIP_EINVAL, "count above 2^32-1 is not supported");
 	if (ws_bytes < rsbd8_hip_workspace_size(keytype, count, layout)) return fail(RSBD8_HIP_EINVAL, "workspace too small");
@@ -265,11 +279,14 @@ extern "C" {
 
 int rsbd8_hip_decode(int keytype, unsigned mode, unsigned direction, int indirect, unsigned *flags)
 {
-	if (!key_bytes_of(keytype) || !flags) return fail(RSBD8_HIP_EINVAL, "bad key type or flags pointer");
+	if (!record_bytes_of(keytype) || !flags) return fail(RSBD8_HIP_EINVAL, "bad key type or flags pointer");
 	if (!(mode <= 7u || (mode >= 64u && mode <= 66u))) return fail(RSBD8_HIP_EINVAL, "not a sortingmode value");
 	if (direction > 3u) return fail(RSBD8_HIP_EINVAL, "not a sortingdirection value");
-	const bool issigned = keytype >= RSBD8_HIP_I8;   // std::is_signed_v: true for float and double
-	const bool isfloat = keytype >= RSBD8_HIP_F32;   // std::is_floating_point_v
+	// std::is_signed_v (true for the floating-point types) and std::is_floating_point_v; the 80-bit
+	// layouts are long double, the 128-bit integers __int128 and unsigned __int128
+	const bool wide = wide_bytes_of(keytype) != 0;
+	const bool isfloat = wide ? wide_bits_of(keytype) == 80 : keytype >= RSBD8_HIP_F32;
+	const bool issigned = wide ? (isfloat || keytype == RSBD8_HIP_I128) : keytype >= RSBD8_HIP_I8;
 	bool abs, sign, fltp;
 	if (mode >= 64u) {
 		abs = mode >= 65u && issigned;    // H:59779-59780
@@ -292,6 +309,7 @@ int rsbd8_hip_decode(int keytype, unsigned mode, unsigned direction, int indirec
 size_t rsbd8_hip_workspace_size(int keytype, size_t count, int layout)
 {
 	const int kb = key_bytes_of(keytype);
+	if (wide_bytes_of(keytype)) return wide_workspace_bytes(count);  // every layout
 	size_t s = sort_workspace_bytes(count);
 	if (layout == RSBD8_HIP_LAYOUT_INDIRECT) s = align_up(s) + 2 * align_up(count * (kb ? kb : 8)) + 2 * align_up(count * 4);
 	return s;
@@ -331,6 +349,20 @@ int rsbd8_hip_profile_end(double *ms, unsigned *launches, int nkinds)
 	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "profile_end");
 }
 
+// Keys wider than 64 bits (the 80-bit layouts and 128-bit integers): one wide sort on the workspace.
+static int run_wide(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes, WideJob job,
+	hipStream_t s, const char *what)
+{
+	job.rec_bytes = wide_bytes_of(keytype);
+	job.value_bits = wide_bits_of(keytype);
+	job.flags = flags;
+	job.n = count;
+	job.cfg = g_tile_cfg.load();
+	job.algo = g_algo.load();
+	const hipError_t e = wide_sort(job, workspace, workspace_bytes, s);
+	return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, what);
+}
+
 int rsbd8_hip_radixsortcopynoalloc(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
 	const void *input, void *output, void *buffer, rsbd8_hip_stream stream)
 {
@@ -340,6 +372,13 @@ int rsbd8_hip_radixsortcopynoalloc(int keytype, unsigned flags, size_t count, vo
 	if (!count) return RSBD8_HIP_OK;
 	if (!input || !output || !workspace) return fail(RSBD8_HIP_EINVAL, "null array");
 	if (input == output || (buffer && (buffer == input || buffer == output))) return fail(RSBD8_HIP_EINVAL, "arrays alias");
+	if (wide_bytes_of(keytype)) {  // no buffer needed: the records are gathered once
+		if ((r = check_device_arrays({workspace, input, output, buffer}, "radixsortcopynoalloc"))) return r;
+		WideJob j;
+		j.recs = input;
+		j.rec_out = output;
+		return run_wide(keytype, flags, count, workspace, workspace_bytes, j, static_cast<hipStream_t>(stream), "radixsortcopynoalloc");
+	}
 	if (!buffer && needs_buffer(kb, flags) && kb != 1) return fail(RSBD8_HIP_EINVAL, "buffer required");
 	if ((r = check_device_arrays({workspace, input, output, buffer}, "radixsortcopynoalloc"))) return r;
 	SortJob job;
@@ -392,6 +431,15 @@ int rsbd8_hip_radixsortnoalloc(int keytype, unsigned flags, size_t count, void *
 	if (buffer == input) return fail(RSBD8_HIP_EINVAL, "arrays alias");
 	if (!buffer && (needs_buffer(kb, flags) || movetobuffer)) return fail(RSBD8_HIP_EINVAL, "buffer required");
 	if ((r = check_device_arrays({workspace, input, buffer}, "radixsortnoalloc"))) return r;
+	if (const int wb = wide_bytes_of(keytype)) {  // gathered into buffer, then copied back unless wanted there
+		const hipStream_t s = static_cast<hipStream_t>(stream);
+		WideJob j;
+		j.recs = input;
+		j.rec_out = buffer;
+		if ((r = run_wide(keytype, flags, count, workspace, workspace_bytes, j, s, "radixsortnoalloc")) || movetobuffer) return r;
+		const hipError_t e = hipMemcpyAsync(input, buffer, count * static_cast<size_t>(wb), hipMemcpyDeviceToDevice, s);
+		return e == hipSuccess ? RSBD8_HIP_OK : fail_hip(e, "radixsortnoalloc copy back");
+	}
 	SortJob job;
 	job.key_bytes = kb; job.flags = flags; job.n = count;
 	job.in = input;
@@ -418,6 +466,14 @@ int rsbd8_hip_sort_pairs(int keytype, unsigned flags, size_t count, void *worksp
 	if ((r = check_device_arrays({workspace, keys_in, keys_out, keys_buffer, payload_in, payload_out, payload_buffer},
 		"sort_pairs")))
 		return r;
+	if (wide_bytes_of(keytype)) {
+		WideJob j;
+		j.recs = keys_in;
+		j.rec_out = keys_out;
+		j.pin = payload_in;
+		j.pout = payload_out;
+		return run_wide(keytype, flags, count, workspace, workspace_bytes, j, static_cast<hipStream_t>(stream), "sort_pairs");
+	}
 	SortJob job;
 	job.key_bytes = key_bytes_of(keytype); job.flags = flags; job.n = count;
 	job.in = keys_in; job.out = keys_out; job.buf = keys_buffer;
@@ -438,6 +494,7 @@ int rsbd8_hip_sort_indices(int keytype, unsigned flags, size_t count, void *work
 	if (idx_in == idx_out) return fail(RSBD8_HIP_EINVAL, "arrays alias");
 	if ((r = check_device_arrays({workspace, base, idx_in, idx_out}, "sort_indices"))) return r;
 	const int kb = key_bytes_of(keytype);
+	if (!kb) return fail(RSBD8_HIP_EINVAL, "sort_indices takes keys of at most 64 bits");
 	hipStream_t s = static_cast<hipStream_t>(stream);
 	IndirectWs w = carve_indirect(workspace, workspace_bytes, count, kb);
 	hipError_t e = gather_by_index(kb, count, base, stride, key_offset, idx_in, w.keys_a, s);
@@ -456,6 +513,13 @@ int rsbd8_hip_sort_indices(int keytype, unsigned flags, size_t count, void *work
 static int sort_pointers(int keytype, unsigned flags, size_t count, void *workspace, size_t workspace_bytes,
 	const void *const *input, void **output, ptrdiff_t ind1, int second, ptrdiff_t ind2, size_t index2, hipStream_t s)
 {
+	if (wide_bytes_of(keytype)) {
+		WideJob j;
+		j.ptrs = input;
+		j.off1 = ind1; j.second_level = second; j.off2 = ind2; j.index2 = index2;
+		j.ptr_out = output;
+		return run_wide(keytype, flags, count, workspace, workspace_bytes, j, s, "sort_pointers");
+	}
 	const int kb = key_bytes_of(keytype);
 	IndirectWs w = carve_indirect(workspace, workspace_bytes, count, kb);
 	hipError_t e = gather_by_pointer(kb, count, input, ind1, second, ind2, index2, w.keys_a, s);
@@ -508,7 +572,7 @@ int rsbd8_hip_radixsortnoalloc_indirect(int keytype, unsigned flags, size_t coun
 static int sort_allocating(int keytype, unsigned flags, size_t count, const void *input, void *output, bool inplace,
 	hipStream_t s)
 {
-	const int kb = key_bytes_of(keytype);
+	const int kb = record_bytes_of(keytype);
 	const size_t bytes = count * kb;
 	const size_t wsb = rsbd8_hip_workspace_size(keytype, count, RSBD8_HIP_LAYOUT_KEYS);
 	const bool dev_in = is_device_memory(input), dev_out = inplace ? dev_in : is_device_memory(output);
@@ -528,7 +592,14 @@ static int sort_allocating(int keytype, unsigned flags, size_t count, const void
 		r = fail_hip(e, "upload");
 		goto out;
 	}
-	{
+	if (wide_bytes_of(keytype)) {
+		// the records are gathered into buf (never onto their own source), then copied to dout
+		WideJob j;
+		j.recs = din;
+		j.rec_out = buf;
+		if ((r = run_wide(keytype, flags, count, ws, wsb, j, s, "sort"))) goto out;
+		if ((e = hipMemcpyAsync(dout, buf, bytes, hipMemcpyDeviceToDevice, s)) != hipSuccess) { r = fail_hip(e, "sort"); goto out; }
+	} else {
 		SortJob job;
 		job.key_bytes = kb; job.flags = flags; job.n = count;
 		job.in = din; job.out = dout; job.buf = buf;
@@ -548,13 +619,13 @@ out:
 
 int rsbd8_hip_radixsortcopy(int keytype, unsigned flags, size_t count, const void *input, void *output, rsbd8_hip_stream stream)
 {
-	if (!key_bytes_of(keytype) || !valid_flags(flags)) return fail(RSBD8_HIP_EINVAL, "bad key type or flags");
+	if (!record_bytes_of(keytype) || !valid_flags(flags)) return fail(RSBD8_HIP_EINVAL, "bad key type or flags");
 	if (count > 0xFFFFFFFFull) return fail(RSBD8_HIP_EINVAL, "count above 2^32-1 is not supported");
 	if (!input || !output) return fail(RSBD8_HIP_EINVAL, "null array");
 	if (input == output) return fail(RSBD8_HIP_EINVAL, "arrays alias");
 	if (count == 0) return RSBD8_HIP_OK;
 	if (count == 1) {  // the reference copies the single element (H:60525)
-		const hipError_t e = hipMemcpyAsync(output, input, key_bytes_of(keytype), hipMemcpyDefault, static_cast<hipStream_t>(stream));
+		const hipError_t e = hipMemcpyAsync(output, input, record_bytes_of(keytype), hipMemcpyDefault, static_cast<hipStream_t>(stream));
 		if (e != hipSuccess) return fail_hip(e, "copy");
 		const hipError_t e2 = hipStreamSynchronize(static_cast<hipStream_t>(stream));
 		return e2 == hipSuccess ? RSBD8_HIP_OK : fail_hip(e2, "synchronize");
@@ -564,7 +635,7 @@ int rsbd8_hip_radixsortcopy(int keytype, unsigned flags, size_t count, const voi
 
 int rsbd8_hip_radixsort(int keytype, unsigned flags, size_t count, void *input, rsbd8_hip_stream stream)
 {
-	if (!key_bytes_of(keytype) || !valid_flags(flags)) return fail(RSBD8_HIP_EINVAL, "bad key type or flags");
+	if (!record_bytes_of(keytype) || !valid_flags(flags)) return fail(RSBD8_HIP_EINVAL, "bad key type or flags");
 	if (count > 0xFFFFFFFFull) return fail(RSBD8_HIP_EINVAL, "count above 2^32-1 is not supported");
 	if (!input) return fail(RSBD8_HIP_EINVAL, "null array");
 	if (count <= 1) return RSBD8_HIP_OK;  // already sorted (H:60465)

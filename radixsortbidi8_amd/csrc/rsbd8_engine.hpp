@@ -119,6 +119,29 @@ hipError_t sort_launch(const SortJob &job, void *ws, size_t ws_bytes, hipStream_
 int default_cfg(int key_bytes, uint64_t n, bool payload, bool float_keys);
 int default_algo(int key_bytes, uint64_t n, bool keys_only, bool float_keys);
 
+// Keys wider than 64 bits (rsbd8_wide.hip): the 80-bit extended float in 10-, 12- or 16-byte
+// records and 128-bit integers, sorted by their 128-bit sort key through two stable pair sorts.
+struct WideJob {
+	int rec_bytes = 16;                  // bytes per key record: 10, 12 or 16
+	int value_bits = 80;                 // 80 (extended float layouts) or 128 (integers)
+	unsigned flags = 0;                  // decoded flags
+	uint64_t n = 0;
+	const void *recs = nullptr;          // the key records, or (when ptrs is set)
+	const void *const *ptrs = nullptr;   // pointers to them: first level at ptrs[i] + off1, second level
+	int64_t off1 = 0;                    // at *(ptrs[i] + off1) + off2 + index2 * rec_bytes
+	int second_level = 0;
+	int64_t off2 = 0;
+	uint64_t index2 = 0;
+	void *rec_out = nullptr;             // the records in sorted order (may be null)
+	void **ptr_out = nullptr;            // the pointers in sorted order (may be null)
+	const uint32_t *pin = nullptr;       // a payload moved along into pout (null: the source index)
+	uint32_t *pout = nullptr;            // (may be null)
+	int cfg = -1, algo = 0;              // plan of the two pair sorts
+};
+size_t wide_array_bytes(uint64_t n);
+size_t wide_workspace_bytes(uint64_t n);
+hipError_t wide_sort(const WideJob &job, void *ws, size_t ws_bytes, hipStream_t stream);
+
 // Gather keys through records or pointers into a dense key array (indirection layouts).
 hipError_t gather_by_index(int key_bytes, uint64_t n, const void *base, uint64_t stride, uint64_t offset,
 	const uint32_t *idx, void *keys_out, hipStream_t stream);

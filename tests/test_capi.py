@@ -27,11 +27,15 @@ def test_library_exports_every_declared_symbol():
 
 def test_decode_matches_oracle():
     import radixsortbidi8_amd as r
-    for kt in range(10):
+    for kt in range(15):  # the ten narrow key types and the five wide ones
         for mode in list(range(8)) + [64, 65, 66]:
             for d in range(4):
                 for ind in (False, True):
                     assert r.decode(kt, mode, d, ind) == oracle.decode(kt, mode, d, ind), (kt, mode, d, ind)
+    # the 80-bit layouts decode as long double (signed floating point), the 128-bit integers as such
+    for kt in (r.LD80, r.LD96, r.LD128):
+        assert r.decode(kt, 64, 0) == r.decode(r.F64, 64, 0) and r.decode(kt, 65, 0) == r.decode(r.F64, 65, 0)
+    assert r.decode(r.U128, 65, 0) == r.decode(r.U64, 65, 0) and r.decode(r.I128, 66, 0) == r.decode(r.I64, 66, 0)
 
 
 def test_decode_rejects_bad_values():
@@ -41,7 +45,7 @@ def test_decode_rejects_bad_values():
     with pytest.raises(r.Rsbd8Error):
         r.decode(3, 64, 4)
     with pytest.raises(r.Rsbd8Error):
-        r.decode(11, 64, 0)
+        r.decode(15, 64, 0)
 
 
 def test_workspace_size_grows_with_count():
