@@ -28,6 +28,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <new>
 #include <type_traits>
@@ -79,31 +80,102 @@ struct memberobjectgenerator {
 
 namespace helper {
 
+// The reference's record types for keys wider than 64 bits (H:1061-1804, H:2186-2404): the 80-bit
+// extended float as a 64-bit mantissa plus sign and exponent in the low 16 bits of signexponent
+// (10-, 12- and 16-byte layouts), and a 128-bit integer, low word first. The template arguments
+// are the isabsvalue, issignmode and isfltpmode that sorting them implies in the native modes.
+template<bool isabsvalue, bool issignmode, bool isfltpmode>
+struct longdoubletest128 {
+	std::uint_least64_t mantissa;
+	std::uint_least64_t signexponent;
+};
+#pragma pack(push, 1)
+template<bool isabsvalue, bool issignmode, bool isfltpmode>
+struct longdoubletest96 {
+	std::uint_least64_t mantissa;
+	std::uint_least32_t signexponent;
+};
+template<bool isabsvalue, bool issignmode, bool isfltpmode>
+struct longdoubletest80 {
+	std::uint_least64_t mantissa;
+	std::uint_least16_t signexponent;
+};
+#pragma pack(pop)
+template<bool isabsvalue, bool issignmode, bool isfltpmode>
+struct test128 {
+	std::uint_least64_t data[2];
+};
+
+// the wide key kind of T: its key type code and the flags its record type implies (0 code: not wide)
+template<typename T>
+struct widekind {
+	static constexpr int code{0};
+	static constexpr bool abs{false}, sign{false}, fltp{false};
+};
+template<template<bool, bool, bool> class R, int C, bool a, bool s, bool f>
+struct widekindof {
+	static constexpr int code{C};
+	static constexpr bool abs{a}, sign{s}, fltp{f};
+};
+template<bool a, bool s, bool f>
+struct widekind<longdoubletest80<a, s, f>> : widekindof<longdoubletest80, RSBD8_HIP_LD80, a, s, f> {};
+template<bool a, bool s, bool f>
+struct widekind<longdoubletest96<a, s, f>> : widekindof<longdoubletest96, RSBD8_HIP_LD96, a, s, f> {};
+template<bool a, bool s, bool f>
+struct widekind<longdoubletest128<a, s, f>> : widekindof<longdoubletest128, RSBD8_HIP_LD128, a, s, f> {};
+template<bool a, bool s, bool f>
+struct widekind<test128<a, s, f>> : widekindof<test128, RSBD8_HIP_U128, a, s, f> {};
+// x86 long double: the 80-bit extended float in 16 bytes (signed, floating point by the type traits)
+template<>
+struct widekind<long double> {
+	static constexpr int code{sizeof(long double) == 16 && 64 == std::numeric_limits<long double>::digits ? RSBD8_HIP_LD128 : 0};
+	static constexpr bool abs{false}, sign{false}, fltp{false};
+};
+#ifdef __SIZEOF_INT128__
+template<>
+struct widekind<unsigned __int128> {
+	static constexpr int code{RSBD8_HIP_U128};
+	static constexpr bool abs{false}, sign{false}, fltp{false};
+};
+template<>
+struct widekind<__int128> {
+	static constexpr int code{RSBD8_HIP_I128};
+	static constexpr bool abs{false}, sign{false}, fltp{false};
+};
+#endif
+
+// std::is_signed_v, also for __int128 where the library leaves it out (strict standard modes)
+template<typename T>
+constexpr bool issigned{std::is_signed_v<T> || widekind<T>::code == RSBD8_HIP_I128};
+
 // unsigned carrier and key type code of T (enums and classes sort by their unsigned bit pattern)
 template<typename T>
 constexpr int keytypeof()
 {
 	using S = std::remove_cv_t<T>;
-	if constexpr (std::is_floating_point_v<S>) {
-		static_assert(sizeof(S) == 4 || sizeof(S) == 8, "only 32- and 64-bit floating-point keys are supported");
+	if constexpr (widekind<S>::code != 0) {
+		return widekind<S>::code;
+	} else if constexpr (std::is_floating_point_v<S>) {
+		static_assert(sizeof(S) == 4 || sizeof(S) == 8, "floating-point keys: 32 or 64 bits, or the x86 80-bit long double");
 		return sizeof(S) == 4 ? RSBD8_HIP_F32 : RSBD8_HIP_F64;
 	} else if constexpr (std::is_signed_v<S>) {
 		return sizeof(S) == 1 ? RSBD8_HIP_I8 : sizeof(S) == 2 ? RSBD8_HIP_I16 : sizeof(S) == 4 ? RSBD8_HIP_I32 : RSBD8_HIP_I64;
 	} else {
-		static_assert(sizeof(S) == 1 || sizeof(S) == 2 || sizeof(S) == 4 || sizeof(S) == 8, "key size must be 1, 2, 4 or 8 bytes");
+		static_assert(sizeof(S) == 1 || sizeof(S) == 2 || sizeof(S) == 4 || sizeof(S) == 8,
+			"key size must be 1, 2, 4 or 8 bytes, or one of the 80-bit and 128-bit types");
 		return sizeof(S) == 1 ? RSBD8_HIP_U8 : sizeof(S) == 2 ? RSBD8_HIP_U16 : sizeof(S) == 4 ? RSBD8_HIP_U32 : RSBD8_HIP_U64;
 	}
 }
 
-// compile-time mode decoding for the standard key types (H:59778-59904)
+// compile-time mode decoding (H:59778-59904); the wide record types add the flags they carry
 template<sortingmode mode, typename T>
-constexpr bool isabsvalue{(sortingmode::nativeabs <= mode && std::is_signed_v<T>) ||
+constexpr bool isabsvalue{(sortingmode::nativeabs <= mode && issigned<T>) || widekind<T>::abs ||
 	(sortingmode::native > mode && static_cast<bool>(1u & static_cast<unsigned char>(mode)))};
 template<sortingmode mode, typename T>
-constexpr bool issignmode{(sortingmode::native <= mode && sortingmode::nativeabs >= mode && std::is_signed_v<T>) ||
+constexpr bool issignmode{(sortingmode::native <= mode && sortingmode::nativeabs >= mode && issigned<T>) || widekind<T>::sign ||
 	(sortingmode::native > mode && static_cast<bool>(1u << 1 & static_cast<unsigned char>(mode)))};
 template<sortingmode mode, typename T>
-constexpr bool isfltpmode{(sortingmode::native <= mode && std::is_floating_point_v<T>) ||
+constexpr bool isfltpmode{(sortingmode::native <= mode && std::is_floating_point_v<T>) || widekind<T>::fltp ||
 	(sortingmode::native > mode && static_cast<bool>(1u << 2 & static_cast<unsigned char>(mode)))};
 template<sortingdirection direction>
 constexpr bool isdescsort{static_cast<bool>(1u & static_cast<unsigned char>(direction))};
@@ -123,9 +195,16 @@ constexpr unsigned flags()
 		(isdescsort<direction> ? RSBD8_HIP_DESC : 0u) | (rev ? RSBD8_HIP_REV : 0u);
 }
 
+// the bytes of a key as the C ABI takes them: an unsigned integer of the key's size, or the record
+// bytes of a wide key
+template<std::size_t N>
+struct widebytes {
+	unsigned char b[N];
+};
 template<typename T>
-using carrier = std::conditional_t<sizeof(T) == 1, std::uint8_t, std::conditional_t<sizeof(T) == 2, std::uint16_t,
-	std::conditional_t<sizeof(T) == 4, std::uint32_t, std::uint64_t>>>;
+using carrier = std::conditional_t<(sizeof(T) > 8), widebytes<sizeof(T)>,
+	std::conditional_t<sizeof(T) == 1, std::uint8_t, std::conditional_t<sizeof(T) == 2, std::uint16_t,
+	std::conditional_t<sizeof(T) == 4, std::uint32_t, std::uint64_t>>>>;
 
 inline bool isdevice(void const *p)
 {

@@ -5,11 +5,15 @@
 //   - every noalloc form, with and without indirection, on device arrays and allocatearray
 //     workspaces (T**, member object, getter, type+offset; copy, in place, to buffer);
 //   - indexed first-level and second-level indirection (one and two index arguments), getters
-//     returning a pointer or a reference, with and without isindexed2, in all four directions.
+//     returning a pointer or a reference, with and without isindexed2, in all four directions;
+//   - the wide keys: main.cpp:21-178 for the three 80-bit layouts (directly, through pointers,
+//     and through the noalloc forms on device arrays), x86 long double and the 128-bit integers.
 // Expected orders come from std::stable_sort over the keys read on the host (ties in source order,
 // or reversed for the rev directions). Exits non-zero on the first mismatch and prints which check
 // failed.
 #include <algorithm>
+#include <cmath>
+#include <limits>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -114,6 +118,49 @@ std::vector<std::size_t> indices(std::vector<V *> const &ptrs, V *base)
 	std::vector<std::size_t> r(ptrs.size());
 	for (std::size_t i{}; i != ptrs.size(); ++i) r[i] = static_cast<std::size_t>(ptrs[i] - base);
 	return r;
+}
+
+// multiplatform/main.cpp:21-178 for one 80-bit layout R: -inf, +inf, the machine QNaN, max normal,
+// min normal, max and min subnormal (arbitrary padding above the 16-bit sign/exponent), sorted
+// ascending and dscrevorder, directly and through pointers; then the same through the noalloc
+// forms on device arrays
+template<typename R, typename SE>
+void widekat(SE const (&se)[7])
+{
+	std::uint64_t const mant[7]{0, 0, 0x8000000000000000u, 0xFFFFFFFFFFFFFFFFu, 0, 0xFFFFFFFFFFFFFFFFu, 1u};
+	R in[7], out[7];
+	for (int i = 0; i < 7; ++i) {
+		in[i].mantissa = mant[i];
+		in[i].signexponent = se[i];
+	}
+	int const asc[7]{2, 0, 6, 5, 4, 3, 1}, dsc[7]{1, 3, 4, 5, 6, 0, 2};
+	auto same = [&](R const *o, int const *w) {
+		for (int i = 0; i < 7; ++i)
+			if (std::memcmp(&o[i], &in[w[i]], sizeof(R))) return false;
+		return true;
+	};
+	CHECK(rsbd8::radixsortcopy(7, in, out) && same(out, asc));
+	CHECK(rsbd8::radixsortcopy<rsbd8::sortingdirection::dscrevorder>(7, in, out) && same(out, dsc));
+	R *pin[7], *pout[7];
+	for (int i = 0; i < 7; ++i) pin[i] = in + i;
+	auto psame = [&](int const *w) {
+		for (int i = 0; i < 7; ++i)
+			if (pout[i] != in + w[i]) return false;
+		return true;
+	};
+	CHECK(rsbd8::radixsortcopy(7, pin, pout) && psame(asc));
+	CHECK(rsbd8::radixsortcopy<rsbd8::sortingdirection::dscrevorder>(7, pin, pout) && psame(dsc));
+	auto [alloc, size] = rsbd8::allocatearray<false, true, true, R, false>(7);
+	CHECK(alloc.pbuffer != nullptr);
+	rsbd8::buffermemorywrapper guard{alloc.pbuffer, size};
+	R *din{todevice(std::vector<R>(in, in + 7))}, *dout{};
+	CHECK(hipMalloc(reinterpret_cast<void **>(&dout), sizeof(in)) == hipSuccess);
+	rsbd8::radixsortcopynoalloc(7, 0u, nullptr, alloc.pzeroedindices, din, dout, alloc.pbuffer);
+	CHECK(rsbd8::lasterror() == 0 && same(tohost(dout, 7).data(), asc));
+	rsbd8::radixsortnoalloc<rsbd8::sortingdirection::dscrevorder>(7, 0u, nullptr, alloc.pzeroedindices, din, alloc.pbuffer, true);
+	CHECK(rsbd8::lasterror() == 0 && same(tohost(alloc.pbuffer, 7).data(), dsc));
+	static_cast<void>(hipFree(din));
+	static_cast<void>(hipFree(dout));
 }
 
 // every indirection form of one direction over n records, host and device arrays
@@ -406,6 +453,40 @@ int main()
 		static_cast<void>(hipFree(din));
 		static_cast<void>(hipFree(dout));
 		static_cast<void>(hipFree(dbuf));
+	}
+	{ // keys wider than 64 bits: the reference's 80-bit known answers, x86 long double, 128-bit integers
+		std::uint16_t const se80[7]{0xFFFFu, 0x7FFFu, 0xFFFFu, 0x7FFEu, 1u, 0, 0};
+		std::uint32_t const se96[7]{0xABABFFFFu, 0xD0017FFFu, 0xEEEEFFFFu, 0x01017FFEu, 0xFFF80001u, 0xC7C80000u, 0xB3710000u};
+		std::uint64_t const se128[7]{0xBBBBAAAAABABFFFFu, 0x22221111D0017FFFu, 0x44443333EEEEFFFFu, 0x6666555501017FFEu,
+			0x88887777FFF80001u, 0xCCCC9999C7C80000u, 0xFFFFDDDDB3710000u};
+		widekat<rsbd8::helper::longdoubletest80<false, true, true>>(se80);
+		widekat<rsbd8::helper::longdoubletest96<false, true, true>>(se96);
+		widekat<rsbd8::helper::longdoubletest128<false, true, true>>(se128);
+		static_assert(10 == sizeof(rsbd8::helper::longdoubletest80<false, true, true>) &&
+			12 == sizeof(rsbd8::helper::longdoubletest96<false, true, true>));
+		if constexpr (sizeof(long double) == 16 && std::numeric_limits<long double>::digits == 64) {
+			long double ld[6]{-1.5L, 0.0L, -0.0L, 2.0L, 1e300L, -1e-4000L};
+			long double lo[6];
+			CHECK(rsbd8::radixsortcopy(6, ld, lo));
+			// -1.5 < -1e-4000 < 0.0 == -0.0 (ties in source order) < 2 < 1e300
+			CHECK(lo[0] == -1.5L && lo[1] == -1e-4000L && !std::signbit(lo[2]) && std::signbit(lo[3]) && lo[3] == 0.0L &&
+				lo[4] == 2.0L && lo[5] == 1e300L);
+			CHECK(rsbd8::radixsortcopy<rsbd8::sortingdirection::dscrevorder>(6, ld, lo));
+			CHECK(lo[0] == 1e300L && lo[1] == 2.0L && std::signbit(lo[2]) && !std::signbit(lo[3]) && lo[4] == -1e-4000L &&
+				lo[5] == -1.5L);
+			CHECK((rsbd8::radixsort<rsbd8::sortingdirection::ascfwdorder, rsbd8::sortingmode::nativeabs>(6, ld)));
+			CHECK(ld[0] == 0.0L && ld[1] == 0.0L && ld[2] == -1e-4000L && ld[3] == -1.5L && ld[4] == 2.0L && ld[5] == 1e300L);
+		}
+#ifdef __SIZEOF_INT128__
+		__int128 const big{static_cast<__int128>(1) << 100};
+		__int128 iv[6]{-3, 5, 0, -big, big * 1024, -1};
+		__int128 io[6];
+		CHECK(rsbd8::radixsortcopy(6, iv, io));
+		CHECK(io[0] == -big && io[1] == -3 && io[2] == -1 && io[3] == 0 && io[4] == 5 && io[5] == big * 1024);
+		unsigned __int128 uv[4]{static_cast<unsigned __int128>(-1), 7, static_cast<unsigned __int128>(big), 0};
+		CHECK((rsbd8::radixsort<rsbd8::sortingdirection::dscfwdorder>(4, uv)));
+		CHECK(uv[0] == static_cast<unsigned __int128>(-1) && uv[1] == static_cast<unsigned __int128>(big) && uv[2] == 7 && uv[3] == 0);
+#endif
 	}
 	for (std::size_t n : {std::size_t{1000}, std::size_t{70001}}) {
 		indirection_forms<rsbd8::sortingdirection::ascfwdorder>(n);
