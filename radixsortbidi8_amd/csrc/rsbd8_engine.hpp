@@ -54,6 +54,7 @@ struct Control {
 	uint32_t hyb_scan_counter;
 	uint32_t hyb_over;     // some bucket exceeds the LDS capacity: use the LSD plan
 	uint32_t hyb_queue;    // buckets queued for the LDS LSD passes (long runs of one third digit)
+	uint32_t joint_max;    // largest bucket of the top-16-bit histogram (the exact MSD plan test)
 };
 
 constexpr size_t kCtrlBytes = 4096;
@@ -62,9 +63,12 @@ constexpr int kScanChunk = 64;  // tiles per block of the chained scan over the 
 constexpr int kTagSlots = 16;   // look-back tag slots per sort: LSD passes 0..7, the MSD plan's scan 8
 constexpr uint32_t kHybBuckets = kRadix * kRadix;  // buckets after the two MSD partitions
 // tables of the two-level MSD plan after the tile counts: first tile of every top-digit segment
-// (257 words), segment starts and lengths (256 + 256), bucket starts and lengths (2 x 65536), and
-// the queue of buckets for the LDS LSD passes (65536)
-constexpr size_t kSegTableWords = 257 + 256 + 256 + 3 * kHybBuckets;
+// (257 words), segment starts and lengths (256 + 256), bucket starts (relative to their segment)
+// and lengths (2 x 65536), the queue of buckets for the LDS LSD passes (65536), the bucket cursors
+// of the second partition (65536), the overflow counts of the top-16-bit histogram (65536, zero
+// between sorts), and the top-digit counts of the first partition's key ranges, then its cursors
+// (8 x 256)
+constexpr size_t kSegTableWords = 257 + 256 + 256 + 5 * kHybBuckets + 8 * 256;
 static_assert(sizeof(Control) <= kCtrlBytes, "control block too large");
 
 // Workspace layout: [control][histograms][bucket starts][MSD tables][tile records ...]. Tile record

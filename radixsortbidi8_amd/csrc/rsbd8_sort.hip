@@ -41,15 +41,19 @@ inline uint32_t *counts_of(void *ws)
 {
 	return reinterpret_cast<uint32_t *>(static_cast<char *>(ws) + kHeaderBytes + kRadix * sizeof(uint64_t));
 }
-inline uint64_t max_tiles(uint64_t n) { return (n + kMinTile - 1) / kMinTile + 1; }
 
 __device__ __forceinline__ uint32_t pass_tag(uint32_t epoch, int slot)
 {
 	return ((epoch * kTagSlots + static_cast<uint32_t>(slot)) % 0x3FFFFFFFu) + 1u;
 }
 
+constexpr uint32_t kJointBins = 65536;    // bins of the top-16-bit histogram
+constexpr uint32_t kJointFlush = 32768;   // keys counted between two overflow sweeps of that histogram
+constexpr uint32_t kRanges = 8;           // key ranges of the MSD plan's first partition (one per XCD)
+
 // The MSD plan's tables (see kSegTableWords): tile_start[257], seg_start[256], seg_len[256],
-// bucket_start[65536], bucket_len[65536], queue[65536].
+// bucket_start[65536] (relative to the bucket's segment), bucket_len[65536], queue[65536],
+// cursor[65536], ovf[65536], rcnt[8 * 256].
 struct SegTables {
 	uint32_t *tile_start;
 	uint32_t *seg_start;
@@ -57,11 +61,15 @@ struct SegTables {
 	uint32_t *bucket_start;
 	uint32_t *bucket_len;
 	uint32_t *queue;
+	uint32_t *cursor;
+	uint32_t *ovf;
+	uint32_t *rcnt;
 };
 __host__ __device__ inline SegTables seg_tables(uint32_t *base)
 {
-	return SegTables{base, base + 257, base + 257 + 256, base + 257 + 512, base + 257 + 512 + kHybBuckets,
-		base + 257 + 512 + 2 * kHybBuckets};
+	uint32_t *b = base + 257 + 512;
+	return SegTables{base, base + 257, base + 257 + 256, b, b + kHybBuckets, b + 2 * kHybBuckets, b + 3 * kHybBuckets,
+		b + 4 * kHybBuckets, b + 5 * kHybBuckets};
 }
 
 // 0: a real sort is needed; 1: the output is the input as it is; 2: the output is the input
@@ -246,6 +254,7 @@ struct PlanArgs {
 	uint32_t tile2;     // tile size of the segmented second pass
 	uint32_t *segtab;   // the MSD plan's tables
 	int partial;        // the histogram read counted only the top two digits (see k_hist_tiles)
+	int joint;          // ... and the exact bucket sizes (k_hist_joint, k_joint_reduce)
 	const uint32_t *gate;  // non-null: plan only when it reads nonzero (the full plan after a fallback)
 };
 
@@ -287,6 +296,16 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 		if (t == kRadix - 1) T.tile_start[kRadix] = before + incl;
 		T.seg_start[t] = top_start;
 		T.seg_len[t] = top_count;
+		if (a.joint) {
+			// the first partition's cursors: range x's keys of top digit t follow those of the
+			// ranges before it (a counting table turned into its own prefix, one column per thread)
+			uint32_t run = top_start;
+			for (uint32_t x = 0; x < kRanges; ++x) {
+				const uint32_t c = T.rcnt[x * kRadix + t];
+				T.rcnt[x * kRadix + t] = run;
+				run += c;
+			}
+		}
 	}
 	if (t != 0) return;
 	PassPlan none{};
@@ -345,12 +364,15 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 		src = X[slot]; psrc = PX[slot]; rev = 0; iota = 0;
 		++j;
 	}
-	// The two-level MSD plan: both top digits must vary, and the largest bucket of the pair, as
-	// estimated from the two histograms, must fit the per-bucket LDS sort with room to spare (the
-	// exact bucket sizes are checked after the second count; larger ones fall back to this LSD plan).
+	// The two-level MSD plan: both top digits must vary, and the largest bucket of the pair must fit
+	// the per-bucket LDS sort: exactly, from the top-16-bit histogram, or else as estimated from the
+	// two digit histograms, with room to spare (the exact bucket sizes are then checked after the
+	// second count; larger ones fall back to this LSD plan).
 	const int P = a.passes;
 	bool msd = a.hybrid && P >= 4 && a.other && !a.payload && !s_uniform[P - 1] && !s_uniform[P - 2];
-	if (msd) {
+	if (msd && a.joint) {
+		msd = ctrl->joint_max <= a.hyb_cap;
+	} else if (msd) {
 		// (the same margin as the host's capacity choice: six standard deviations of a Poisson count)
 		const float est = static_cast<float>(static_cast<double>(s_max[P - 1]) * s_max[P - 2] / static_cast<double>(a.n));
 		msd = est + 6.f * sqrtf(est) + 64.f <= static_cast<float>(a.hyb_cap);
@@ -633,6 +655,116 @@ __global__ __launch_bounds__(THREADS) void k_hist_tiles(const K *__restrict__ in
 	}
 }
 
+// The MSD plan's read when the key transform is injective: the histogram of the top 16 bits of
+// every transformed key, which gives the exact sizes of the 65536 buckets the two partitions make.
+// The keys are split into kRanges ranges of whole tiles of the first partition (range x: the tiles
+// x * per .. x * per + per - 1 in logical order), and the blocks into as many groups, so that the
+// reduction also yields every range's top-digit counts: the first partition's blocks take the
+// output ranges of a range's tiles from that range's cursors. One block per CU holds the 65536
+// counters as 16-bit halves of LDS words (128 KB). The block counts its keys kJointFlush at a time
+// and then moves 32768 out of every counter that reached it into ovf (global; only a few dominant
+// buckets make it happen), so no half ever exceeds 0xFFFF. The block's counts go to its own
+// 65536-word slice of part; k_joint_reduce adds the slices.
+
+template <typename K, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_hist_joint(const K *__restrict__ in, uint64_t n, KeyXform xf,
+	uint64_t range_keys, int reverse, const Control *ctrl, uint32_t *__restrict__ part, uint32_t *__restrict__ ovf)
+{
+	if (presort_result(ctrl->check, reverse)) return;  // the input is already in order
+	constexpr int P = sizeof(K);
+	static_assert(P >= 4, "the two-level MSD plan sorts 4- and 8-byte keys");
+	__shared__ uint32_t s_j[kJointBins / 2];  // bin b in the half (b & 1) of word b >> 1
+	const uint32_t t = threadIdx.x;
+	for (uint32_t i = t; i < kJointBins / 2; i += THREADS) s_j[i] = 0;
+	// this block's share of its range (logical positions), as a physical range
+	const uint32_t per_range = gridDim.x / kRanges;
+	const uint32_t x = blockIdx.x / per_range, j = blockIdx.x % per_range;
+	const uint64_t r0 = min(n, static_cast<uint64_t>(x) * range_keys), r1 = min(n, r0 + range_keys);
+	const uint64_t share = (r1 - r0 + per_range - 1) / per_range;
+	const uint64_t b0 = min(r1, r0 + j * share), b1 = min(r1, b0 + share);
+	const uint64_t lo = reverse ? n - b1 : b0, hi = reverse ? n - b0 : b1;
+	__syncthreads();
+	for (uint64_t c = lo; c < hi; c += kJointFlush) {
+		const uint32_t len = static_cast<uint32_t>(min<uint64_t>(kJointFlush, hi - c));
+		for_each_key<K, THREADS>(in, c, len, [&](K key) {
+			const uint32_t bin = static_cast<uint32_t>(sort_key<K>(key, xf) >> (8 * P - 16));
+			// a wave whose keys share one bin adds them with one atomic (sorted input), else one per lane
+			const uint64_t active = __ballot(1);
+			const uint32_t bin0 = __builtin_amdgcn_readfirstlane(bin);
+			const bool same = __ballot(bin == bin0) == active;
+			if (!same || lanes_below(active) == 0)
+				atomicAdd(&s_j[bin >> 1], (same ? static_cast<uint32_t>(__popcll(active)) : 1u) << ((bin & 1u) * 16u));
+		});
+		__syncthreads();
+		// every half is below 0x8000 after the sweep, and at most kJointFlush keys are counted
+		// before the next one
+		for (uint32_t i = t; i < kJointBins / 2; i += THREADS) {
+			const uint32_t v = s_j[i];
+			if (v & 0x80008000u) {
+				if (v & 0x8000u) atomicAdd(&ovf[2 * i], 0x8000u);
+				if (v & 0x80000000u) atomicAdd(&ovf[2 * i + 1], 0x8000u);
+				s_j[i] = v & 0x7FFF7FFFu;
+			}
+		}
+		__syncthreads();
+	}
+	uint2 *slice = reinterpret_cast<uint2 *>(part + static_cast<uint64_t>(blockIdx.x) * kJointBins);
+	for (uint32_t i = t; i < kJointBins / 2; i += THREADS) {
+		const uint32_t v = s_j[i];
+		slice[i] = make_uint2(v & 0xFFFFu, v >> 16);
+	}
+}
+
+// One block per top digit, one thread per second digit: the bucket's count (the slices of
+// k_hist_joint and the overflow, which is left zeroed), its start relative to its segment (also the
+// second partition's cursor), the histograms of the top two digits, every range's count of the top
+// digit (rcnt[x * 256 + top digit]) and the largest bucket.
+template <int P>
+__global__ __launch_bounds__(256) void k_joint_reduce(Control *ctrl, int reverse, const uint32_t *__restrict__ part,
+	uint32_t slices, SegTables T, uint32_t *__restrict__ hist)
+{
+	if (presort_result(ctrl->check, reverse)) return;
+	__shared__ uint32_t s_wave[4];
+	__shared__ uint32_t s_r[kRanges][4];
+	const uint32_t sg = blockIdx.x, d = threadIdx.x, lane = d & 63u, w = d >> 6;
+	const uint32_t b = sg * kRadix + d;
+	uint32_t v = T.ovf[b];
+	if (v) T.ovf[b] = 0;
+	const uint32_t per_range = slices / kRanges;
+	uint32_t rv[kRanges] = {};
+	// slice k of every range per step: eight independent loads, four steps in flight
+#pragma unroll 4
+	for (uint32_t k = 0; k < per_range; ++k) {
+#pragma unroll
+		for (uint32_t x = 0; x < kRanges; ++x) rv[x] += part[static_cast<uint64_t>(x * per_range + k) * kJointBins + b];
+	}
+#pragma unroll
+	for (uint32_t x = 0; x < kRanges; ++x) v += rv[x];
+	T.bucket_len[b] = v;
+	const uint32_t incl = wave_inclusive_sum(v);
+	uint32_t m = v;
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), o, 64)));
+#pragma unroll
+	for (uint32_t x = 0; x < kRanges; ++x) {
+		uint32_t r = rv[x];
+#pragma unroll
+		for (int o = 32; o > 0; o >>= 1) r += static_cast<uint32_t>(__shfl_xor(static_cast<int>(r), o, 64));
+		if (lane == 0) s_r[x][w] = r;
+	}
+	if (lane == 63) s_wave[w] = incl;
+	if (lane == 0) atomicMax(&ctrl->joint_max, m);
+	__syncthreads();
+	uint32_t before = 0;
+	for (uint32_t u = 0; u < w; ++u) before += s_wave[u];
+	const uint32_t rel = before + incl - v;
+	T.bucket_start[b] = rel;
+	T.cursor[b] = rel;
+	if (d == kRadix - 1) hist[(P - 1) * kRadix + sg] = rel + v;
+	if (d < kRanges) T.rcnt[d * kRadix + sg] = s_r[d][0] + s_r[d][1] + s_r[d][2] + s_r[d][3];
+	if (v) atomicAdd(&hist[(P - 2) * kRadix + d], v);
+}
+
 // per-tile counts of one digit (passes after the first, or when the histogram read did not count)
 template <typename K, int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_count(const Control *ctrl, int pass, KeyXform xf, uint64_t n,
@@ -718,7 +850,7 @@ struct SegMap {
 
 // first key and key count of tile tl of a segment map (s_ts: tile_start staged in LDS)
 __device__ __forceinline__ void seg_tile(const uint32_t *s_ts, const SegMap &m, uint32_t TILE, uint32_t tl, uint64_t &base,
-	uint32_t &valid)
+	uint32_t &valid, uint32_t &segment)
 {
 	// the last segment whose first tile is <= tl (empty segments share their successor's first tile)
 	uint32_t sg = 0;
@@ -729,6 +861,7 @@ __device__ __forceinline__ void seg_tile(const uint32_t *s_ts, const SegMap &m, 
 	base = static_cast<uint64_t>(m.seg_start[sg]) + static_cast<uint64_t>(k) * TILE;
 	const uint32_t left = m.seg_len[sg] - k * static_cast<uint32_t>(TILE);
 	valid = left < static_cast<uint32_t>(TILE) ? left : static_cast<uint32_t>(TILE);
+	segment = sg;
 }
 
 // pp: the pass (inactive: exit); shift: its digit. With SEG, the tiles are those of the segment map
@@ -736,10 +869,16 @@ __device__ __forceinline__ void seg_tile(const uint32_t *s_ts, const SegMap &m, 
 // keys of a digit are not kept in order within the tile (LDS atomics instead of block ranking):
 // only for the MSD plan's partitions when the key transform is injective, where the per-bucket
 // sort that follows orders every bucket by the whole sort key and equal sort keys are equal keys.
-template <typename K, bool PAY, int THREADS, int KPT, bool SEG = false, bool GROUP = false>
+// With CUR (GROUP only) the tiles take their output ranges from cursors (one atomic per digit and
+// tile) instead of precomputed tile offsets, and fill a digit's run in any order, which is again
+// invisible after the per-bucket sort: with SEG the cursors of the tile's segment (positions
+// relative to it), else those of the tile's key range (range x holds the tiles x * per ..
+// x * per + per - 1; absolute positions).
+template <typename K, bool PAY, int THREADS, int KPT, bool SEG = false, bool GROUP = false, bool CUR = false>
 __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32_t shift, KeyXform xf, uint64_t n,
-	uint32_t num_tiles, const uint32_t *__restrict__ offsets, SegMap seg)
+	uint32_t num_tiles, const uint32_t *__restrict__ offsets, SegMap seg, uint32_t *__restrict__ cursor)
 {
+	static_assert(!CUR || GROUP, "cursors are for the grouping passes");
 	constexpr int TILE = THREADS * KPT;
 	constexpr int NW = THREADS / 64;
 	const PassPlan pp = *ppp;
@@ -766,10 +905,11 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 		__syncthreads();
 		real_tiles = s_ts[kRadix];
 	}
-	auto geom = [&](uint32_t tl, uint64_t &base, uint32_t &valid) {
+	auto geom = [&](uint32_t tl, uint64_t &base, uint32_t &valid, uint32_t &sgm) {
 		if constexpr (SEG) {
-			seg_tile(s_ts, seg, TILE, tl, base, valid);
+			seg_tile(s_ts, seg, TILE, tl, base, valid, sgm);
 		} else {
+			sgm = 0;
 			base = static_cast<uint64_t>(tl) * TILE;
 			valid = (n - base) < static_cast<uint64_t>(TILE) ? static_cast<uint32_t>(n - base) : static_cast<uint32_t>(TILE);
 		}
@@ -796,9 +936,9 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 	uint32_t k = blockIdx.x >> 3;
 	uint32_t tile = x * per + k;
 	uint64_t base = 0;
-	uint32_t valid = 0;
+	uint32_t valid = 0, sgm = 0;
 	if (k < per && tile < real_tiles) {
-		geom(tile, base, valid);
+		geom(tile, base, valid, sgm);
 		load_tile(base, valid);
 	}
 	while (k < per && tile < real_tiles) {
@@ -813,7 +953,20 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 			(void)total;
 			(void)s_peer;
 			block_group<THREADS, KPT>(dig, valid, pos, s_whist, s_digoff, s_wsum);
-			if (t < kRadix) s_glob[t] = offsets[static_cast<uint64_t>(tile) * kCountStride + t] - s_digoff[t];
+			if constexpr (CUR) {
+				if (t < kRadix) {
+					const uint32_t c = (t + 1 < kRadix ? s_digoff[t + 1] : valid) - s_digoff[t];
+					if constexpr (SEG) {
+						const uint32_t at = c ? atomicAdd(&cursor[sgm * kRadix + t], c) : 0u;
+						s_glob[t] = seg.seg_start[sgm] + at - s_digoff[t];
+					} else {
+						const uint32_t at = c ? atomicAdd(&cursor[x * kRadix + t], c) : 0u;
+						s_glob[t] = at - s_digoff[t];
+					}
+				}
+			} else {
+				if (t < kRadix) s_glob[t] = offsets[static_cast<uint64_t>(tile) * kCountStride + t] - s_digoff[t];
+			}
 #pragma unroll
 			for (int j = 0; j < KPT; ++j)
 				if (pos[j] != 0xFFFFFFFFu) s_keys[pos[j]] = keys[j];
@@ -836,9 +989,9 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 		// prefetch the next tile of this block
 		const uint32_t k2 = k + step, tile2 = x * per + k2;
 		uint64_t base2 = 0;
-		uint32_t valid2 = 0;
+		uint32_t valid2 = 0, sgm2 = 0;
 		if (k2 < per && tile2 < real_tiles) {
-			geom(tile2, base2, valid2);
+			geom(tile2, base2, valid2, sgm2);
 			load_tile(base2, valid2);
 		}
 		__syncthreads();
@@ -861,6 +1014,7 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 		tile = tile2;
 		base = base2;
 		valid = valid2;
+		sgm = sgm2;
 	}
 }
 
@@ -886,8 +1040,8 @@ __global__ __launch_bounds__(THREADS) void k_hyb_count(const PassPlan *ppp, uint
 		for (int i = threadIdx.x; i < kRadix * C; i += THREADS) s_t[i] = 0;
 		__syncthreads();
 		uint64_t base;
-		uint32_t len;
-		seg_tile(s_ts, seg, tile_keys, tile, base, len);
+		uint32_t len, sg;
+		seg_tile(s_ts, seg, tile_keys, tile, base, len, sg);
 		for_each_key<K, THREADS>(src, base, len, [&](K x) { wave_count<C>(s_t, digit_of<K>(x, xf, shift)); });
 		__syncthreads();
 		for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kCountStride + i] = copies_sum<C>(s_t, i);
@@ -918,7 +1072,7 @@ __global__ __launch_bounds__(256) void k_hyb_segscan(Control *ctrl, SegTables T,
 	uint32_t before = 0;
 	for (uint32_t v = 0; v < w; ++v) before += s_wave[v];
 	uint32_t run = T.seg_start[sg] + before + incl - tot;
-	T.bucket_start[sg * kRadix + d] = run;
+	T.bucket_start[sg * kRadix + d] = before + incl - tot;
 	T.bucket_len[sg * kRadix + d] = tot;
 	if (tot > cap) atomicOr(&ctrl->hyb_over, 1u);
 	for (uint32_t tl = t0; tl < t1; tl += 8) {
@@ -944,19 +1098,21 @@ __global__ void k_hyb_check(Control *ctrl)
 	ctrl->need_full = 1;
 }
 
-// One block per bucket (all keys share the top two digits, so the transformed key shifted left by
-// 16 keeps every digit below them). Each key becomes the word (transformed key << 16) | l, with l
+// The per-bucket sort, one block per bucket (a persistent form that loads the next bucket while
+// sorting the current one measured slower: its registers halve the resident blocks). All keys of a
+// bucket share the top two digits, so the transformed key shifted left by 16 keeps every digit
+// below them. Each key becomes the word (transformed key << 16) | l, with l
 // its position in the bucket: distinct words whose order is the stable order. The words are
 // partitioned by their top PB bits with LDS atomics (in any order), then every word's final
 // position inside its run of equal top bits is the number of smaller words in the run. The raw
 // keys reach their positions through their owners. A bucket with a run longer than kLocalRunMax is
-// queued for k_local_lsd instead. The bucket is read whole before it is written, so source and
+// queued for k_local_lsd instead. A bucket is read whole before it is written, so source and
 // destination may coincide.
 constexpr uint32_t kLocalRunMax = 96;
 
 template <typename K, int THREADS, int KPT, int PB, bool INJ>
-__global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, const uint32_t *__restrict__ bstart,
-	const uint32_t *__restrict__ blen, uint32_t *__restrict__ queue)
+__global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, const uint32_t *__restrict__ seg_start,
+	const uint32_t *__restrict__ bstart, const uint32_t *__restrict__ blen, uint32_t *__restrict__ queue)
 {
 	constexpr int CAP = THREADS * KPT;
 	constexpr int P = sizeof(K);
@@ -976,122 +1132,142 @@ __global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, c
 	__shared__ uint32_t s_big;
 	const PassPlan pp = ctrl->hyb[2];
 	if (!pp.active) return;
-	const uint32_t len = blen[blockIdx.x];
-	if (len == 0) return;
-	const uint32_t start = bstart[blockIdx.x];
-	const K *__restrict__ src = static_cast<const K *>(pp.src) + start;
-	K *__restrict__ dst = static_cast<K *>(pp.dst) + start;
+	const K *__restrict__ src_all = static_cast<const K *>(pp.src);
+	K *__restrict__ dst_all = static_cast<K *>(pp.dst);
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
-	if (len == 1) {
-		if (t == 0 && pp.src != pp.dst) dst[0] = src[0];
-		return;
-	}
-	// the top 16 bits of the sort key, which every key of the bucket shares (read before any store:
-	// source and destination may coincide)
-	const W pre = INJ ? (sort_key<K>(src[0], xf) >> (8 * P - 16)) << (8 * P - 16) : W(0);
 	K key[KPT];
+	uint32_t len = 0, start = 0;
+	auto fetch = [&](uint32_t bk, K (&kk)[KPT], uint32_t &ln, uint32_t &st) {
+		ln = 0;
+		st = 0;
+		if (bk < kHybBuckets) {
+			ln = blen[bk];
+			st = seg_start[bk >> 8] + bstart[bk];
+		}
+		const K *__restrict__ src = src_all + st;
 #pragma unroll
-	for (int j = 0; j < KPT; ++j) {
-		const uint32_t l = w * 64u * KPT + j * 64u + lane;
-		key[j] = l < len ? src[l] : K(0);
-	}
-	uint32_t *s_cur = s_u;
-	for (uint32_t i = t; i < NB; i += THREADS) s_cur[i] = 0;
-	if (t == 0) s_big = 0;
-	__syncthreads();
+		for (int j = 0; j < KPT; ++j) {
+			const uint32_t l = w * 64u * KPT + j * 64u + lane;
+			kk[j] = l < ln ? src[l] : K(0);
+		}
+	};
+	const uint32_t bk = blockIdx.x;
+	fetch(bk, key, len, start);
+	{
+		K *__restrict__ dst = dst_all + start;
+		if (len == 1) {
+			if (t == 0 && pp.src != pp.dst) dst[0] = key[0];
+		} else if (len > 1) {
+			// the top 16 bits of the sort key, which every key of the bucket shares: the bucket index
+			const W pre = INJ ? static_cast<W>(static_cast<W>(bk) << (8 * P - 16)) : W(0);
+			uint32_t *s_cur = s_u;
+			for (uint32_t i = t; i < NB; i += THREADS) s_cur[i] = 0;
+			if (t == 0) s_big = 0;
+			__syncthreads();
 #pragma unroll
-	for (int j = 0; j < KPT; ++j) {
-		const uint32_t l = w * 64u * KPT + j * 64u + lane;
-		if (l < len) atomicAdd(&s_cur[static_cast<uint32_t>(static_cast<W>(sort_key<K>(key[j], xf) << 16) >> kTop)], 1u);
-	}
-	__syncthreads();
-	// exclusive scan of the counters, PER consecutive ones per thread
-	uint32_t c[PER], sum = 0, big = 0;
+			for (int j = 0; j < KPT; ++j) {
+				const uint32_t l = w * 64u * KPT + j * 64u + lane;
+				if (l < len) atomicAdd(&s_cur[static_cast<uint32_t>(static_cast<W>(sort_key<K>(key[j], xf) << 16) >> kTop)], 1u);
+			}
+			__syncthreads();
+			// exclusive scan of the counters, PER consecutive ones per thread
+			uint32_t c[PER], sum = 0, big = 0;
 #pragma unroll
-	for (uint32_t k = 0; k < PER; ++k) {
-		c[k] = s_cur[t * PER + k];
-		sum += c[k];
-		big |= c[k] > kLocalRunMax;
-	}
-	const uint32_t incl = wave_inclusive_sum(sum);
-	if (lane == 63) s_wsum[w] = incl;
-	if (big) s_big = 1;
-	__syncthreads();
-	uint32_t run = incl - sum;
-	for (uint32_t v = 0; v < w; ++v) run += s_wsum[v];
+			for (uint32_t k = 0; k < PER; ++k) {
+				c[k] = s_cur[t * PER + k];
+				sum += c[k];
+				big |= c[k] > kLocalRunMax;
+			}
+			const uint32_t incl = wave_inclusive_sum(sum);
+			if (lane == 63) s_wsum[w] = incl;
+			if (big) s_big = 1;
+			__syncthreads();
+			uint32_t run = incl - sum;
+			for (uint32_t v = 0; v < w; ++v) run += s_wsum[v];
 #pragma unroll
-	for (uint32_t k = 0; k < PER; ++k) {
-		s_off[t * PER + k] = static_cast<uint16_t>(run);
-		s_cur[t * PER + k] = run;
-		run += c[k];
-	}
-	if (t == THREADS - 1) s_off[NB] = static_cast<uint16_t>(run);
-	__syncthreads();
-	if (s_big) {
-		if (t == 0) queue[atomicAdd(&ctrl->hyb_queue, 1u)] = blockIdx.x;
-		return;
-	}
+			for (uint32_t k = 0; k < PER; ++k) {
+				s_off[t * PER + k] = static_cast<uint16_t>(run);
+				s_cur[t * PER + k] = run;
+				run += c[k];
+			}
+			if (t == THREADS - 1) s_off[NB] = static_cast<uint16_t>(run);
+			__syncthreads();
+			if (s_big) {
+				if (t == 0) queue[atomicAdd(&ctrl->hyb_queue, 1u)] = bk;
+			} else {
 #pragma unroll
-	for (int j = 0; j < KPT; ++j) {
-		const uint32_t l = w * 64u * KPT + j * 64u + lane;
-		if (l < len) {
-			const W word = static_cast<W>(sort_key<K>(key[j], xf) << 16) | static_cast<W>(l);
-			s_t[atomicAdd(&s_cur[static_cast<uint32_t>(word >> kTop)], 1u)] = word;
+				for (int j = 0; j < KPT; ++j) {
+					const uint32_t l = w * 64u * KPT + j * 64u + lane;
+					if (l < len) {
+						const W word = static_cast<W>(sort_key<K>(key[j], xf) << 16) | static_cast<W>(l);
+						s_t[atomicAdd(&s_cur[static_cast<uint32_t>(word >> kTop)], 1u)] = word;
+					}
+				}
+				__syncthreads();
+				// the final position of every word (its rank in its run, after the run's start)
+				if constexpr (INJ) {
+					// Equal sort keys are equal keys (the transform only complements bits), so every
+					// word gives back its raw key: the bucket's top 16 bits, the word's key bits, the
+					// transform undone. The ranks go to LDS as the inverse permutation (position ->
+					// word), and the output is gathered from the words in order.
+					uint16_t *s_inv = reinterpret_cast<uint16_t *>(s_u);  // the cursors are dead
+#pragma unroll
+					for (int j = 0; j < KPT; ++j) {
+						const uint32_t i = j * THREADS + t;
+						if (i < len) {
+							const W x = s_t[i];
+							const uint32_t d = static_cast<uint32_t>(x >> kTop);
+							const uint32_t lo = s_off[d], hi = s_off[d + 1];
+							uint32_t n_less = 0;  // the word itself is in the run and not smaller
+							for (uint32_t q = lo; q < hi; ++q) n_less += static_cast<uint32_t>(s_t[q] < x);
+							s_inv[lo + n_less] = static_cast<uint16_t>(i);
+						}
+					}
+					__syncthreads();
+					for (uint32_t o = t; o < len; o += THREADS)
+						dst[o] = static_cast<K>(((s_t[s_inv[o]] >> 16) | pre) ^ static_cast<W>(xf.E));
+				} else {
+					// the raw keys travel through their owners: position l's final place, then the
+					// owner of l writes its raw key there
+					uint16_t *s_fin = reinterpret_cast<uint16_t *>(s_u);
+					uint32_t fin[KPT];
+#pragma unroll
+					for (int j = 0; j < KPT; ++j) {
+						const uint32_t i = j * THREADS + t;
+						fin[j] = 0xFFFFFFFFu;
+						if (i < len) {
+							const W x = s_t[i];
+							const uint32_t d = static_cast<uint32_t>(x >> kTop);
+							const uint32_t lo = s_off[d], hi = s_off[d + 1];
+							uint32_t n_less = 0;
+							for (uint32_t q = lo; q < hi; ++q) n_less += static_cast<uint32_t>(s_t[q] < x);
+							fin[j] = ((static_cast<uint32_t>(x) & 0xFFFFu) << 16) | (lo + n_less);
+						}
+					}
+					__syncthreads();  // every run has been read: s_u and s_t take the results
+#pragma unroll
+					for (int j = 0; j < KPT; ++j)
+						if (fin[j] != 0xFFFFFFFFu) s_fin[fin[j] >> 16] = static_cast<uint16_t>(fin[j]);
+					__syncthreads();
+					K *s_out = reinterpret_cast<K *>(s_t);
+#pragma unroll
+					for (int j = 0; j < KPT; ++j) {
+						const uint32_t l = w * 64u * KPT + j * 64u + lane;
+						if (l < len) s_out[s_fin[l]] = key[j];
+					}
+					__syncthreads();
+					for (uint32_t i = t; i < len; i += THREADS) dst[i] = s_out[i];
+				}
+			}
 		}
 	}
-	__syncthreads();
-	uint16_t *s_fin = reinterpret_cast<uint16_t *>(s_u);
-	uint32_t fin[KPT];
-	uint32_t at[KPT];
-	W xw[INJ ? KPT : 1];
-#pragma unroll
-	for (int j = 0; j < KPT; ++j) {
-		const uint32_t i = j * THREADS + t;
-		at[j] = 0xFFFFFFFFu;
-		if (i < len) {
-			const W x = s_t[i];
-			const uint32_t d = static_cast<uint32_t>(x >> kTop);
-			const uint32_t lo = s_off[d], hi = s_off[d + 1];
-			uint32_t n_less = 0;  // the word itself is in the run and not smaller
-			for (uint32_t q = lo; q < hi; ++q) n_less += static_cast<uint32_t>(s_t[q] < x);
-			fin[j] = lo + n_less;
-			at[j] = static_cast<uint32_t>(x) & 0xFFFFu;
-			if constexpr (INJ) xw[j] = x;
-		}
-	}
-	if constexpr (INJ) {
-		// Equal sort keys are equal keys (the transform only complements bits), so each word gives
-		// back its raw key: the bucket's top 16 bits, the word's key bits, the transform undone.
-		// The keys then go straight to their final places, with no exchange through their owners.
-		__syncthreads();  // every run has been read: s_t takes the keys in final order
-		K *s_out = reinterpret_cast<K *>(s_t);
-#pragma unroll
-		for (int j = 0; j < KPT; ++j)
-			if (at[j] != 0xFFFFFFFFu) s_out[fin[j]] = static_cast<K>(((xw[j] >> 16) | pre) ^ static_cast<W>(xf.E));
-		__syncthreads();
-		for (uint32_t i = t; i < len; i += THREADS) dst[i] = s_out[i];
-		return;
-	}
-	__syncthreads();  // the cursors are dead: s_u now holds the final positions
-#pragma unroll
-	for (int j = 0; j < KPT; ++j)
-		if (at[j] != 0xFFFFFFFFu) s_fin[at[j]] = static_cast<uint16_t>(fin[j]);
-	__syncthreads();
-	K *s_raw = reinterpret_cast<K *>(s_t);
-#pragma unroll
-	for (int j = 0; j < KPT; ++j) {
-		const uint32_t l = w * 64u * KPT + j * 64u + lane;
-		if (l < len) s_raw[s_fin[l]] = key[j];
-	}
-	__syncthreads();
-	for (uint32_t i = t; i < len; i += THREADS) dst[i] = s_raw[i];
 }
 
 // The queued buckets (a run of more than kLocalRunMax keys share the partition bits): stable LDS LSD
 // passes over the digits below the top two, from the bucket's original order.
 template <typename K, int THREADS, int KPT>
-__global__ __launch_bounds__(THREADS) void k_local_lsd(const Control *ctrl, KeyXform xf, const uint32_t *__restrict__ bstart,
-	const uint32_t *__restrict__ blen, const uint32_t *__restrict__ queue)
+__global__ __launch_bounds__(THREADS) void k_local_lsd(const Control *ctrl, KeyXform xf, const uint32_t *__restrict__ seg_start,
+	const uint32_t *__restrict__ bstart, const uint32_t *__restrict__ blen, const uint32_t *__restrict__ queue)
 {
 	constexpr int CAP = THREADS * KPT;
 	constexpr int NW = THREADS / 64;
@@ -1108,7 +1284,7 @@ __global__ __launch_bounds__(THREADS) void k_local_lsd(const Control *ctrl, KeyX
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	for (uint32_t qi = blockIdx.x; qi < queued; qi += gridDim.x) {
 		const uint32_t b = queue[qi];
-		const uint32_t len = blen[b], start = bstart[b];
+		const uint32_t len = blen[b], start = seg_start[b >> 8] + bstart[b];
 		const K *__restrict__ src = static_cast<const K *>(pp.src) + start;
 		K *__restrict__ dst = static_cast<K *>(pp.dst) + start;
 		K key[KPT];
@@ -1307,15 +1483,18 @@ __device__ __forceinline__ void reset_control(Control *ctrl)
 		ctrl->need_full = 0;
 		ctrl->key_or = 0;
 		ctrl->key_nor = 0;
+		ctrl->joint_max = 0;
 	}
 }
 
 __global__ void k_finish(Control *ctrl) { reset_control(ctrl); }
 
-// the histograms (which k_plan leaves zeroed) and the control block, whatever state they are in
-__global__ __launch_bounds__(256) void k_reset(Control *ctrl, uint32_t *hist)
+// the histograms (which k_plan leaves zeroed), the overflow counts of the top-16-bit histogram
+// (which k_joint_reduce leaves zeroed) and the control block, whatever state they are in
+__global__ __launch_bounds__(256) void k_reset(Control *ctrl, uint32_t *hist, uint32_t *ovf)
 {
 	for (uint32_t i = threadIdx.x; i < kMaxPasses * kRadix; i += blockDim.x) hist[i] = 0;
+	for (uint32_t i = threadIdx.x; i < kJointBins; i += blockDim.x) ovf[i] = 0;
 	reset_control(ctrl);
 }
 
@@ -1338,6 +1517,18 @@ struct ProfRec {
 std::mutex g_prof_mu;
 bool g_prof_on = false;
 std::vector<ProfRec> g_prof;
+std::vector<hipEvent_t> g_prof_pool;  // events of earlier records, reused (creating them is slow)
+
+hipEvent_t prof_event()
+{
+	if (!g_prof_pool.empty()) {
+		hipEvent_t e = g_prof_pool.back();
+		g_prof_pool.pop_back();
+		return e;
+	}
+	hipEvent_t e = nullptr;
+	return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+}
 
 struct ProfScope {
 	bool on;
@@ -1347,14 +1538,27 @@ struct ProfScope {
 	{
 		std::lock_guard<std::mutex> lock(g_prof_mu);
 		if (!g_prof_on) return;
-		if (hipEventCreate(&rec.a) != hipSuccess || hipEventCreate(&rec.b) != hipSuccess) return;
+		rec.a = prof_event();
+		rec.b = prof_event();
+		if (!rec.a || !rec.b) {
+			if (rec.a) g_prof_pool.push_back(rec.a);
+			if (rec.b) g_prof_pool.push_back(rec.b);
+			return;
+		}
 		on = hipEventRecord(rec.a, s) == hipSuccess;
+		if (!on) { g_prof_pool.push_back(rec.a); g_prof_pool.push_back(rec.b); }
 	}
 	~ProfScope()
 	{
 		if (!on) return;
 		std::lock_guard<std::mutex> lock(g_prof_mu);
-		if (hipEventRecord(rec.b, stream) == hipSuccess) g_prof.push_back(rec);
+		if (hipEventRecord(rec.b, stream) == hipSuccess) {
+			g_prof.push_back(rec);
+		} else {
+			(void)hipEventSynchronize(rec.a);
+			g_prof_pool.push_back(rec.a);
+			g_prof_pool.push_back(rec.b);
+		}
 	}
 };
 
@@ -1408,22 +1612,23 @@ hipError_t launch_pass_cfg(int cfg, Control *ctrl, const uint32_t *bucket, uint6
 	}
 }
 
-template <typename K, bool PAY, int THREADS, int KPT, bool SEG = false, bool GROUP = false>
+template <typename K, bool PAY, int THREADS, int KPT, bool SEG = false, bool GROUP = false, bool CUR = false>
 hipError_t launch_scatter(const PassPlan *pp, uint32_t shift, const KeyXform &xf, uint64_t n, uint32_t tiles,
-	const uint32_t *offsets, SegMap seg, hipStream_t stream)
+	const uint32_t *offsets, SegMap seg, hipStream_t stream, uint32_t *cursor = nullptr)
 {
 	static int occ = -1;
 	if (occ < 0) {
 		int o = 0;
-		if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_scatter<K, PAY, THREADS, KPT, SEG, GROUP>, THREADS, 0) != hipSuccess ||
-			o < 1) o = 1;
+		if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_scatter<K, PAY, THREADS, KPT, SEG, GROUP, CUR>, THREADS, 0) !=
+				hipSuccess || o < 1)
+			o = 1;
 		occ = o;
 	}
 	const uint32_t per = (tiles + 7) / 8;
 	const uint32_t want = static_cast<uint32_t>(occ * devinfo().cus * 2 + 7) / 8;
 	const uint32_t grid = 8u * std::max<uint32_t>(1u, std::min<uint32_t>(per, want));
-	hipLaunchKernelGGL((k_scatter<K, PAY, THREADS, KPT, SEG, GROUP>), dim3(grid), dim3(THREADS), 0, stream, pp, shift, xf, n,
-		tiles, offsets, seg);
+	hipLaunchKernelGGL((k_scatter<K, PAY, THREADS, KPT, SEG, GROUP, CUR>), dim3(grid), dim3(THREADS), 0, stream, pp, shift, xf,
+		n, tiles, offsets, seg, cursor);
 	return hipGetLastError();
 }
 
@@ -1470,15 +1675,15 @@ hipError_t launch_local_cfg(Control *ctrl, const KeyXform &xf, const SegTables &
 {
 	if (xform_injective(xf, sizeof(K)))
 		hipLaunchKernelGGL((k_local<K, THREADS, KPT, PB, true>), dim3(kHybBuckets), dim3(THREADS), 0, stream, ctrl, xf,
-			T.bucket_start, T.bucket_len, T.queue);
+			T.seg_start, T.bucket_start, T.bucket_len, T.queue);
 	else
 		hipLaunchKernelGGL((k_local<K, THREADS, KPT, PB, false>), dim3(kHybBuckets), dim3(THREADS), 0, stream, ctrl, xf,
-			T.bucket_start, T.bucket_len, T.queue);
+			T.seg_start, T.bucket_start, T.bucket_len, T.queue);
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess) return e;
-	const uint32_t grid = static_cast<uint32_t>(4 * devinfo().cus);
-	hipLaunchKernelGGL((k_local_lsd<K, THREADS, KPT>), dim3(grid), dim3(THREADS), 0, stream, ctrl, xf, T.bucket_start,
-		T.bucket_len, T.queue);
+	const uint32_t lgrid = static_cast<uint32_t>(4 * devinfo().cus);
+	hipLaunchKernelGGL((k_local_lsd<K, THREADS, KPT>), dim3(lgrid), dim3(THREADS), 0, stream, ctrl, xf, T.seg_start,
+		T.bucket_start, T.bucket_len, T.queue);
 	return hipGetLastError();
 }
 
@@ -1497,16 +1702,37 @@ hipError_t launch_local(int lcfg, Control *ctrl, const KeyXform &xf, const SegTa
 }
 
 // The two-level MSD plan's launches after k_plan (every one exits at once when the plan did not
-// choose it; k_hyb_check switches the rest off when a bucket is too large).
+// choose it; k_hyb_check switches the rest off when a bucket is too large). With joint (the read
+// was k_hist_joint: the bucket sizes are known and the plan checked them), the second partition
+// takes its output ranges from the bucket cursors and needs no count of its own.
 template <typename K, int THREADS, int KPT>
 hipError_t launch_hybrid(Control *ctrl, const uint32_t *bucket, uint32_t *counts, uint64_t *lookback, const SegTables &T,
-	const KeyXform &xf, uint64_t n, uint32_t tiles, int lcfg, void *ws, hipStream_t stream)
+	const KeyXform &xf, uint64_t n, uint32_t tiles, int lcfg, bool joint, void *ws, hipStream_t stream)
 {
 	constexpr int P = sizeof(K);
 	constexpr uint32_t TILE = THREADS * KPT;
 	const int cus = devinfo().cus;
 	const bool inj = xform_injective(xf, P);
 	hipError_t e;
+	const uint32_t tiles2 = tiles + kRadix;  // bound: one partial tile per segment
+	const SegMap m{T.tile_start, T.seg_start, T.seg_len};
+	if (joint && inj) {
+		// both partitions take their output ranges from cursors: no count or scan launches
+		{
+			ProfScope ps(5, 0, ws, stream);
+			e = launch_scatter<K, false, THREADS, KPT, false, true, true>(&ctrl->hyb[0], 8u * (P - 1), xf, n, tiles, nullptr,
+				SegMap{}, stream, T.rcnt);
+			if (e != hipSuccess) return e;
+		}
+		{
+			ProfScope ps(5, 1, ws, stream);
+			e = launch_scatter<K, false, THREADS, KPT, true, true, true>(&ctrl->hyb[1], 8u * (P - 2), xf, n, tiles2, nullptr, m,
+				stream, T.cursor);
+			if (e != hipSuccess) return e;
+		}
+		ProfScope ps(6, 0, ws, stream);
+		return launch_local<K>(lcfg, ctrl, xf, T, stream);
+	}
 	{
 		ProfScope ps(7, 0, ws, stream);
 		const uint32_t nchunks = (tiles + kScanChunk - 1) / kScanChunk;
@@ -1522,8 +1748,6 @@ hipError_t launch_hybrid(Control *ctrl, const uint32_t *bucket, uint32_t *counts
 		        : launch_scatter<K, false, THREADS, KPT>(&ctrl->hyb[0], 8u * (P - 1), xf, n, tiles, counts, SegMap{}, stream);
 		if (e != hipSuccess) return e;
 	}
-	const uint32_t tiles2 = tiles + kRadix;  // bound: one partial tile per segment
-	const SegMap m{T.tile_start, T.seg_start, T.seg_len};
 	{
 		ProfScope ps(7, 1, ws, stream);
 		const uint32_t cgrid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles2, 8u * cus));
@@ -1547,16 +1771,58 @@ hipError_t launch_hybrid(Control *ctrl, const uint32_t *bucket, uint32_t *counts
 
 template <typename K>
 hipError_t launch_hybrid_cfg(int cfg, Control *ctrl, const uint32_t *bucket, uint32_t *counts, uint64_t *lookback,
-	const SegTables &T, const KeyXform &xf, uint64_t n, uint32_t tiles, int lcfg, void *ws, hipStream_t stream)
+	const SegTables &T, const KeyXform &xf, uint64_t n, uint32_t tiles, int lcfg, bool joint, void *ws, hipStream_t stream)
 {
 	switch (cfg) {
-	case 0: return launch_hybrid<K, 256, 8>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
-	case 1: return launch_hybrid<K, 256, 16>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
-	case 2: return launch_hybrid<K, 512, 8>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
-	case 3: return launch_hybrid<K, 1024, 12>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
-	case 4: return launch_hybrid<K, 256, 24>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
-	default: return launch_hybrid<K, 1024, 16>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, ws, stream);
+	case 0: return launch_hybrid<K, 256, 8>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, joint, ws, stream);
+	case 1: return launch_hybrid<K, 256, 16>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, joint, ws, stream);
+	case 2: return launch_hybrid<K, 512, 8>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, joint, ws, stream);
+	case 3: return launch_hybrid<K, 1024, 12>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, joint, ws, stream);
+	case 4: return launch_hybrid<K, 256, 24>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, joint, ws, stream);
+	default: return launch_hybrid<K, 1024, 16>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, joint, ws, stream);
 	}
+}
+
+// One side stream per device (default priority, not synchronising with the null stream) and the
+// events of the fork and the join. The events are recorded and waited for under a lock, so that
+// every wait sees the record it was paired with.
+struct SideStream {
+	hipStream_t s = nullptr;
+	hipEvent_t fork = nullptr, join = nullptr;
+};
+std::mutex g_side_mu;
+SideStream g_side[64];
+
+// side = a stream whose later work starts after everything enqueued on stream so far
+hipError_t fork_side(hipStream_t stream, hipStream_t *side)
+{
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if (e != hipSuccess) return e;
+	if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+	std::lock_guard<std::mutex> lock(g_side_mu);
+	SideStream &ss = g_side[dev];
+	if (!ss.s) {
+		if ((e = hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking)) != hipSuccess) return e;
+		if ((e = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming)) != hipSuccess) return e;
+		if ((e = hipEventCreateWithFlags(&ss.join, hipEventDisableTiming)) != hipSuccess) return e;
+	}
+	if ((e = hipEventRecord(ss.fork, stream)) != hipSuccess) return e;
+	if ((e = hipStreamWaitEvent(ss.s, ss.fork, 0)) != hipSuccess) return e;
+	*side = ss.s;
+	return hipSuccess;
+}
+
+// stream's later work starts after everything enqueued on side so far
+hipError_t join_side(hipStream_t side, hipStream_t stream)
+{
+	int dev = 0;
+	hipError_t e = hipGetDevice(&dev);
+	if (e != hipSuccess) return e;
+	std::lock_guard<std::mutex> lock(g_side_mu);
+	SideStream &ss = g_side[dev];
+	if ((e = hipEventRecord(ss.join, side)) != hipSuccess) return e;
+	return hipStreamWaitEvent(stream, ss.join, 0);
 }
 
 template <typename K>
@@ -1669,6 +1935,14 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	// count + scan plan, the top digit for the MSD plan
 	const bool counts0 = algo == kAlgoCountScan && !fill;
 	const bool count_tiles = counts0 || hybrid;
+	// The exact bucket sizes (k_hist_joint): integer keys under an injective transform. The read
+	// leaves one 65536-word slice per block in the array the first partition writes (X1, never the
+	// source), so the number of blocks is bounded by that array's size.
+	const bool inj_xf = xform_injective(xf, P);
+	const uint32_t joint_slices = static_cast<uint32_t>(std::min<uint64_t>(static_cast<uint64_t>(cus),
+		n * P / (kJointBins * sizeof(uint32_t)))) / kRanges * kRanges;  // the same number of blocks per key range
+	const bool joint = hybrid && P >= 4 && !float_keys && inj_xf && joint_slices >= kRanges;
+	uint32_t *joint_part = static_cast<uint32_t *>(job.buf != job.in ? job.buf : job.out);
 	if (job.presort_check && !fill) {
 		const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 4095) / 4096, 4u * cus)));
 		hipLaunchKernelGGL((k_check<K, 256>), dim3(grid), dim3(256), 0, stream, static_cast<const K *>(job.in), n, xf, ctrl);
@@ -1683,7 +1957,16 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 			// cluster, usually fail its bucket estimate; for them the read counts every digit, so the
 			// LSD plan it falls back to needs no second read.
 			if constexpr (P >= 4) {
-				if (!float_keys)
+				if (joint) {
+					// the key ranges: whole first-partition tiles, as its blocks deal them to the XCDs
+					const uint64_t range_keys = static_cast<uint64_t>((htiles + kRanges - 1) / kRanges) * htile_keys;
+					hipLaunchKernelGGL((k_hist_joint<K, 1024>), dim3(joint_slices), dim3(1024), 0, stream,
+						static_cast<const K *>(job.in), n, xf, range_keys, (job.flags & kRev) ? 1 : 0, ctrl, joint_part,
+						seg_tables(segtab_of(ws)).ovf);
+					if ((e = hipGetLastError()) != hipSuccess) return e;
+					hipLaunchKernelGGL((k_joint_reduce<P>), dim3(kRadix), dim3(256), 0, stream, ctrl, (job.flags & kRev) ? 1 : 0,
+						joint_part, joint_slices, seg_tables(segtab_of(ws)), hist);
+				} else if (!float_keys)
 					hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS, true>), dim3(grid), dim3(256), 0, stream,
 						static_cast<const K *>(job.in), n, xf, hist, counts, htile_keys, htiles, (job.flags & kRev) ? 1 : 0, ctrl,
 						8u * (P - 1), nullptr);
@@ -1717,14 +2000,22 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	pa.tile2 = htile_keys;
 	pa.segtab = segtab_of(ws);
 	pa.partial = (hybrid && !float_keys) ? 1 : 0;
+	pa.joint = joint ? 1 : 0;
 	pa.gate = nullptr;
 
 	hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, stream, pa, ctrl, hist, bucket);
 	if ((e = hipGetLastError()) != hipSuccess) return e;
+	hipStream_t ls = stream;  // the stream of the LSD launches below
 	if constexpr (P >= 4) {
 		if (hybrid) {
-			e = launch_hybrid_cfg<K>(hcfg, ctrl, bucket, counts, lookback, seg_tables(segtab_of(ws)), xf, n, htiles, lcfg, ws,
-				stream);
+			// With the exact bucket sizes the plan's choice is final once k_plan has run: the fallback
+			// launches then go to a side stream that waits for k_plan and runs alongside the MSD
+			// launches (each of the two sets exits at once unless it was chosen), so that the
+			// fallback's launches, about 30 dispatches, cost no time after the bucket sort. The
+			// stream waits for the side stream before the sort ends.
+			if (joint && (e = fork_side(stream, &ls)) != hipSuccess) return e;
+			e = launch_hybrid_cfg<K>(hcfg, ctrl, bucket, counts, lookback, seg_tables(segtab_of(ws)), xf, n, htiles, lcfg, joint,
+				ws, stream);
 			if (e != hipSuccess) return e;
 			// The LSD count + scan plan as the fallback: a full histogram read and a second plan,
 			// both gated on need_full, then the launches below (inactive unless planned). Onesweep
@@ -1732,14 +2023,15 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 			// plan runs at (2^28 f32 in [0, 1): 3.2 ms per pass against about 1 ms).
 			algo = kAlgoCountScan;
 			const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, 8u * cus));
-			hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS>), dim3(grid), dim3(256), 0, stream, static_cast<const K *>(job.in), n,
+			hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS>), dim3(grid), dim3(256), 0, ls, static_cast<const K *>(job.in), n,
 				xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0, ctrl, 0u, &ctrl->need_full);
 			if ((e = hipGetLastError()) != hipSuccess) return e;
 			pa.counts0 = 1;
 			pa.hybrid = 0;
 			pa.partial = 0;
+			pa.joint = 0;
 			pa.gate = &ctrl->need_full;
-			hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, stream, pa, ctrl, hist, bucket);
+			hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, ls, pa, ctrl, hist, bucket);
 			if ((e = hipGetLastError()) != hipSuccess) return e;
 		}
 	}
@@ -1747,38 +2039,39 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	const uint32_t cgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 8u * cus)));
 	if (fill) {
 		// bijective 8-bit modes: the sorted output follows from the counts alone
-		hipLaunchKernelGGL(k_fill8, dim3(cgrid), dim3(256), 0, stream, bucket, xf, static_cast<uint8_t *>(job.out), n);
+		hipLaunchKernelGGL(k_fill8, dim3(cgrid), dim3(256), 0, ls, bucket, xf, static_cast<uint8_t *>(job.out), n);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 	} else {
-		hipLaunchKernelGGL((k_copy<K>), dim3(cgrid), dim3(256), 0, stream, ctrl, 0, n, job.payload ? 1 : 0);
+		hipLaunchKernelGGL((k_copy<K>), dim3(cgrid), dim3(256), 0, ls, ctrl, 0, n, job.payload ? 1 : 0);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 		for (int p = 0; p < P; ++p) {
 			if (algo == kAlgoOnesweep) {
-				ProfScope ps(1, p, ws, stream);
-				e = job.payload ? launch_pass_cfg<K, true>(cfg, ctrl, bucket, lookback, n, xf, p, stream)
-				                : launch_pass_cfg<K, false>(cfg, ctrl, bucket, lookback, n, xf, p, stream);
+				ProfScope ps(1, p, ws, ls);
+				e = job.payload ? launch_pass_cfg<K, true>(cfg, ctrl, bucket, lookback, n, xf, p, ls)
+				                : launch_pass_cfg<K, false>(cfg, ctrl, bucket, lookback, n, xf, p, ls);
 				if (e != hipSuccess) return e;
 				continue;
 			}
 			{
-				ProfScope ps(3, p, ws, stream);
+				ProfScope ps(3, p, ws, ls);
 				const uint32_t cgrid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, 8u * cus));
-				hipLaunchKernelGGL((k_count<K, 256, HITEMS>), dim3(cgrid), dim3(256), 0, stream, ctrl, p, xf, n, tile_keys, tiles, counts);
+				hipLaunchKernelGGL((k_count<K, 256, HITEMS>), dim3(cgrid), dim3(256), 0, ls, ctrl, p, xf, n, tile_keys, tiles, counts);
 				if ((e = hipGetLastError()) != hipSuccess) return e;
 				const uint32_t nchunks = (tiles + kScanChunk - 1) / kScanChunk;
 				const uint32_t sgrid = std::max<uint32_t>(1u, std::min<uint32_t>(nchunks, 4u * cus));
-				hipLaunchKernelGGL(k_scan, dim3(sgrid), dim3(256), 0, stream, ctrl, &ctrl->pass[p], &ctrl->scan_counter[p],
+				hipLaunchKernelGGL(k_scan, dim3(sgrid), dim3(256), 0, ls, ctrl, &ctrl->pass[p], &ctrl->scan_counter[p],
 					bucket + p * kRadix, p, counts, lookback, tiles);
 				if ((e = hipGetLastError()) != hipSuccess) return e;
 			}
-			ProfScope ps(1, p, ws, stream);
-			e = job.payload ? launch_scatter_cfg<K, true>(cfg, ctrl, p, xf, n, tiles, counts, stream)
-			                : launch_scatter_cfg<K, false>(cfg, ctrl, p, xf, n, tiles, counts, stream);
+			ProfScope ps(1, p, ws, ls);
+			e = job.payload ? launch_scatter_cfg<K, true>(cfg, ctrl, p, xf, n, tiles, counts, ls)
+			                : launch_scatter_cfg<K, false>(cfg, ctrl, p, xf, n, tiles, counts, ls);
 			if (e != hipSuccess) return e;
 		}
-		hipLaunchKernelGGL((k_copy<K>), dim3(cgrid), dim3(256), 0, stream, ctrl, 1, n, job.payload ? 1 : 0);
+		hipLaunchKernelGGL((k_copy<K>), dim3(cgrid), dim3(256), 0, ls, ctrl, 1, n, job.payload ? 1 : 0);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 	}
+	if (ls != stream && (e = join_side(ls, stream)) != hipSuccess) return e;
 	hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, stream, ctrl);
 	return hipGetLastError();
 }
@@ -1790,7 +2083,7 @@ int device_cu_count() { return devinfo().cus; }
 void profile_begin()
 {
 	std::lock_guard<std::mutex> lock(g_prof_mu);
-	for (auto &r : g_prof) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+	for (auto &r : g_prof) { (void)hipEventSynchronize(r.b); g_prof_pool.push_back(r.a); g_prof_pool.push_back(r.b); }
 	g_prof.clear();
 	g_prof_on = true;
 }
@@ -1827,9 +2120,9 @@ hipError_t profile_end(double *ms, unsigned *launches, int nkinds)
 		}
 		if (e != hipSuccess) err = e;
 		else if (kind < nkinds) { ms[kind] += t; ++launches[kind]; }
-		(void)hipEventDestroy(r.a);
-		(void)hipEventDestroy(r.b);
 	}
+	std::lock_guard<std::mutex> lock(g_prof_mu);
+	for (auto &r : recs) { g_prof_pool.push_back(r.a); g_prof_pool.push_back(r.b); }
 	return err;
 }
 
@@ -1889,7 +2182,7 @@ size_t sort_workspace_bytes(uint64_t n)
 
 hipError_t workspace_reset(void *ws, hipStream_t stream)
 {
-	hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, stream, ctrl_of(ws), hist_of(ws));
+	hipLaunchKernelGGL(k_reset, dim3(1), dim3(256), 0, stream, ctrl_of(ws), hist_of(ws), seg_tables(segtab_of(ws)).ovf);
 	return hipGetLastError();
 }
 
