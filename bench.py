@@ -24,11 +24,54 @@ METRIC = "sorted GKeys/s & % HBM roofline (u64 2^28); learned vs fixed plan spee
 HBM_PEAK_GBS = 8000.0
 
 
+def host_cpus():
+    """The host's logical CPU count, the CPUs this process may run on, the thread share the job is
+    given (OMP_NUM_THREADS when set, else every CPU it may run on) and the CPU model."""
+    nproc = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = nproc
+    share = allowed
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        share = min(allowed, int(env))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": nproc, "affinity": allowed, "threads": share, "model": model}
+
+
+def _multiset_sums(a):
+    """Two order-independent sums of a u64 array (plain and mixed, mod 2^64): equal for a
+    permutation, and a lost or duplicated key changes them."""
+    import numpy as np
+    s0 = s1 = 0
+    m = np.empty(min(a.size, 1 << 22), np.uint64)   # one reused chunk: no array-sized temporaries
+    for i in range(0, a.size, m.size):
+        c = a[i:i + m.size]
+        t = m[:c.size]
+        np.multiply(c, np.uint64(0x9E3779B97F4A7C15), out=t)
+        t ^= t >> np.uint64(29)
+        s0 += int(c.sum(dtype=np.uint64))
+        s1 += int(t.sum(dtype=np.uint64))
+    return s0 % (1 << 64), s1 % (1 << 64)
+
+
 def cpu_baseline(sample_log2: int, budget_s: float):
+    """The oracle's LSD sort (the reference's 11-bit digit plan, key range sliced over the job's
+    CPU share) on the same workload as the GPU line, verified by order and multiset sums."""
     import numpy as np
 
     import oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
+    cpus = host_cpus()
+    threads = cpus["threads"]
     n = 1 << sample_log2
     rng = np.random.default_rng(0x5EED0001)
     keys = rng.integers(0, np.iinfo(np.uint64).max, n, dtype=np.uint64, endpoint=True)
@@ -39,11 +82,14 @@ def cpu_baseline(sample_log2: int, budget_s: float):
         t0 = time.perf_counter()
         oracle.radixsort_mt(keys, 0, threads, out)
         times.append(time.perf_counter() - t0)
-    assert np.array_equal(out, np.sort(keys)), "cpu baseline produced an unsorted result"
+    ok = bool((out[1:] >= out[:-1]).all()) and _multiset_sums(out) == _multiset_sums(keys)
+    assert ok, "cpu baseline produced a result that is not a sorted permutation"
     med = statistics.median(times)
     return {"value": round(n / med / 1e9, 4), "unit": "GKeys/s", "cores": threads, "kind": "port",
-            "sample": f"2^{sample_log2} uniform u64 keys (same distribution), oracle LSD sort with the "
-                      f"reference's 11-bit digit plan sliced over {threads} threads, median of {len(times)}"}
+            "host": cpus, "median_s": round(med, 4), "runs": len(times),
+            "sample": f"2^{sample_log2} uniform u64 keys (same distribution as the GPU line), oracle LSD sort "
+                      f"with the reference's 11-bit digit plan, key range sliced over {threads} threads, "
+                      f"median of {len(times)}, checked sorted and a permutation (multiset sums)"}
 
 
 def learned_vs_fixed(requests: int, seed: int):
@@ -83,7 +129,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n-log2", type=int, default=28)
-    ap.add_argument("--cpu-sample-log2", type=int, default=26)
+    ap.add_argument("--cpu-sample-log2", type=int, default=28)
     ap.add_argument("--cpu-budget-s", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-selector", action="store_true", help="skip the learned-vs-fixed plan comparison")
@@ -118,15 +164,24 @@ def main():
     torch.cuda.synchronize()
     print(f"bench: warmup done (n=2^{args.n_log2})", file=sys.stderr, flush=True)
 
-    r.profile_begin()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r.radixsortcopynoalloc(keys, out, buf, ws)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    prof = r.profile_end()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
+    # the same steps again with every launch phase bracketed by HIP events on the launch stream
+    # (the events add about 0.1 ms per sort, so the headline steps above run without them)
+    r.profile_begin()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        r.radixsortcopynoalloc(keys, out, buf, ws)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    prof = r.profile_end()
+    ms_per_step_events = (t3 - t2) * 1e3 / args.steps
 
     # verify the last result: monotone under the sign-flipped int64 view, and a permutation
     flip = out.view(torch.int64) ^ torch.tensor(-(1 << 63), dtype=torch.int64, device=dev)
@@ -150,7 +205,14 @@ def main():
         "histogram": (prof["hist"], k * n),
     }
     msd = prof["msd_local"][1] > 0
-    if msd:
+    if msd and prof["msd_countscan"][1] == 0:
+        # top-16-bit histogram read, top-digit scatter, segmented second-digit scatter, LDS sort
+        sort_bytes = k * n + 2 * k * n + 2 * k * n + 2 * k * n
+        passes = 2
+        plan = ("exact top-16-bit histogram read, two MSD partitions by the top two 8-bit digits placed by "
+                "atomic cursors (per key range, then per bucket of the segment), then one LDS sort per "
+                "top-16-bit bucket (an LDS partition on the next 11 bits and exact ranking inside its runs)")
+    elif msd:
         # hist read, top-digit scatter, second-digit count read, segmented scatter, LDS sort
         sort_bytes = k * n + 2 * k * n + k * n + 2 * k * n + 2 * k * n
         passes = 2
@@ -187,6 +249,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step_with_events": round(ms_per_step_events, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
