@@ -104,15 +104,19 @@ __device__ __forceinline__ typename ItemWord<K>::W item_key(K raw, const KeyXfor
 }
 
 // Items i < *count (or count_value when count is null): keys [start, start + len) of src, read back
-// to front under reverse, sorted into the same range of dst. The keys of an item are partitioned
-// with LDS atomics by (key - min) >> sh, sh chosen so that the item's key range spans the PB
-// partition bits; each word's final position is its partition's start plus the words of the
-// partition that precede it in the stable order (smaller key, or equal key read earlier), counted
-// with independent LDS reads; the raw keys are staged in LDS at their final positions. An item
-// whose keys are all equal is copied in order. Items with a partition of more than kItemRunMax
-// keys are queued for k_item_lsd. With INJ (the transform only complements bits, so equal sort keys
-// are equal keys and a key is its sort key xor E) the order among equal keys is unobservable: ties
-// are broken by the index in the partitioned array, and the sorted words are turned back into keys.
+// to front under reverse, sorted into the same range of dst. Every block takes the items
+// blockIdx.x, blockIdx.x + gridDim.x, ... and loads the keys of its next item while it sorts the
+// current one (the loads are issued before the current item's stores, so waiting for them never
+// waits for those). The keys of an item are partitioned with LDS atomics by (key - min) >> sh, sh
+// chosen so that the item's key range spans the PB partition bits; each word's final position is
+// its partition's start plus the words of the partition that precede it in the stable order
+// (smaller key, or equal key read earlier), counted with independent LDS reads; the raw keys are
+// staged in LDS at their final positions. An item whose keys are all equal is copied in order.
+// Items with a partition of more than kItemRunMax keys are queued for k_item_lsd. With INJ (the
+// transform only complements bits, so equal sort keys are equal keys and a key is its sort key xor
+// E) the order among equal keys is unobservable: ties are broken by the index in the partitioned
+// array, the ranks go to LDS as the inverse permutation (position -> word) and the output is
+// gathered from the words in order, which needs no second word array.
 template <typename K, int THREADS, int KPT, int PB, bool INJ>
 __global__ __launch_bounds__(THREADS) void k_item_sort(const K *__restrict__ src, K *__restrict__ dst,
 	const uint64_t *__restrict__ istart, const uint32_t *__restrict__ ilen, const uint32_t *count, uint32_t count_value,
@@ -128,31 +132,45 @@ __global__ __launch_bounds__(THREADS) void k_item_sort(const K *__restrict__ src
 	constexpr uint32_t PER = (NB + THREADS - 1) / THREADS;
 	static_assert(CAP <= 65536, "positions are held in 16 bits");
 	static_assert(sizeof(K) <= sizeof(W), "the raw keys are staged in the word array");
-	__shared__ __align__(16) W s_w[CAP];     // words, then the raw keys in final order
-	__shared__ uint16_t s_l[INJ ? 1 : CAP];  // positions of unpacked words, then each key's final position
-	__shared__ __align__(16) W s_o[INJ ? CAP : 1];  // with INJ: the sorted words
-	__shared__ uint32_t s_cur[NB];           // partition counters, then cursors (after the scatter: ends)
+	__shared__ __align__(16) W s_w[CAP];  // words, then (without INJ) the raw keys in final order
+	__shared__ uint16_t s_l[(INJ || !IW::packed) ? CAP : 1];  // positions of unpacked words / the inverse permutation
+	// partition counters, then cursors (after the scatter: ends): two 16-bit halves per word when
+	// every thread of the scan owns whole words (counts and positions stay below CAP <= 65536)
+	constexpr bool PACK = PER % 2 == 0;
+	constexpr uint32_t NBW = PACK ? NB / 2 : NB;
+	__shared__ uint32_t s_cur[NBW];
+	auto cur_word = [](uint32_t b) { return PACK ? b >> 1 : b; };
+	auto cur_shift = [](uint32_t b) { return PACK ? (b & 1u) * 16u : 0u; };
+	auto cur_get = [&](uint32_t b) { return PACK ? (s_cur[b >> 1] >> ((b & 1u) * 16u)) & 0xFFFFu : s_cur[b]; };
 	__shared__ uint32_t s_wsum[NW];
 	__shared__ W s_min[NW], s_max[NW];
 	__shared__ uint32_t s_big;
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	const uint32_t nitems = count ? *count : count_value;
-	for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-		const uint32_t len = ilen[it];
-		if (len == 0) continue;
-		const uint64_t start = istart[it];
-		if (len == 1) {
-			if (t == 0) dst[start] = src[start];
-			continue;
+	K key[KPT];
+	uint32_t len = 0;
+	uint64_t start = 0;
+	auto fetch = [&](uint32_t item, K (&kk)[KPT], uint32_t &ln, uint64_t &st) {
+		ln = 0;
+		st = 0;
+		if (item < nitems) {
+			ln = ilen[item];
+			st = istart[item];
 		}
-		K key[KPT];
-		W tk[KPT];
-		W mn = ~W(0), mx = 0;
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) {
 			const uint32_t l = w * 64u * KPT + j * 64u + lane;
-			key[j] = l < len ? src[start + (reverse ? len - 1 - l : l)] : K(0);
+			kk[j] = l < ln ? src[st + (reverse ? ln - 1 - l : l)] : K(0);
 		}
+	};
+	auto sort_item = [&](uint32_t it) {
+		if (len == 0) return;
+		if (len == 1) {
+			if (t == 0) dst[start] = key[0];
+			return;
+		}
+		W tk[KPT];
+		W mn = ~W(0), mx = 0;
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) {
 			const uint32_t l = w * 64u * KPT + j * 64u + lane;
@@ -167,8 +185,7 @@ __global__ __launch_bounds__(THREADS) void k_item_sort(const K *__restrict__ src
 			mn = min(mn, static_cast<W>(__shfl_xor(mn, o, 64)));
 			mx = max(mx, static_cast<W>(__shfl_xor(mx, o, 64)));
 		}
-		__syncthreads();  // the previous item's last reads of the shared arrays are done
-		for (uint32_t i = t; i < NB; i += THREADS) s_cur[i] = 0;
+		for (uint32_t i = t; i < NBW; i += THREADS) s_cur[i] = 0;
 		if (t == 0) s_big = 0;
 		if (lane == 0) { s_min[w] = mn; s_max[w] = mx; }
 		__syncthreads();
@@ -184,7 +201,7 @@ __global__ __launch_bounds__(THREADS) void k_item_sort(const K *__restrict__ src
 				const uint32_t l = w * 64u * KPT + j * 64u + lane;
 				if (l < len) dst[start + l] = key[j];
 			}
-			continue;
+			return;
 		}
 		// highest bit of the key range; the partition digit is (key - min) >> sh, below NB
 		int hb;
@@ -194,14 +211,17 @@ __global__ __launch_bounds__(THREADS) void k_item_sort(const K *__restrict__ src
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) {
 			const uint32_t l = w * 64u * KPT + j * 64u + lane;
-			if (l < len) atomicAdd(&s_cur[static_cast<uint32_t>((tk[j] - mn) >> sh)], 1u);
+			if (l < len) {
+				const uint32_t b = static_cast<uint32_t>((tk[j] - mn) >> sh);
+				atomicAdd(&s_cur[cur_word(b)], 1u << cur_shift(b));
+			}
 		}
 		__syncthreads();
 		uint32_t c[PER], sum = 0, big = 0;
 #pragma unroll
 		for (uint32_t k = 0; k < PER; ++k) {
 			const uint32_t b = t * PER + k;
-			c[k] = b < NB ? s_cur[b] : 0u;
+			c[k] = b < NB ? cur_get(b) : 0u;
 			sum += c[k];
 			big |= c[k] > kItemRunMax;
 		}
@@ -211,45 +231,59 @@ __global__ __launch_bounds__(THREADS) void k_item_sort(const K *__restrict__ src
 		__syncthreads();
 		uint32_t run = incl - sum;
 		for (uint32_t v = 0; v < w; ++v) run += s_wsum[v];
+		if constexpr (PACK) {
 #pragma unroll
-		for (uint32_t k = 0; k < PER; ++k) {
-			const uint32_t b = t * PER + k;
-			if (b < NB) s_cur[b] = run;
-			run += c[k];
+			for (uint32_t k = 0; k < PER; k += 2) {
+				const uint32_t b = t * PER + k;
+				const uint32_t r0 = run, r1 = run + c[k];
+				run = r1 + c[k + 1];
+				if (b < NB) s_cur[b >> 1] = r0 | (r1 << 16);
+			}
+		} else {
+#pragma unroll
+			for (uint32_t k = 0; k < PER; ++k) {
+				const uint32_t b = t * PER + k;
+				if (b < NB) s_cur[b] = run;
+				run += c[k];
+			}
 		}
 		__syncthreads();
 		if (s_big) {
 			if (t == 0) queue[atomicAdd(qcount, 1u)] = qbase + it;  // an index into the whole item table
-			continue;
+			return;
 		}
 		if constexpr (INJ) {
 #pragma unroll
 			for (int j = 0; j < KPT; ++j) {
 				const uint32_t l = w * 64u * KPT + j * 64u + lane;
-				if (l < len) s_w[atomicAdd(&s_cur[static_cast<uint32_t>((tk[j] - mn) >> sh)], 1u)] = tk[j];
+				if (l < len) {
+					const uint32_t b = static_cast<uint32_t>((tk[j] - mn) >> sh);
+					s_w[(atomicAdd(&s_cur[cur_word(b)], 1u << cur_shift(b)) >> cur_shift(b)) & 0xFFFFu] = tk[j];
+				}
 			}
 			__syncthreads();
 			for (uint32_t i = t; i < len; i += THREADS) {
 				const W x = s_w[i];
 				const uint32_t d = static_cast<uint32_t>((x - mn) >> sh);
-				const uint32_t lo = d ? s_cur[d - 1] : 0u, hi = s_cur[d];
+				const uint32_t lo = d ? cur_get(d - 1) : 0u, hi = cur_get(d);
+				// ties go by the index in the partitioned array: the words before i count when not
+				// larger, those after it when smaller
 				uint32_t n_less = 0;
-				for (uint32_t q = lo; q < hi; ++q) {
-					const W y = s_w[q];
-					n_less += static_cast<uint32_t>(y < x) | (static_cast<uint32_t>(y == x) & static_cast<uint32_t>(q < i));
-				}
-				s_o[lo + n_less] = x;
+				for (uint32_t q = lo; q < i; ++q) n_less += static_cast<uint32_t>(s_w[q] <= x);
+				for (uint32_t q = i + 1; q < hi; ++q) n_less += static_cast<uint32_t>(s_w[q] < x);
+				s_l[lo + n_less] = static_cast<uint16_t>(i);
 			}
 			__syncthreads();
 			const W e = static_cast<W>(xf.E);
-			for (uint32_t i = t; i < len; i += THREADS) dst[start + i] = static_cast<K>(s_o[i] ^ e);
-			continue;
+			for (uint32_t i = t; i < len; i += THREADS) dst[start + i] = static_cast<K>(s_w[s_l[i]] ^ e);
+			return;
 		}
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) {
 			const uint32_t l = w * 64u * KPT + j * 64u + lane;
 			if (l < len) {
-				const uint32_t p = atomicAdd(&s_cur[static_cast<uint32_t>((tk[j] - mn) >> sh)], 1u);
+				const uint32_t b = static_cast<uint32_t>((tk[j] - mn) >> sh);
+				const uint32_t p = (atomicAdd(&s_cur[cur_word(b)], 1u << cur_shift(b)) >> cur_shift(b)) & 0xFFFFu;
 				if constexpr (IW::packed) {
 					s_w[p] = (tk[j] << 16) | static_cast<W>(l);
 				} else {
@@ -277,7 +311,7 @@ __global__ __launch_bounds__(THREADS) void k_item_sort(const K *__restrict__ src
 				}
 				// the cursors have advanced to the ends: partition d holds words [end of d - 1, end of d)
 				const uint32_t d = static_cast<uint32_t>((kx - mn) >> sh);
-				const uint32_t lo = d ? s_cur[d - 1] : 0u, hi = s_cur[d];
+				const uint32_t lo = d ? cur_get(d - 1) : 0u, hi = cur_get(d);
 				uint32_t n_less = 0;
 				if constexpr (IW::packed) {
 					for (uint32_t q = lo; q < hi; ++q) n_less += static_cast<uint32_t>(s_w[q] < x);
@@ -304,6 +338,19 @@ __global__ __launch_bounds__(THREADS) void k_item_sort(const K *__restrict__ src
 		}
 		__syncthreads();
 		for (uint32_t i = t; i < len; i += THREADS) dst[start + i] = s_raw[i];
+	};
+	fetch(blockIdx.x, key, len, start);
+	for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+		K nkey[KPT];
+		uint32_t nlen;
+		uint64_t nstart;
+		fetch(it + gridDim.x, nkey, nlen, nstart);
+		__syncthreads();  // the previous item's last reads of the shared arrays are done
+		sort_item(it);
+#pragma unroll
+		for (int j = 0; j < KPT; ++j) key[j] = nkey[j];
+		len = nlen;
+		start = nstart;
 	}
 }
 
@@ -955,7 +1002,11 @@ hipError_t launch_lsd_t(const void *src, void *dst, const uint64_t *istart, cons
 }
 
 // capacity tiers: 0 = 256x4 (1024 keys), 1 = 256x16 (4096), 2 = 512x16 (8192), partition bits chosen
-// for about four keys per partition
+// for about four keys per partition (two in the 4096 tier, whose counters take 16-bit halves: its
+// LDS, 29 KB, still lets four blocks share a CU). Measured on the C4 batch scaled to 2^28 keys, the
+// 4096 tier's sort takes 1.30-1.34 ms whether partitions hold four, two or one key (with one, 33 KB
+// of LDS leaves three blocks per CU); its counters put it at about 95 vector instructions per key,
+// so that it is bound by instruction issue rather than by memory.
 template <typename K>
 hipError_t launch_sort_cfg(int cfg, const void *src, void *dst, const uint64_t *istart, const uint32_t *ilen,
 	const uint32_t *count, uint32_t count_value, const KeyXform &xf, int reverse, uint32_t *queue, uint32_t *qcount,
@@ -963,7 +1014,7 @@ hipError_t launch_sort_cfg(int cfg, const void *src, void *dst, const uint64_t *
 {
 	switch (cfg) {
 	case 0: return launch_sort_t<K, 256, 4, 8>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase, s);
-	case 1: return launch_sort_t<K, 256, 16, 10>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase, s);
+	case 1: return launch_sort_t<K, 256, 16, 11>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase, s);
 	default: return launch_sort_t<K, 512, 16, 11>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase, s);
 	}
 }
