@@ -453,7 +453,7 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(Control *ctrl, const uint3
 	__shared__ uint32_t s_digoff[kRadix];
 	__shared__ uint32_t s_glob[kRadix];
 	__shared__ uint32_t s_wsum[4];
-	__shared__ uint32_t s_tile;
+	__shared__ uint32_t s_tile[2];
 	__shared__ uint8_t s_dig[PAY ? TILE : 1];
 	constexpr int kStage = (PAY && sizeof(K) < 4) ? 4 : static_cast<int>(sizeof(K));
 	// the tile staging, and before it the ranking's peer words
@@ -463,37 +463,52 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(Control *ctrl, const uint3
 
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	const uint32_t *pbucket = bucket + pass * kRadix;
-
-	for (;;) {
-		if (t == 0) s_tile = atomicAdd(&ctrl->tile_counter[pass], 1u);
-		for (int i = t; i < NW * kRadix; i += THREADS) {
-			s_whist[i] = 0;
-			s_peer[i] = 0;
-		}
-		__syncthreads();
-		const uint32_t tile = s_tile;
-		if (tile >= num_tiles) break;
-		const uint64_t base = static_cast<uint64_t>(tile) * TILE;
-		const uint32_t valid = (n - base) < static_cast<uint64_t>(TILE) ? static_cast<uint32_t>(n - base) : static_cast<uint32_t>(TILE);
-
-		K keys[KPT];
-		uint32_t dig[KPT];
-		uint32_t rank[KPT];
-		uint32_t pay[PAY ? KPT : 1];
+	auto valid_of = [&](uint32_t tl) {
+		const uint64_t base = static_cast<uint64_t>(tl) * TILE;
+		return (n - base) < static_cast<uint64_t>(TILE) ? static_cast<uint32_t>(n - base) : static_cast<uint32_t>(TILE);
+	};
+	K keys[KPT];
+	uint32_t pay[PAY ? KPT : 1], npay[PAY ? KPT : 1];
+	// the keys (and payload) of tile tl in logical order l = w*64*KPT + j*64 + lane
+	auto load = [&](uint32_t tl, uint32_t (&py)[PAY ? KPT : 1]) {
+		if (tl >= num_tiles) return;
+		const uint64_t base = static_cast<uint64_t>(tl) * TILE;
+		const uint32_t valid = valid_of(tl);
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) {
 			const uint32_t l = w * 64u * KPT + j * 64u + lane;
 			const uint64_t g = base + l;
 			if (l < valid) {
-				const uint64_t s = pp.reverse ? n - 1 - g : g;
-				keys[j] = src[s];
-				dig[j] = digit_of<K>(keys[j], xf, shift);
-				if constexpr (PAY) pay[j] = pp.iota ? static_cast<uint32_t>(s) : pp.psrc[s];
+				const uint64_t sidx = pp.reverse ? n - 1 - g : g;
+				keys[j] = src[sidx];
+				if constexpr (PAY) py[j] = pp.iota ? static_cast<uint32_t>(sidx) : pp.psrc[sidx];
 			} else {
 				keys[j] = K(0);
-				dig[j] = 0xFFu;  // past the end: ranked last, never written
-				if constexpr (PAY) pay[j] = 0;
+				if constexpr (PAY) py[j] = 0;
 			}
+		}
+	};
+	// tiles are taken in ticket order, one ticket ahead: the next tile's loads are issued before
+	// this tile's look-back and stores, so that they overlap both
+	if (t == 0) s_tile[0] = atomicAdd(&ctrl->tile_counter[pass], 1u);
+	__syncthreads();
+	uint32_t tile = s_tile[0];
+	load(tile, pay);
+	for (uint32_t it = 0; tile < num_tiles; ++it) {
+		if (t == 0) s_tile[(it + 1) & 1u] = atomicAdd(&ctrl->tile_counter[pass], 1u);
+		for (int i = t; i < NW * kRadix; i += THREADS) {
+			s_whist[i] = 0;
+			s_peer[i] = 0;
+		}
+		__syncthreads();
+		const uint32_t next = s_tile[(it + 1) & 1u];
+		const uint32_t valid = valid_of(tile);
+		uint32_t dig[KPT];
+		uint32_t rank[KPT];
+#pragma unroll
+		for (int j = 0; j < KPT; ++j) {
+			const uint32_t l = w * 64u * KPT + j * 64u + lane;
+			dig[j] = l < valid ? digit_of<K>(keys[j], xf, shift) : 0xFFu;  // past the end: ranked last, never written
 		}
 		// rank within the wave in logical order (j-major, lane-minor)
 #pragma unroll
@@ -524,22 +539,25 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(Control *ctrl, const uint3
 		if (t < kRadix) {
 			uint32_t before = 0;
 			for (uint32_t v = 0; v < w; ++v) before += s_wsum[v];
-			const uint32_t digoff = s_digoff[t] + before;
-			s_digoff[t] = digoff;
+			s_digoff[t] += before;
+		}
+		__syncthreads();
+		// stage the tile in LDS in digit order; the key registers are then free for the next tile
+		uint32_t pos[KPT];
+#pragma unroll
+		for (int j = 0; j < KPT; ++j) {
+			pos[j] = s_digoff[dig[j]] + s_whist[w * kRadix + dig[j]] + rank[j];
+			s_keys[pos[j]] = keys[j];
+		}
+		load(next, npay);
+		if (t < kRadix) {
 			// decoupled look-back over the preceding tiles of this digit
 			uint32_t excl = 0;
 			if (tile > 0) {
 				excl = lookback_window<16, kLbStride>(lookback + t, tile, 0, tag);
 				atomic_store_agent(&lookback[static_cast<uint64_t>(tile) * kLbStride + t], lb_word(excl + pub, kLbInclusive, tag));
 			}
-			s_glob[t] = pbucket[t] + excl - digoff;
-		}
-		__syncthreads();
-		// stage the tile in LDS in digit order
-#pragma unroll
-		for (int j = 0; j < KPT; ++j) {
-			const uint32_t pos = s_digoff[dig[j]] + s_whist[w * kRadix + dig[j]] + rank[j];
-			s_keys[pos] = keys[j];
+			s_glob[t] = pbucket[t] + excl - s_digoff[t];
 		}
 		__syncthreads();
 		for (uint32_t i = t; i < valid; i += THREADS) {
@@ -552,14 +570,14 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(Control *ctrl, const uint3
 			__syncthreads();
 			uint32_t *s_pay = reinterpret_cast<uint32_t *>(s_raw);
 #pragma unroll
-			for (int j = 0; j < KPT; ++j) {
-				const uint32_t pos = s_digoff[dig[j]] + s_whist[w * kRadix + dig[j]] + rank[j];
-				s_pay[pos] = pay[j];
-			}
+			for (int j = 0; j < KPT; ++j) s_pay[pos[j]] = pay[j];
 			__syncthreads();
 			for (uint32_t i = t; i < valid; i += THREADS) pp.pdst[s_glob[s_dig[i]] + i] = s_pay[i];
+#pragma unroll
+			for (int j = 0; j < KPT; ++j) pay[j] = npay[j];
 		}
 		__syncthreads();
+		tile = next;
 	}
 }
 
@@ -2017,11 +2035,12 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 			e = launch_hybrid_cfg<K>(hcfg, ctrl, bucket, counts, lookback, seg_tables(segtab_of(ws)), xf, n, htiles, lcfg, joint,
 				ws, stream);
 			if (e != hipSuccess) return e;
-			// The LSD count + scan plan as the fallback: a full histogram read and a second plan,
-			// both gated on need_full, then the launches below (inactive unless planned). Onesweep
-			// would take fewer launches, but its look-back makes it far slower at the sizes this
-			// plan runs at (2^28 f32 in [0, 1): 3.2 ms per pass against about 1 ms).
-			algo = kAlgoCountScan;
+			// The LSD plan as the fallback: a full histogram read and a second plan, both gated on
+			// need_full, then the launches below (inactive unless planned). Count + scan passes,
+			// except for 8-byte float keys, whose first read already counted every digit: there the
+			// onesweep passes, which need no count read, are ahead (f64 from N(0, 1), 2^26 / 2^28:
+			// 3.21 / 12.2 ms against 3.87 / 14.0), while for 4-byte keys the two are within 6%.
+			algo = (P == 8 && float_keys) ? kAlgoOnesweep : kAlgoCountScan;
 			const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, 8u * cus));
 			hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS>), dim3(grid), dim3(256), 0, ls, static_cast<const K *>(job.in), n,
 				xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0, ctrl, 0u, &ctrl->need_full);

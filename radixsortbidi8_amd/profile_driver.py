@@ -1,7 +1,8 @@
 """A short, fixed workload for rocprofv3 runs (kernel traces and counter passes).
 
   python -m radixsortbidi8_amd.profile_driver [--n-log2 28] [--dtype u64] [--reps 2] [--algo A]
-                                               [--tile-config C] [--copy]
+                                               [--tile-config C] [--copy] [--normal] [--payload]
+                                               [--mode M] [--direction D]
                                                [--segmented [--small-cfg S] [--sweep]]
 
 Sorts 2^n seeded uniform keys (the benchmark's distribution) reps times after one warm-up sort,
@@ -23,6 +24,12 @@ def main():
     ap.add_argument("--algo", type=int, default=0)
     ap.add_argument("--tile-config", type=int, default=-1)
     ap.add_argument("--copy", action="store_true")
+    ap.add_argument("--normal", action="store_true",
+                    help="f32/f64 keys from N(0,1) instead of uniform bit patterns")
+    ap.add_argument("--payload", action="store_true",
+                    help="stable key sort carrying the u32 source index (rsbd8_hip_sort_pairs)")
+    ap.add_argument("--mode", type=int, default=64, help="sortingmode (64 native, 65 absolute value, ...)")
+    ap.add_argument("--direction", type=int, default=0, help="sortingdirection (0 ascfwdorder ... 3 dscrevorder)")
     ap.add_argument("--segmented", action="store_true",
                     help="instead: 65,536 int32 segments, log-uniform lengths scaled to 2^n keys in total")
     ap.add_argument("--small-cfg", type=int, default=-1, help="segmented: LDS tier of the plan (-1 automatic)")
@@ -39,18 +46,32 @@ def main():
 
     n = 1 << args.n_log2
     gen = torch.Generator(device="cuda").manual_seed(0x5EED0001)
-    keys = pl.make_keys(args.dtype, n, "uniform", gen)
+    if args.normal and args.dtype in ("f32", "f64"):
+        keys = torch.randn(n, dtype=pl.torch_dtype(args.dtype), device="cuda", generator=gen)
+    else:
+        keys = pl.make_keys(args.dtype, n, "uniform", gen)
     out = torch.empty_like(keys)
     buf = torch.empty_like(keys)
-    ws = r.Workspace(r.keytype_of(keys.dtype), n)
+    kt = r.keytype_of(keys.dtype)
+    ws = r.Workspace(kt, n, r.LAYOUT_PAIRS if args.payload else r.LAYOUT_KEYS)
     r.set_algorithm(args.algo)
     if args.tile_config >= 0:
         r.set_tile_config(args.tile_config)
-    r.radixsortcopynoalloc(keys, out, buf, ws)
+    if args.payload:
+        p_out = torch.empty(n, dtype=torch.int32, device="cuda")
+        p_buf = torch.empty_like(p_out)
+
+        def sort():
+            r.sort_pairs(keys, out, buf, None, p_out, p_buf, ws, args.direction, args.mode)
+    else:
+        def sort():
+            r.radixsortcopynoalloc(keys, out, buf, ws, args.direction, args.mode)
+    what = f"2^{args.n_log2} {args.dtype}{' N(0,1)' if args.normal else ''}{' + u32 index payload' if args.payload else ''}"
+    sort()
     torch.cuda.synchronize()
-    print(f"profile_driver: warm-up sort of 2^{args.n_log2} {args.dtype} done", flush=True)
+    print(f"profile_driver: warm-up sort of {what} (mode {args.mode}, direction {args.direction}) done", flush=True)
     for i in range(args.reps):
-        r.radixsortcopynoalloc(keys, out, buf, ws)
+        sort()
         torch.cuda.synchronize()
         print(f"profile_driver: sort {i + 1}/{args.reps} done", flush=True)
     if args.copy:
