@@ -92,23 +92,31 @@ def cpu_baseline(sample_log2: int, budget_s: float):
                       f"median of {len(times)}, checked sorted and a permutation (multiset sums)"}
 
 
-def learned_vs_fixed(requests: int, seed: int):
-    """The committed selector (radixsortbidi8_amd/tuning/) against the committed fixed plan table on a
-    seeded request mix of 2^14-2^26 keys (six dtypes, six distributions): completion time of every
-    sort from an idle device, the learned one including its selection round trip above the size
-    gate; every output compared with the fixed plan's."""
+def learned_vs_fixed(requests: int, seed: int, reps: int):
+    """The committed selector (radixsortbidi8_amd/tuning/) against the committed fixed plan table on
+    the seeded mixed workload: request_stream(requests, seed, 10, 29), sizes log-uniform in
+    2^10-2^29 keys, six dtypes, six distributions. Every request is timed reps times with each plan
+    (alternately, from an idle device; the learned one including its selection round trip above
+    the size gate); the totals are sums of per-request medians, and every output is compared with
+    the fixed plan's. runs: the speed-up of each timing round on its own."""
     from radixsortbidi8_amd import planner as pl
     model = os.path.join(pl.TUNING_DIR, "selector_model.npz")
     if not os.path.exists(model):
         return None
     sel = pl.Selector.load(model)
     table = pl.load_table()
-    res = pl.evaluate(pl.request_stream(requests, seed, 14, 26), sel, table)
-    return {"requests": res["requests"], "fixed_ms": round(res["fixed_ms"], 3),
+
+    def log(msg):
+        print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+    res = pl.evaluate(pl.request_stream(requests, seed, 10, 29), sel, table, log=log, reps=reps)
+    sp = [r["speedup"] for r in res["runs"]]
+    return {"requests": res["requests"], "reps": reps, "fixed_ms": round(res["fixed_ms"], 3),
             "learned_ms": round(res["learned_ms"], 3),
             "speedup": round(res["fixed_ms"] / max(res["learned_ms"], 1e-9), 4),
+            "speedup_runs": [round(x, 4) for x in sp], "speedup_spread": round(max(sp) - min(sp), 4),
             "mismatches": res["mismatches"], "gate_min_count": sel.min_count,
-            "stream": f"planner.request_stream({requests}, {seed:#x}, 14, 26)"}
+            "stream": f"planner.request_stream({requests}, {seed:#x}, 10, 29)"}
 
 
 def traffic_from_profile(n_log2: int, family: str):
@@ -133,6 +141,9 @@ def main():
     ap.add_argument("--cpu-budget-s", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-selector", action="store_true", help="skip the learned-vs-fixed plan comparison")
+    ap.add_argument("--selector-requests", type=int, default=4096,
+                    help="requests of the mixed workload (4096: the whole stream the metric is stated on)")
+    ap.add_argument("--selector-reps", type=int, default=3, help="timings per request and plan")
     ap.add_argument("--tile-config", type=int, default=-1)
     ap.add_argument("--algo", type=int, default=0, help="0 automatic, 1 onesweep, 2 count + scan, 4 MSD plan")
     args = ap.parse_args()
@@ -292,7 +303,7 @@ def main():
         print("bench: learned vs fixed plan", file=sys.stderr, flush=True)
         del keys, out, buf, ws
         torch.cuda.empty_cache()
-        line["learned_vs_fixed"] = learned_vs_fixed(64, 0x5EED0006)
+        line["learned_vs_fixed"] = learned_vs_fixed(args.selector_requests, 0x5EED0006, args.selector_reps)
     if not args.no_cpu_baseline:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(args.cpu_sample_log2, args.cpu_budget_s)

@@ -562,17 +562,20 @@ def regret(requests, X, T, M, selector: Selector, table: dict):
     return tot
 
 
-def evaluate(requests, selector: Selector, table: dict, verify: bool = True, log=None, max_n=None):
+def evaluate(requests, selector: Selector, table: dict, verify: bool = True, log=None, max_n=None, reps: int = 1):
     """Sorts every request with the fixed and the learned plan. Returns the total completion times
-    (time_call: from an idle device to the end of the sort, in ms) and checks that both outputs are
+    (time_call: from an idle device to the end of the sort, in ms; with reps > 1 the median of reps
+    timings per request, the two plans timed alternately) and checks that both outputs are
     bit-identical. The learned time is that of the whole rsbd8_hip_radixsortcopynoalloc_learned
     call: sampling, network and the host round trip of the choice (above the size gate), then the
-    sort."""
+    sort. runs holds the totals of each of the reps timing rounds on its own (the run-to-run
+    spread of the speed-up)."""
     torch = _torch()
     max_n = max_n or max(r[1] for r in requests)
     arena = Arena(max_n)
     out2 = torch.empty(max_n * 8, dtype=torch.uint8, device="cuda")
     tot_fixed = tot_learned = 0.0
+    run_fixed, run_learned = [0.0] * reps, [0.0] * reps
     chosen = {}
     mismatches = 0
     for i, (dt, n, dist, seed) in enumerate(requests):
@@ -582,24 +585,33 @@ def evaluate(requests, selector: Selector, table: dict, verify: bool = True, log
         out, buf = arena.views(keys)
         nb = keys.element_size()
         pf = fixed_plan(table, nb, n)
-        sort_with_plan(keys, out, buf, arena.ws, pf, flags)
-        tf = time_call(lambda: sort_with_plan(keys, out, buf, arena.ws, pf, flags), 1)
+
+        def fixed():
+            sort_with_plan(keys, out, buf, arena.ws, pf, flags)
 
         def learned():
             return selector.sort(keys, out2[: n * nb].view(keys.dtype), buf, arena.ws, flags, below=pf)
 
+        fixed()
         idx = learned()
-        tl = time_call(learned, 1)
+        tf, tl = [], []
+        for k in range(reps):
+            tf.append(time_call(fixed, 1))
+            tl.append(time_call(learned, 1))
+            run_fixed[k] += tf[-1]
+            run_learned[k] += tl[-1]
+        fixed()
         idx = learned()
         key = PLANS[idx] if idx >= 0 else "below gate (fixed plan)"
         chosen[key] = chosen.get(key, 0) + 1
         if verify and not torch.equal(out.view(torch.uint8)[: n * nb], out2[: n * nb]):
             mismatches += 1
-        tot_fixed += tf
-        tot_learned += tl
+        tot_fixed += float(np.median(tf))
+        tot_learned += float(np.median(tl))
         if log and i % 256 == 0:
             log(f"eval {i}/{len(requests)} fixed {tot_fixed:.1f} ms learned {tot_learned:.1f} ms")
         del keys
-    return {"requests": len(requests), "fixed_ms": tot_fixed, "learned_ms": tot_learned,
-            "speedup": tot_fixed / max(tot_learned, 1e-9), "mismatches": mismatches,
+    runs = [{"fixed_ms": f, "learned_ms": l, "speedup": f / max(l, 1e-9)} for f, l in zip(run_fixed, run_learned)]
+    return {"requests": len(requests), "reps": reps, "fixed_ms": tot_fixed, "learned_ms": tot_learned,
+            "speedup": tot_fixed / max(tot_learned, 1e-9), "mismatches": mismatches, "runs": runs,
             "chosen": {str(k): v for k, v in sorted(chosen.items(), key=lambda kv: -kv[1])}}

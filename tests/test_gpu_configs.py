@@ -265,9 +265,14 @@ def test_c5_selector_stream_4096_requests():
         o2 = out2[: n * nb].view(keys.dtype)
         pf = pl.fixed_plan(table, nb, n)
         pl.sort_with_plan(keys, o1, buf, arena.ws, pf, flags)
-        t_fixed += pl.time_call(lambda: pl.sort_with_plan(keys, o1, buf, arena.ws, pf, flags), 1)
         sel.sort(keys, o2, buf, arena.ws, flags, below=pf)
-        t_learned += pl.time_call(lambda: sel.sort(keys, o2, buf, arena.ws, flags, below=pf), 1)
+        # the median of three completion times per request and plan, the plans timed alternately
+        tf, tl = [], []
+        for _ in range(3):
+            tf.append(pl.time_call(lambda: pl.sort_with_plan(keys, o1, buf, arena.ws, pf, flags), 1))
+            tl.append(pl.time_call(lambda: sel.sort(keys, o2, buf, arena.ws, flags, below=pf), 1))
+        t_fixed += float(np.median(tf))
+        t_learned += float(np.median(tl))
         assert torch.equal(o1.view(torch.uint8), o2.view(torch.uint8)), (i, dt, n, dist)
         assert torch.equal(pl._sortable(o1), torch.sort(pl._sortable(keys)).values), (i, dt, n, dist)
         if n <= (1 << 16) and i % 4 == 0:
@@ -278,4 +283,4 @@ def test_c5_selector_stream_4096_requests():
     say(f"C5: fixed {t_fixed:.1f} ms, learned {t_learned:.1f} ms, speedup {t_fixed / t_learned:.4f}, "
         f"{checked} requests against the oracle")
     assert checked > 50
-    assert t_learned <= 1.10 * t_fixed, (t_fixed, t_learned)
+    assert t_learned < t_fixed, (t_fixed, t_learned)
