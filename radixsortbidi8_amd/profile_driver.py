@@ -121,6 +121,12 @@ def segmented(args, torch, r):
             print(f"profile_driver: segmented {name}: {ms:.3f} ms per batch, {total / ms / 1e6:.2f} GKeys/s "
                   f"({nseg} segments, {total} keys; tile_cfg {tc} small_cfg {sc})", flush=True)
         plan.close()
+    if args.copy:
+        for i in range(args.reps):
+            buf.copy_(keys)
+            torch.cuda.synchronize()
+            print(f"profile_driver: copy {i + 1}/{args.reps} of {keys.numel() * keys.element_size()} bytes done",
+                  flush=True)
     return 0
 
 
