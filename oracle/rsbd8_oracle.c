@@ -19,6 +19,13 @@
  *   rsbd8o_mergesort    -- the same known answers and numpy.argsort(kind="stable").
  *   rsbd8o_sort_indirect, rsbd8o_sort_segmented, rsbd8o_histogram8, rsbd8o_radixsort_mt --
  *                          equality with rsbd8o_radixsort / numpy on the same inputs.
+ *   rsbd8o_sortkey_wide, rsbd8o_radixsort_wide (keys wider than 64 bits; tests/test_oracle_wide.py) --
+ *                          the known answers of multiplatform/main.cpp:21-178 (long double in 10-,
+ *                          12- and 16-byte records, the 128-bit test records, ascending and
+ *                          dscrevorder, direct and through pointers); numpy.longdouble ordering of
+ *                          canonical 80-bit patterns (zeros of both signs, subnormals, normals,
+ *                          infinities) in every float mode; Python integer ordering of 128-bit
+ *                          patterns in every integer mode; padding bytes never reaching the key.
  *
  * Citations: H = Radixsortbidi8.hpp in the reference repository.
  */
