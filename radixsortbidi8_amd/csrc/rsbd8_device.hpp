@@ -74,6 +74,51 @@ __device__ __forceinline__ uint32_t digit_of(K raw, const KeyXform &t, uint32_t 
 	return static_cast<uint32_t>(sort_key<K>(raw, t) >> shift) & 0xFFu;
 }
 
+// A splitter table of the sample partition plan staged in LDS: 256 sorted bounds (unused ones
+// all-ones), per bound index t the open bucket id (bits 0-7) and whether bound t has an equality
+// bucket (bit 8, id + 1), and a cell table over [lo, hi] (the lowest and highest real bound): cell
+// c covers the keys lo + (c << sh) ... lo + ((c + 1) << sh) - 1 and holds the number of bounds below
+// its first key (bits 0-7) and how many more lie below its last key (bits 8-15).
+constexpr uint32_t kSplitCells = 4096;
+template <typename W>
+struct SplitLds {
+	W b[256];
+	uint32_t info[256];
+	uint16_t cell[kSplitCells];
+	W lo, hi;
+	uint32_t sh;
+};
+
+// Bucket id of the transformed key k: t = the number of bounds below k (0 at or below lo, all of
+// them above hi, else from the key's cell, refined by a binary search over the few bounds that
+// fall inside the cell), then the open bucket below bound t, or bound t's equality bucket when k
+// equals it. Ids grow with the key.
+template <typename W>
+__device__ __forceinline__ uint32_t split_id(const SplitLds<W> &tb, W k)
+{
+	uint32_t t;
+	if (k <= tb.lo) {
+		t = 0;
+	} else if (k > tb.hi) {
+		t = tb.info[255] >> 16;  // the number of real bounds
+	} else {
+		const uint32_t e = tb.cell[static_cast<uint32_t>((k - tb.lo) >> tb.sh)];
+		t = e & 0xFFu;
+		uint32_t span = e >> 8;
+		while (span) {
+			const uint32_t half = span >> 1;
+			if (tb.b[t + half] < k) {
+				t += half + 1;
+				span -= half + 1;
+			} else {
+				span = half;
+			}
+		}
+	}
+	const uint32_t inf = tb.info[t];
+	return (inf & 0xFFu) + (((inf >> 8) & 1u) & static_cast<uint32_t>(tb.b[t] == k));
+}
+
 // lanes of the wave holding the same 8-bit digit (wave64 ballot match)
 __device__ __forceinline__ uint64_t match_digit(uint32_t d)
 {
@@ -375,6 +420,52 @@ __device__ __forceinline__ void for_each_key(const K *__restrict__ p, uint64_t l
 		}
 	}
 	for (uint32_t i = nvec * VEC + threadIdx.x; i < body; i += THREADS) f(qa[i]);
+}
+
+// Counts the splitter bucket id of every key of the physical range [lo, lo + len) of p into s_t
+// (C counter copies), reading as for_each_key does; the ids of all the vectors a thread holds are
+// searched before any is counted, so that the searches' LDS loads overlap instead of each waiting
+// behind the previous key's counter update.
+template <typename K, int THREADS, int C>
+__device__ __forceinline__ void count_split_ids(const K *__restrict__ p, uint64_t lo, uint32_t len, const KeyXform &xf,
+	const SplitLds<typename KeyTraits<K>::W> &tb, uint32_t *s_t)
+{
+	using W = typename KeyTraits<K>::W;
+	constexpr uint32_t VEC = 16 / sizeof(K);
+	constexpr int INFLIGHT = 8;
+	const K *q = p + lo;
+	const uint32_t mis = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(q) & 15u) / static_cast<uint32_t>(sizeof(K));
+	uint32_t head = mis ? VEC - mis : 0u;
+	if (head > len) head = len;
+	if (threadIdx.x < head) wave_count<C>(s_t, split_id<W>(tb, sort_key<K>(q[threadIdx.x], xf)));
+	const K *qa = q + head;
+	const uint32_t body = len - head;
+	const uint32_t nvec = body / VEC;
+	const uint4 *qv = reinterpret_cast<const uint4 *>(qa);
+	for (uint32_t v = threadIdx.x; v < nvec; v += THREADS * INFLIGHT) {
+		uint4 u[INFLIGHT];
+#pragma unroll
+		for (int jj = 0; jj < INFLIGHT; ++jj) {
+			const uint32_t idx = v + jj * THREADS;
+			u[jj] = idx < nvec ? qv[idx] : make_uint4(0, 0, 0, 0);
+		}
+		uint32_t id[INFLIGHT][VEC];
+#pragma unroll
+		for (int jj = 0; jj < INFLIGHT; ++jj) {
+			K ks[VEC];
+			__builtin_memcpy(ks, &u[jj], 16);
+#pragma unroll
+			for (uint32_t e = 0; e < VEC; ++e) id[jj][e] = split_id<W>(tb, sort_key<K>(ks[e], xf));
+		}
+#pragma unroll
+		for (int jj = 0; jj < INFLIGHT; ++jj) {
+			if (v + jj * THREADS >= nvec) break;
+#pragma unroll
+			for (uint32_t e = 0; e < VEC; ++e) wave_count<C>(s_t, id[jj][e]);
+		}
+	}
+	for (uint32_t i = nvec * VEC + threadIdx.x; i < body; i += THREADS)
+		wave_count<C>(s_t, split_id<W>(tb, sort_key<K>(qa[i], xf)));
 }
 
 } // namespace rsbd8hip

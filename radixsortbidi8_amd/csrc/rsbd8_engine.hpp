@@ -55,6 +55,7 @@ struct Control {
 	uint32_t hyb_over;     // some bucket exceeds the LDS capacity: use the LSD plan
 	uint32_t hyb_queue;    // buckets queued for the LDS LSD passes (long runs of one third digit)
 	uint32_t joint_max;    // largest bucket of the top-16-bit histogram (the exact MSD plan test)
+	uint32_t bucket_count; // sample partition plan: non-empty buckets queued for the per-bucket sort
 };
 
 constexpr size_t kCtrlBytes = 4096;
@@ -69,14 +70,28 @@ constexpr uint32_t kHybBuckets = kRadix * kRadix;  // buckets after the two MSD 
 // between sorts), and the top-digit counts of the first partition's key ranges, then its cursors
 // (8 x 256)
 constexpr size_t kSegTableWords = 257 + 256 + 256 + 5 * kHybBuckets + 8 * 256;
+// Splitter tables of the sample partition plan (algorithm 4 for keys it cannot bucket by bits):
+// one for the first partition and one per first-partition segment for the second. A table maps a
+// transformed key to a bucket id by the number of its bounds below the key (see split_id):
+// bounds as 256 u64 (unused ones all-ones), 256 info words (open bucket id in bits 0-7, bit 8: the
+// bound has an equality bucket of its own, whose id is the open id + 1), the number of real bounds,
+// the number of ids, an 8-word mask of the equality buckets, then the cell table that finds a key's
+// place among the bounds: the lowest and highest bound (u64 each), the cell shift, and
+// kSplitCells u16 cells (see SplitLds).
+constexpr uint32_t kSplitBoundsWord = 0, kSplitInfoWord = 512, kSplitCountWord = 768, kSplitIdsWord = 769,
+	kSplitEqWord = 770, kSplitLoWord = 784, kSplitHiWord = 786, kSplitShWord = 788, kSplitCellWord = 792;
+constexpr size_t kSplitTabWords = kSplitCellWord + kSplitCells / 2 + 8;
+constexpr size_t kSplitTables = 1 + kRadix;
 static_assert(sizeof(Control) <= kCtrlBytes, "control block too large");
 
-// Workspace layout: [control][histograms][bucket starts][MSD tables][tile records ...]. Tile record
+// Workspace layout: [control][histograms][bucket starts][MSD tables][splitter tables][tile records ...]. Tile record
 // t holds the 256 look-back words of tile (or scan chunk) t followed by the 256 digit counts of
 // tile t. Every region sits at the same address for every sort on a workspace, whatever its size,
 // so a look-back word is only ever overwritten by another (epoch-tagged) look-back word: counts,
 // tables or keys left by a sort of another size can never pass for a published one.
-constexpr size_t kHeaderBytes = kCtrlBytes + 2 * kHistBytes + ((kSegTableWords * 4 + 255) / 256) * 256;
+constexpr size_t kSegTableBytes = ((kSegTableWords * 4 + 255) / 256) * 256;
+constexpr size_t kSplitTableBytes = ((kSplitTables * kSplitTabWords * 4 + 255) / 256) * 256;
+constexpr size_t kHeaderBytes = kCtrlBytes + 2 * kHistBytes + kSegTableBytes + kSplitTableBytes;
 constexpr size_t kRecordBytes = kRadix * (sizeof(uint64_t) + sizeof(uint32_t));
 constexpr size_t kLbStride = kRecordBytes / sizeof(uint64_t);     // u64 words from one tile's look-back row to the next
 constexpr size_t kCountStride = kRecordBytes / sizeof(uint32_t);  // u32 words from one tile's count row to the next

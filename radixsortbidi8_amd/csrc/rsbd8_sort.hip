@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "rsbd8_engine.hpp"
@@ -65,6 +66,33 @@ struct SegTables {
 	uint32_t *ovf;
 	uint32_t *rcnt;
 };
+// the splitter tables (kSplitTables of kSplitTabWords words each): the first partition's, then one
+// per first-partition segment
+inline uint32_t *splittab_of(void *ws)
+{
+	return reinterpret_cast<uint32_t *>(static_cast<char *>(ws) + kCtrlBytes + 2 * kHistBytes + kSegTableBytes);
+}
+
+// stages a splitter table in LDS (bounds narrowed to the transformed key's width); every thread of
+// the block takes part
+template <typename W, int THREADS>
+__device__ __forceinline__ void split_load(const uint32_t *__restrict__ tab, SplitLds<W> &tb)
+{
+	const uint64_t *b = reinterpret_cast<const uint64_t *>(tab + kSplitBoundsWord);
+	for (int i = threadIdx.x; i < kRadix; i += THREADS) {
+		tb.b[i] = static_cast<W>(b[i]);
+		tb.info[i] = tab[kSplitInfoWord + i];
+	}
+	const uint32_t *c = tab + kSplitCellWord;
+	uint32_t *dc = reinterpret_cast<uint32_t *>(tb.cell);
+	for (int i = threadIdx.x; i < static_cast<int>(kSplitCells / 2); i += THREADS) dc[i] = c[i];
+	if (threadIdx.x == 0) {
+		tb.lo = static_cast<W>(*reinterpret_cast<const uint64_t *>(tab + kSplitLoWord));
+		tb.hi = static_cast<W>(*reinterpret_cast<const uint64_t *>(tab + kSplitHiWord));
+		tb.sh = tab[kSplitShWord];
+	}
+}
+
 __host__ __device__ inline SegTables seg_tables(uint32_t *base)
 {
 	uint32_t *b = base + 257 + 512;
@@ -892,11 +920,18 @@ __device__ __forceinline__ void seg_tile(const uint32_t *s_ts, const SegMap &m, 
 // invisible after the per-bucket sort: with SEG the cursors of the tile's segment (positions
 // relative to it), else those of the tile's key range (range x holds the tiles x * per ..
 // x * per + per - 1; absolute positions).
-template <typename K, bool PAY, int THREADS, int KPT, bool SEG = false, bool GROUP = false, bool CUR = false>
+// With SPL the digit is not a byte of the sort key but its bucket id under a splitter table (the
+// sample partition plan): SPL 1, the table at split for every tile; SPL 2 (with SEG), the table of
+// the tile's segment, split + segment * kSplitTabWords. The digits are staged beside the keys, so
+// the stores do not search again.
+template <typename K, bool PAY, int THREADS, int KPT, bool SEG = false, bool GROUP = false, bool CUR = false, int SPL = 0>
 __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32_t shift, KeyXform xf, uint64_t n,
-	uint32_t num_tiles, const uint32_t *__restrict__ offsets, SegMap seg, uint32_t *__restrict__ cursor)
+	uint32_t num_tiles, const uint32_t *__restrict__ offsets, SegMap seg, uint32_t *__restrict__ cursor,
+	const uint32_t *__restrict__ split)
 {
 	static_assert(!CUR || GROUP, "cursors are for the grouping passes");
+	static_assert(!(SPL && GROUP) && (SPL != 2 || SEG), "splitter digits: ranked passes; per-segment tables on segment tiles");
+	using W = typename KeyTraits<K>::W;
 	constexpr int TILE = THREADS * KPT;
 	constexpr int NW = THREADS / 64;
 	const PassPlan pp = *ppp;
@@ -908,7 +943,8 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 	__shared__ uint32_t s_glob[kRadix];
 	__shared__ uint32_t s_wsum[4];
 	__shared__ uint32_t s_ts[SEG ? kRadix + 1 : 1];
-	__shared__ uint8_t s_dig[PAY ? TILE : 1];
+	__shared__ uint8_t s_dig[(PAY || SPL) ? TILE : 1];
+	__shared__ std::conditional_t<SPL != 0, SplitLds<W>, uint32_t> s_tab;  // the splitter table (SPL only)
 	constexpr int kStage = (PAY && sizeof(K) < 4) ? 4 : static_cast<int>(sizeof(K));
 	// the tile staging, and before it the ranking's peer words
 	__shared__ __align__(16) unsigned char s_raw[GROUP ? TILE * kStage : max_bytes(TILE * kStage, rank_peer_bytes(THREADS))];
@@ -922,6 +958,11 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 		for (int i = t; i <= kRadix; i += THREADS) s_ts[i] = seg.tile_start[i];
 		__syncthreads();
 		real_tiles = s_ts[kRadix];
+	}
+	uint32_t tab_seg = 0xFFFFFFFFu;  // segment whose table is staged (SPL 2)
+	if constexpr (SPL == 1) {
+		split_load<W, THREADS>(split, s_tab);
+		__syncthreads();
 	}
 	auto geom = [&](uint32_t tl, uint64_t &base, uint32_t &valid, uint32_t &sgm) {
 		if constexpr (SEG) {
@@ -961,10 +1002,19 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 	}
 	while (k < per && tile < real_tiles) {
 		uint32_t dig[KPT], pos[KPT], total;
+		if constexpr (SPL == 2) {
+			if (sgm != tab_seg) {  // uniform across the block
+				__syncthreads();
+				split_load<W, THREADS>(split + static_cast<size_t>(sgm) * kSplitTabWords, s_tab);
+				__syncthreads();
+				tab_seg = sgm;
+			}
+		}
 #pragma unroll
 		for (int j = 0; j < KPT; ++j) {
 			const uint32_t l = w * 64u * KPT + j * 64u + lane;
-			dig[j] = l < valid ? digit_of<K>(keys[j], xf, shift) : 0xFFu;
+			if constexpr (SPL) dig[j] = l < valid ? split_id<W>(s_tab, sort_key<K>(keys[j], xf)) : 0xFFu;
+			else dig[j] = l < valid ? digit_of<K>(keys[j], xf, shift) : 0xFFu;
 		}
 		if constexpr (GROUP) {
 			static_assert(!PAY, "grouping without order is for keys only");
@@ -993,6 +1043,10 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 			if (t < kRadix) s_glob[t] = offsets[static_cast<uint64_t>(tile) * kCountStride + t] - s_digoff[t];
 #pragma unroll
 			for (int j = 0; j < KPT; ++j) s_keys[pos[j]] = keys[j];
+			if constexpr (SPL) {
+#pragma unroll
+				for (int j = 0; j < KPT; ++j) s_dig[pos[j]] = static_cast<uint8_t>(dig[j]);
+			}
 		}
 		uint32_t ppos[PAY ? KPT : 1];
 		if constexpr (PAY) {
@@ -1015,8 +1069,13 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 		__syncthreads();
 		for (uint32_t i = t; i < valid; i += THREADS) {
 			const K kk = s_keys[i];
-			const uint32_t dd = digit_of<K>(kk, xf, shift);
-			if constexpr (PAY) s_dig[i] = static_cast<uint8_t>(dd);
+			uint32_t dd;
+			if constexpr (SPL) {
+				dd = s_dig[i];
+			} else {
+				dd = digit_of<K>(kk, xf, shift);
+				if constexpr (PAY) s_dig[i] = static_cast<uint8_t>(dd);
+			}
 			dst[s_glob[dd] + i] = kk;
 		}
 		if constexpr (PAY) {
@@ -1040,27 +1099,40 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 // two-level MSD plan: segmented count and scan of the second digit, fallback switch, and the
 // per-bucket LDS sort of the remaining digits
 
-// per-tile counts of the second digit over the tiles of the segment map
-template <typename K, int THREADS>
+// per-tile counts of the second digit over the tiles of the segment map (SPL: the bucket id under
+// the tile's segment table, split + segment * kSplitTabWords)
+template <typename K, int THREADS, bool SPL = false>
 __global__ __launch_bounds__(THREADS) void k_hyb_count(const PassPlan *ppp, uint32_t shift, KeyXform xf, uint32_t tile_keys,
-	SegMap seg, uint32_t *__restrict__ counts)
+	SegMap seg, uint32_t *__restrict__ counts, const uint32_t *__restrict__ split)
 {
+	using W = typename KeyTraits<K>::W;
 	const PassPlan pp = *ppp;
 	if (!pp.active) return;
 	const K *__restrict__ src = static_cast<const K *>(pp.src);
 	constexpr int C = count_copies(1);
 	__shared__ uint32_t s_ts[kRadix + 1];
 	__shared__ uint32_t s_t[kRadix * C];
+	__shared__ std::conditional_t<SPL != 0, SplitLds<W>, uint32_t> s_tab;  // the splitter table (SPL only)
 	for (int i = threadIdx.x; i <= kRadix; i += THREADS) s_ts[i] = seg.tile_start[i];
 	__syncthreads();
 	const uint32_t num = s_ts[kRadix];
+	uint32_t tab_seg = 0xFFFFFFFFu;
 	for (uint32_t tile = blockIdx.x; tile < num; tile += gridDim.x) {
 		for (int i = threadIdx.x; i < kRadix * C; i += THREADS) s_t[i] = 0;
-		__syncthreads();
 		uint64_t base;
 		uint32_t len, sg;
 		seg_tile(s_ts, seg, tile_keys, tile, base, len, sg);
-		for_each_key<K, THREADS>(src, base, len, [&](K x) { wave_count<C>(s_t, digit_of<K>(x, xf, shift)); });
+		if constexpr (SPL) {
+			if (sg != tab_seg) {  // uniform across the block; the last reads of the old table ended at a barrier
+				split_load<W, THREADS>(split + static_cast<size_t>(sg) * kSplitTabWords, s_tab);
+				tab_seg = sg;
+			}
+		}
+		__syncthreads();
+		if constexpr (SPL)
+			count_split_ids<K, THREADS, C>(src, base, len, xf, s_tab, s_t);
+		else
+			for_each_key<K, THREADS>(src, base, len, [&](K x) { wave_count<C>(s_t, digit_of<K>(x, xf, shift)); });
 		__syncthreads();
 		for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kCountStride + i] = copies_sum<C>(s_t, i);
 		__syncthreads();
@@ -1069,11 +1141,14 @@ __global__ __launch_bounds__(THREADS) void k_hyb_count(const PassPlan *ppp, uint
 
 // One block per top-digit segment, one thread per second digit: the bucket totals, their
 // exclusive offsets inside the segment, the bucket table, and the tile counts turned into global
-// output offsets. A bucket larger than cap marks the plan for the LSD fallback.
-__global__ __launch_bounds__(256) void k_hyb_segscan(Control *ctrl, SegTables T, uint32_t *__restrict__ counts, uint32_t cap)
+// output offsets. A bucket larger than cap marks the plan for the LSD fallback. With bq, the
+// non-empty buckets are also queued there (ctrl->bucket_count of them; one atomic per block).
+__global__ __launch_bounds__(256) void k_hyb_segscan(Control *ctrl, SegTables T, uint32_t *__restrict__ counts, uint32_t cap,
+	uint32_t *__restrict__ bq)
 {
 	if (!ctrl->hyb[1].active) return;
 	__shared__ uint32_t s_wave[4];
+	__shared__ uint32_t s_qw[4], s_qbase;
 	const uint32_t sg = blockIdx.x, d = threadIdx.x, lane = d & 63u, w = d >> 6;
 	const uint32_t t0 = T.tile_start[sg], t1 = T.tile_start[sg + 1];
 	uint32_t tot = 0;
@@ -1093,6 +1168,19 @@ __global__ __launch_bounds__(256) void k_hyb_segscan(Control *ctrl, SegTables T,
 	T.bucket_start[sg * kRadix + d] = before + incl - tot;
 	T.bucket_len[sg * kRadix + d] = tot;
 	if (tot > cap) atomicOr(&ctrl->hyb_over, 1u);
+	if (bq) {
+		const uint64_t nz = __ballot(tot != 0);
+		if (lane == 0) s_qw[w] = static_cast<uint32_t>(__popcll(nz));
+		__syncthreads();
+		if (d == 0) {
+			const uint32_t all = s_qw[0] + s_qw[1] + s_qw[2] + s_qw[3];
+			s_qbase = all ? atomicAdd(&ctrl->bucket_count, all) : 0u;
+		}
+		__syncthreads();
+		uint32_t at = s_qbase + lanes_below(nz);
+		for (uint32_t v = 0; v < w; ++v) at += s_qw[v];
+		if (tot) bq[at] = sg * kRadix + d;
+	}
 	for (uint32_t tl = t0; tl < t1; tl += 8) {
 		uint32_t c[8];
 #pragma unroll
@@ -1502,6 +1590,7 @@ __device__ __forceinline__ void reset_control(Control *ctrl)
 		ctrl->key_or = 0;
 		ctrl->key_nor = 0;
 		ctrl->joint_max = 0;
+		ctrl->bucket_count = 0;
 	}
 }
 
@@ -1630,14 +1719,14 @@ hipError_t launch_pass_cfg(int cfg, Control *ctrl, const uint32_t *bucket, uint6
 	}
 }
 
-template <typename K, bool PAY, int THREADS, int KPT, bool SEG = false, bool GROUP = false, bool CUR = false>
+template <typename K, bool PAY, int THREADS, int KPT, bool SEG = false, bool GROUP = false, bool CUR = false, int SPL = 0>
 hipError_t launch_scatter(const PassPlan *pp, uint32_t shift, const KeyXform &xf, uint64_t n, uint32_t tiles,
-	const uint32_t *offsets, SegMap seg, hipStream_t stream, uint32_t *cursor = nullptr)
+	const uint32_t *offsets, SegMap seg, hipStream_t stream, uint32_t *cursor = nullptr, const uint32_t *split = nullptr)
 {
 	static int occ = -1;
 	if (occ < 0) {
 		int o = 0;
-		if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_scatter<K, PAY, THREADS, KPT, SEG, GROUP, CUR>, THREADS, 0) !=
+		if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_scatter<K, PAY, THREADS, KPT, SEG, GROUP, CUR, SPL>, THREADS, 0) !=
 				hipSuccess || o < 1)
 			o = 1;
 		occ = o;
@@ -1645,8 +1734,8 @@ hipError_t launch_scatter(const PassPlan *pp, uint32_t shift, const KeyXform &xf
 	const uint32_t per = (tiles + 7) / 8;
 	const uint32_t want = static_cast<uint32_t>(occ * devinfo().cus * 2 + 7) / 8;
 	const uint32_t grid = 8u * std::max<uint32_t>(1u, std::min<uint32_t>(per, want));
-	hipLaunchKernelGGL((k_scatter<K, PAY, THREADS, KPT, SEG, GROUP, CUR>), dim3(grid), dim3(THREADS), 0, stream, pp, shift, xf,
-		n, tiles, offsets, seg, cursor);
+	hipLaunchKernelGGL((k_scatter<K, PAY, THREADS, KPT, SEG, GROUP, CUR, SPL>), dim3(grid), dim3(THREADS), 0, stream, pp, shift, xf,
+		n, tiles, offsets, seg, cursor, split);
 	return hipGetLastError();
 }
 
@@ -1770,9 +1859,9 @@ hipError_t launch_hybrid(Control *ctrl, const uint32_t *bucket, uint32_t *counts
 		ProfScope ps(7, 1, ws, stream);
 		const uint32_t cgrid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles2, 8u * cus));
 		hipLaunchKernelGGL((k_hyb_count<K, 256>), dim3(cgrid), dim3(256), 0, stream, &ctrl->hyb[1], 8u * (P - 2), xf, TILE, m,
-			counts);
+			counts, nullptr);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
-		hipLaunchKernelGGL(k_hyb_segscan, dim3(kRadix), dim3(256), 0, stream, ctrl, T, counts, kLocalCaps[lcfg]);
+		hipLaunchKernelGGL(k_hyb_segscan, dim3(kRadix), dim3(256), 0, stream, ctrl, T, counts, kLocalCaps[lcfg], nullptr);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 		hipLaunchKernelGGL(k_hyb_check, dim3(1), dim3(64), 0, stream, ctrl);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -1799,6 +1888,122 @@ hipError_t launch_hybrid_cfg(int cfg, Control *ctrl, const uint32_t *bucket, uin
 	case 4: return launch_hybrid<K, 256, 24>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, joint, ws, stream);
 	default: return launch_hybrid<K, 1024, 16>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, joint, ws, stream);
 	}
+}
+
+#include "rsbd8_sample.hpp"
+
+// Mean bucket length the sample partition plan aims its second partition at (half the capacity of
+// its per-bucket sort, so that the sampling's spread rarely sends a bucket through HBM).
+constexpr uint32_t kSampleTarget = 4096;
+
+// The sample partition plan (rsbd8_sample.hpp) with THREADS x KPT partition tiles.
+template <typename K, bool PAY, int THREADS, int KPT>
+hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, hipStream_t stream)
+{
+	constexpr uint32_t TILE = THREADS * KPT;
+	const uint64_t n = job.n;
+	Control *ctrl = ctrl_of(ws);
+	uint32_t *hist = hist_of(ws), *bucket = bucket_of(ws), *counts = counts_of(ws), *split = splittab_of(ws);
+	uint64_t *lookback = lookback_of(ws);
+	const SegTables T = seg_tables(segtab_of(ws));
+	const int cus = devinfo().cus;
+	const int rev = (job.flags & kRev) ? 1 : 0;
+	const uint32_t tiles = static_cast<uint32_t>((n + TILE - 1) / TILE);
+	const K *in = static_cast<const K *>(job.in);
+	hipError_t e;
+	if (job.presort_check) {
+		const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 4095) / 4096, 4u * cus)));
+		hipLaunchKernelGGL((k_check<K, 256>), dim3(grid), dim3(256), 0, stream, in, n, xf, ctrl);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+	}
+	{
+		ProfScope ps(0, 0, ws, stream);
+		// chunks of at least kSampleMax keys each, a power of two (every chunk keeps kSampleMax / G samples)
+		uint32_t G = 1;
+		while (G < kL1Chunks && static_cast<uint64_t>(2 * G) * kSampleMax <= n) G *= 2;
+		hipLaunchKernelGGL((k_samp_chunk<K, 1024>), dim3(G), dim3(1024), 0, stream, in, n, xf, ctrl, rev, split, nullptr);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+		hipLaunchKernelGGL((k_samp_l1<K, 1024>), dim3(1), dim3(1024), 0, stream, ctrl, rev, split, nullptr);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+		const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, 4u * cus));
+		hipLaunchKernelGGL((k_split_count1<K, 256>), dim3(grid), dim3(256), 0, stream, in, n, xf, hist, counts, TILE, tiles, rev,
+			ctrl, split, nullptr);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+	}
+	PlanArgs pa{};
+	pa.n = n; pa.passes = static_cast<int>(sizeof(K)); pa.src = job.in; pa.fin = job.out; pa.other = job.buf;
+	pa.psrc = job.payload ? job.pin : nullptr;
+	pa.pfin = job.payload ? job.pout : nullptr;
+	pa.pother = job.payload ? job.pbuf : nullptr;
+	pa.payload = job.payload;
+	pa.reverse = rev;
+	pa.keys_final_any = job.keys_final_any;
+	pa.tile2 = TILE;
+	pa.segtab = segtab_of(ws);
+	pa.gate = nullptr;
+	hipLaunchKernelGGL(k_plan_sample, dim3(1), dim3(256), 0, stream, pa, ctrl, hist, bucket);
+	if ((e = hipGetLastError()) != hipSuccess) return e;
+	{
+		ProfScope ps(7, 0, ws, stream);
+		const uint32_t nchunks = (tiles + kScanChunk - 1) / kScanChunk;
+		const uint32_t sgrid = std::max<uint32_t>(1u, std::min<uint32_t>(nchunks, 4u * cus));
+		hipLaunchKernelGGL(k_scan, dim3(sgrid), dim3(256), 0, stream, ctrl, &ctrl->hyb[0], &ctrl->hyb_scan_counter, bucket,
+			kMaxPasses, counts, lookback, tiles);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+	}
+	{
+		ProfScope ps(5, 0, ws, stream);
+		e = launch_scatter<K, PAY, THREADS, KPT, false, false, false, 1>(&ctrl->hyb[0], 0, xf, n, tiles, counts, SegMap{}, stream,
+			nullptr, split);
+		if (e != hipSuccess) return e;
+	}
+	const uint32_t tiles2 = tiles + kRadix;  // bound: one partial tile per segment
+	const SegMap m{T.tile_start, T.seg_start, T.seg_len};
+	{
+		ProfScope ps(7, 1, ws, stream);
+		hipLaunchKernelGGL((k_samp_l2<K, 1024>), dim3(kRadix), dim3(1024), 0, stream, ctrl, xf, T, split, kSampleTarget);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+		const uint32_t cgrid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles2, 8u * cus));
+		hipLaunchKernelGGL((k_hyb_count<K, 256, true>), dim3(cgrid), dim3(256), 0, stream, &ctrl->hyb[1], 0u, xf, TILE, m, counts,
+			split + kSplitTabWords);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+		hipLaunchKernelGGL(k_hyb_segscan, dim3(kRadix), dim3(256), 0, stream, ctrl, T, counts, 0xFFFFFFFFu, T.queue);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+	}
+	{
+		ProfScope ps(5, 1, ws, stream);
+		e = launch_scatter<K, PAY, THREADS, KPT, true, false, false, 2>(&ctrl->hyb[1], 0, xf, n, tiles2, counts, m, stream,
+			nullptr, split + kSplitTabWords);
+		if (e != hipSuccess) return e;
+	}
+	{
+		ProfScope ps(6, 0, ws, stream);
+		// one block per queued bucket: a segment of len keys has at most 1.5 * ceil(len / target) + 1
+		// non-empty buckets, so 2 n / target + 512 blocks cover every queue (the rest exit at once)
+		const uint32_t bgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(std::min<uint64_t>(n, kHybBuckets),
+			2 * n / kSampleTarget + 512)));
+		hipLaunchKernelGGL((k_bucket_sort<K, PAY, 1024, 8, 11>), dim3(bgrid), dim3(1024), 0, stream, ctrl, xf, T, split);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+		hipLaunchKernelGGL((k_bucket_hard<K, PAY, 1024, 8>), dim3(2 * cus), dim3(1024), 0, stream, ctrl, xf, T);
+		if ((e = hipGetLastError()) != hipSuccess) return e;
+	}
+	const uint32_t cgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 255) / 256, 8u * cus)));
+	hipLaunchKernelGGL((k_copy<K>), dim3(cgrid), dim3(256), 0, stream, ctrl, 1, n, job.payload ? 1 : 0);
+	if ((e = hipGetLastError()) != hipSuccess) return e;
+	hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, stream, ctrl);
+	return hipGetLastError();
+}
+
+// The partitions take 512x8 tiles whatever the configuration: with the splitter table in LDS the
+// 1024x12 tiles of the LSD passes spill registers (measured at 2^28 f32 with a payload: 2.3 ms per
+// partition against 1.75).
+template <typename K>
+hipError_t launch_sample(int cfg, const SortJob &job, void *ws, hipStream_t stream)
+{
+	(void)cfg;
+	const KeyXform xf = make_xform(8 * sizeof(K), job.flags);
+	if (job.payload) return launch_sample_cfg<K, true, 512, 8>(job, ws, xf, stream);
+	return launch_sample_cfg<K, false, 512, 8>(job, ws, xf, stream);
 }
 
 // One side stream per device (default priority, not synchronising with the null stream) and the
@@ -1897,6 +2102,17 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	const int cfg = job.cfg >= 0 ? job.cfg : default_cfg(P, n, job.payload, float_keys);
 	const bool keys_only = !job.payload && !job.keys_final_any;
 	int algo = job.algo ? job.algo : default_algo(P, n, keys_only, float_keys);
+	if constexpr (P == 8) {
+		// The automatic plan takes the sample partition plan where it was measured ahead of the LSD
+		// passes (uniform and N(0,1) keys, MI355X): keys-only sorts of 8-byte floats or under a
+		// transform that is not injective, above the single-block size (f64 N(0,1) at 2^16 / 2^20 /
+		// 2^24 / 2^28: 0.197 / 0.317 / 0.84 / 8.7 ms against 0.207 / 0.336 / 1.03 / 14.0), and 8-byte
+		// keys with a payload from 2^27 (u64 + u32 index at 2^28: 9.7 ms against 19.1; at 2^24 the
+		// LSD passes are ahead, 1.31 against 1.41).
+		if (!job.algo && job.buf && n > kLdsMax &&
+			((keys_only && (float_keys || !xform_injective(xf, P))) || (job.payload && n >= (1ull << 27))))
+			return launch_sample<K>(cfg, job, ws, stream);
+	}
 	// small sorts with a payload: one LDS block as well (up to kLdsPayMax keys)
 	if (!job.algo && job.payload && n <= kLdsPayMax) algo = kAlgoLds;
 	if (algo == kAlgoLds) {
@@ -1931,10 +2147,20 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 		}
 		algo = default_algo(P, n, false, float_keys);
 	}
+	const bool inj_xf = xform_injective(xf, P);
+	const uint32_t joint_slices = static_cast<uint32_t>(std::min<uint64_t>(static_cast<uint64_t>(cus),
+		n * P / (kJointBins * sizeof(uint32_t)))) / kRanges * kRanges;  // the same number of blocks per key range
 	int lcfg = -1;
 	if (algo == kAlgoHybrid) {
 		lcfg = local_cfg_for(n);
 		if (P < 4 || !keys_only || !job.buf || lcfg < 0) lcfg = -1;
+		// The MSD plan by the top two bytes needs exact bucket sizes (integer keys under an
+		// injective transform, keys only); every other sort of 4- and 8-byte keys with a buffer
+		// takes the sample partition plan.
+		const bool exact = lcfg >= 0 && !float_keys && inj_xf && joint_slices >= kRanges;
+		if constexpr (P >= 4) {
+			if (!exact && job.buf) return launch_sample<K>(cfg, job, ws, stream);
+		}
 		if (lcfg < 0) algo = kAlgoCountScan;
 	}
 	const bool hybrid = lcfg >= 0;
@@ -1956,9 +2182,6 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	// The exact bucket sizes (k_hist_joint): integer keys under an injective transform. The read
 	// leaves one 65536-word slice per block in the array the first partition writes (X1, never the
 	// source), so the number of blocks is bounded by that array's size.
-	const bool inj_xf = xform_injective(xf, P);
-	const uint32_t joint_slices = static_cast<uint32_t>(std::min<uint64_t>(static_cast<uint64_t>(cus),
-		n * P / (kJointBins * sizeof(uint32_t)))) / kRanges * kRanges;  // the same number of blocks per key range
 	const bool joint = hybrid && P >= 4 && !float_keys && inj_xf && joint_slices >= kRanges;
 	uint32_t *joint_part = static_cast<uint32_t *>(job.buf != job.in ? job.buf : job.out);
 	if (job.presort_check && !fill) {

@@ -5,7 +5,9 @@ LDS sort per bucket) against the oracle, bit for bit, including the paths a skew
 * a block whose bucket has a run of more than 96 keys sharing the third digit sorts the rest of
   its digits with LDS LSD passes;
 * a bucket larger than the LDS capacity switches the whole sort back to the LSD plan after the
-  second count (the input is still intact then).
+  second count (the input is still intact then), for integer keys under an injective transform;
+* keys the plan cannot bucket by their top bytes (floats, the absolute-value modes) take the
+  sample partition plan instead, whose buckets come from splitters and so stay small.
 Stability is observed through the absolute-value modes, where distinct bit patterns rank equal.
 The profile kinds tell which plan ran.
 """
@@ -96,9 +98,10 @@ def test_msd_long_runs_take_the_block_lsd_path(nb, msd):
 
 @pytest.mark.parametrize("nb,kt,mode", [(4, r.U32, 64), (8, r.U64, 64), (4, r.F32, 64), (8, r.F64, 65)])
 def test_msd_oversized_bucket_falls_back(nb, kt, mode, msd):
-    """One bucket of 3000 keys with capacity 1024: the sort must switch back to the LSD plan (the
-    onesweep passes, from the intact input), for injective and non-injective transforms, with ±0.0
-    among the float keys so that tie order is observed."""
+    """One top-16-bit bucket of 3000 keys with capacity 1024: integer keys under an injective
+    transform switch back to the LSD plan (from the intact input); float keys take the sample
+    partition plan, whose splitters divide that bucket. ±0.0 among the float keys makes tie order
+    observable."""
     rng = np.random.default_rng(77 + nb)
     n = 1 << 20
     bits = spread(nb, n, rng)
@@ -113,13 +116,15 @@ def test_msd_oversized_bucket_falls_back(nb, kt, mode, msd):
         got, used = run(bits, kt, mode, d)
         exp, _ = oracle.radixsort(bits, r.decode(kt, mode, d))
         assert np.array_equal(got, exp), d
-        assert not used
+        assert used == (kt in (r.F32, r.F64))
 
 
 @pytest.mark.parametrize("nb", [4, 8])
 def test_msd_uniform_top_digit_keeps_lsd(nb, msd):
+    """Integer keys whose top digit is one value: the plan by the top two bytes is not taken (n is
+    large enough for its exact bucket sizes at both key widths)."""
     rng = np.random.default_rng(5 + nb)
-    n = 500000
+    n = 1 << 20
     bits = spread(nb, n, rng)
     bits &= UINT[nb]((1 << (8 * nb - 8)) - 1)
     bits |= UINT[nb](0x7F << (8 * nb - 8))
