@@ -178,11 +178,21 @@ int rsbd8_hip_set_tile_config(int cfg);
 /* Process-wide pass algorithm: 1 onesweep (each scatter pass resolves its tile offsets with a
  * decoupled look-back over the preceding tiles), 2 count + scan + scatter (per-tile digit counts,
  * a decoupled look-back chained scan of the count matrix, then a scatter with no inter-block
- * dependency), 4 two MSD partitions by the top two digits followed by one LDS sort per resulting
- * bucket (keys-only sorts of 4- and 8-byte keys; others use 2, and the device itself falls back to
- * 2 when the top digits are too skewed for the buckets to fit in LDS), 0 automatic. Returns the
- * previous setting. */
+ * dependency), 4 two MSD partitions followed by one LDS sort per resulting bucket, 0 automatic.
+ * Algorithm 4 partitions keys-only integer sorts under an injective transform (unsigned and
+ * signed modes) by their top two digits, with exact bucket sizes from one histogram read (the
+ * device falls back to 2 when those buckets are too large for LDS); every other sort of 4- and
+ * 8-byte keys with a buffer (floats, the absolute-value modes, sorts with a payload) is
+ * partitioned by splitters sampled from the keys; 1- and 2-byte keys use 2. Returns the previous
+ * setting. */
 int rsbd8_hip_set_algorithm(int algo);
+
+/* Process-wide mean bucket length the splitter-sampled partitions of algorithm 4 aim at (keys per
+ * bucket after the second partition; at most 256 buckets per first-partition segment, so large
+ * sorts may exceed it): 0 restores the default (4096). Buckets above the per-bucket LDS sort's
+ * capacity (8192 keys) are sorted through HBM, so a large target exercises that path. Returns the
+ * previous setting (0 when it was the default). No reference counterpart. */
+int rsbd8_hip_set_sample_target(int keys);
 
 /* A sort plan chosen per call (no reference counterpart; the reference picks its plan from
  * compile-time thresholds, H:12948-13152): algo 0 automatic, 1 onesweep, 2 count + scan + scatter,

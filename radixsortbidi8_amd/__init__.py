@@ -94,6 +94,7 @@ def _load():
         "rsbd8_hip_sort_pairs": ([ci, cu, sz, vp, sz, vp, vp, vp, vp, vp, vp, vp], ci),
         "rsbd8_hip_set_tile_config": ([ci], ci),
         "rsbd8_hip_set_algorithm": ([ci], ci),
+        "rsbd8_hip_set_sample_target": ([ci], ci),
         "rsbd8_hip_last_error": ([], ctypes.c_char_p),
         "rsbd8_hip_segmented_plan_create": ([ci, cu, sz, vp, ci, ci, ctypes.POINTER(vp)], ci),
         "rsbd8_hip_segmented_run": ([vp, vp, vp, vp, vp, vp], ci),
@@ -208,9 +209,19 @@ ALGO_AUTO, ALGO_ONESWEEP, ALGO_COUNTSCAN, ALGO_MSD2 = 0, 1, 2, 4
 
 def set_algorithm(algo: int) -> int:
     """Force the pass algorithm (ALGO_ONESWEEP, ALGO_COUNTSCAN, or ALGO_MSD2: two MSD partitions and
-    a per-bucket LDS sort for keys-only 4/8-byte sorts, others fall back to ALGO_COUNTSCAN) or
-    ALGO_AUTO; returns the previous."""
+    a per-bucket LDS sort for 4/8-byte keys, by the top two digits for keys-only integer sorts under
+    an injective transform, else by splitters sampled from the keys; 1/2-byte keys fall back to
+    ALGO_COUNTSCAN) or ALGO_AUTO; returns the previous."""
     prev = _load().rsbd8_hip_set_algorithm(int(algo))
+    if prev < 0:
+        _check(prev)
+    return prev
+
+
+def set_sample_target(keys: int) -> int:
+    """Mean bucket length the splitter-sampled partitions of ALGO_MSD2 aim at (0 restores the
+    default, 4096); returns the previous setting."""
+    prev = _load().rsbd8_hip_set_sample_target(int(keys))
     if prev < 0:
         _check(prev)
     return prev

@@ -334,6 +334,12 @@ int rsbd8_hip_set_algorithm(int algo)
 	return g_algo.exchange(algo);
 }
 
+int rsbd8_hip_set_sample_target(int keys)
+{
+	if (keys < 0) return fail(RSBD8_HIP_EINVAL, "sample target out of range");
+	return static_cast<int>(set_sample_target(static_cast<uint32_t>(keys)));
+}
+
 const char *rsbd8_hip_last_error(void) { return g_last_error.c_str(); }
 
 int rsbd8_hip_profile_begin(void)

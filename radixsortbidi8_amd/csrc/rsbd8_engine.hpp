@@ -112,8 +112,9 @@ struct SortJob {
 	bool keys_final_any = false;   // only the payload is wanted: keys may end in out or buf
 	int algo = 0;                  // 0 automatic, 1 onesweep (look-back in the scatter), 2 count + chained scan + scatter,
 	                               // 3 one block sorts everything in LDS (keys only, n <= kLdsMax),
-	                               // 4 two MSD partitions + per-bucket LDS sort (keys only, 4/8-byte keys;
-	                               //   the device falls back to algorithm 2 for skewed keys)
+	                               // 4 two MSD partitions + per-bucket LDS sort (4/8-byte keys: by the top two
+	                               //   digits for keys-only integer sorts under an injective transform, the
+	                               //   device falling back to algorithm 2 for skewed keys; else by splitters)
 	bool presort_check = false;    // first test whether the input is already in order (or strictly reversed)
 };
 
@@ -136,6 +137,8 @@ hipError_t workspace_init(void *ws, size_t bytes, hipStream_t stream);
 hipError_t workspace_reset(void *ws, hipStream_t stream);
 hipError_t sort_launch(const SortJob &job, void *ws, size_t ws_bytes, hipStream_t stream);
 int default_cfg(int key_bytes, uint64_t n, bool payload, bool float_keys);
+// the sample partition plan's mean bucket target (0: the default); returns the previous setting
+uint32_t set_sample_target(uint32_t keys);
 int default_algo(int key_bytes, uint64_t n, bool keys_only, bool float_keys);
 
 // Keys wider than 64 bits (rsbd8_wide.hip): the 80-bit extended float in 10-, 12- or 16-byte

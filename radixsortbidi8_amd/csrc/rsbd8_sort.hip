@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <mutex>
 #include <type_traits>
@@ -1893,8 +1894,10 @@ hipError_t launch_hybrid_cfg(int cfg, Control *ctrl, const uint32_t *bucket, uin
 #include "rsbd8_sample.hpp"
 
 // Mean bucket length the sample partition plan aims its second partition at (half the capacity of
-// its per-bucket sort, so that the sampling's spread rarely sends a bucket through HBM).
+// its per-bucket sort, so that the sampling's spread rarely sends a bucket through HBM), unless a
+// process-wide setting overrides it.
 constexpr uint32_t kSampleTarget = 4096;
+std::atomic<uint32_t> g_sample_target{0};
 
 // The sample partition plan (rsbd8_sample.hpp) with THREADS x KPT partition tiles.
 template <typename K, bool PAY, int THREADS, int KPT>
@@ -1910,6 +1913,8 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 	const int rev = (job.flags & kRev) ? 1 : 0;
 	const uint32_t tiles = static_cast<uint32_t>((n + TILE - 1) / TILE);
 	const K *in = static_cast<const K *>(job.in);
+	const uint32_t st = g_sample_target.load();
+	const uint32_t target = st ? st : kSampleTarget;
 	hipError_t e;
 	if (job.presort_check) {
 		const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 4095) / 4096, 4u * cus)));
@@ -1961,7 +1966,7 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 	const SegMap m{T.tile_start, T.seg_start, T.seg_len};
 	{
 		ProfScope ps(7, 1, ws, stream);
-		hipLaunchKernelGGL((k_samp_l2<K, 1024>), dim3(kRadix), dim3(1024), 0, stream, ctrl, xf, T, split, kSampleTarget);
+		hipLaunchKernelGGL((k_samp_l2<K, 1024>), dim3(kRadix), dim3(1024), 0, stream, ctrl, xf, T, split, target);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 		const uint32_t cgrid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles2, 8u * cus));
 		hipLaunchKernelGGL((k_hyb_count<K, 256, true>), dim3(cgrid), dim3(256), 0, stream, &ctrl->hyb[1], 0u, xf, TILE, m, counts,
@@ -1981,7 +1986,7 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 		// one block per queued bucket: a segment of len keys has at most 1.5 * ceil(len / target) + 1
 		// non-empty buckets, so 2 n / target + 512 blocks cover every queue (the rest exit at once)
 		const uint32_t bgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(std::min<uint64_t>(n, kHybBuckets),
-			2 * n / kSampleTarget + 512)));
+			2 * n / target + 512)));
 		hipLaunchKernelGGL((k_bucket_sort<K, PAY, 1024, 8, 11>), dim3(bgrid), dim3(1024), 0, stream, ctrl, xf, T, split);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 		hipLaunchKernelGGL((k_bucket_hard<K, PAY, 1024, 8>), dim3(2 * cus), dim3(1024), 0, stream, ctrl, xf, T);
@@ -2321,6 +2326,8 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 } // namespace
 
 int device_cu_count() { return devinfo().cus; }
+
+uint32_t set_sample_target(uint32_t keys) { return g_sample_target.exchange(keys); }
 
 void profile_begin()
 {

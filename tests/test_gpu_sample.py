@@ -1,0 +1,192 @@
+"""GPU parity of the sample partition plan (algorithm 4 for floats, the absolute-value modes and
+sorts with a payload: splitters sampled from the keys, two stable partitions, one LDS sort per
+bucket) against the oracle, bit for bit, keys and payload, through each of its paths:
+
+* equality buckets: values repeated often enough to take two candidate slots (±0.0, NaN, a few
+  integers at 5-20% of the keys) are kept in source order by the partitions alone;
+* the per-bucket LDS sort, with the position packed beside (key - min) in one word;
+* buckets whose key range leaves no room for the position, or with a run of more than 96 keys of
+  one partition value: stable LDS LSD passes (k_bucket_hard);
+* buckets above the LDS capacity (8192 keys), forced with a large mean-bucket target: stable LSD
+  passes through HBM, in place and into the buffer;
+* the presort check, in-place sorts and sorts into the buffer.
+Stability is observed through payloads and through modes where distinct bit patterns rank equal.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+r = pytest.importorskip("radixsortbidi8_amd")
+
+UINT = {4: np.uint32, 8: np.uint64}
+FLOAT = {4: np.float32, 8: np.float64}
+TORCH = {r.U32: torch.uint32, r.I32: torch.int32, r.F32: torch.float32, r.U64: torch.uint64, r.I64: torch.int64,
+         r.F64: torch.float64}
+
+
+@pytest.fixture
+def msd():
+    prev = r.set_algorithm(r.ALGO_MSD2)
+    yield
+    r.set_algorithm(prev)
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def heavy_floats(nb, n, rng):
+    """N(0,1) floats with ±0.0 (2%), ±inf (1% each), a quiet NaN (1%) and subnormals (1%)."""
+    bits = rng.normal(size=n).astype(FLOAT[nb]).view(UINT[nb])
+    hi = 8 * nb - 1
+    pats = [0, 1 << hi, (0x7F800000 if nb == 4 else 0x7FF0000000000000),
+            (0xFF800000 if nb == 4 else 0xFFF0000000000000), (0x7FC00000 if nb == 4 else 0x7FF8000000000000)]
+    u = rng.random(n)
+    for i, p in enumerate(pats):
+        bits[(u >= 0.01 * i) & (u < 0.01 * (i + 1))] = UINT[nb](p)
+    sub = (u >= 0.06) & (u < 0.07)
+    bits[sub] = rng.integers(1, 1 << 20, int(sub.sum()), dtype=np.uint64).astype(UINT[nb])
+    return bits
+
+
+def heavy_ints(nb, n, rng):
+    """Uniform integers, a fifth of them one value, a tenth another, 5% a third, some sign-flipped
+    twins (equal under the absolute-value modes)."""
+    bits = rng.integers(0, np.iinfo(UINT[nb]).max, n, dtype=UINT[nb], endpoint=True)
+    u = rng.random(n)
+    v = rng.integers(0, np.iinfo(UINT[nb]).max, 3, dtype=UINT[nb], endpoint=True)
+    bits[u < 0.2] = v[0]
+    bits[(u >= 0.2) & (u < 0.3)] = v[1]
+    bits[(u >= 0.3) & (u < 0.35)] = v[2]
+    k = rng.integers(0, n, n // 16)
+    h = k.size // 2
+    bits[k[:h]] = bits[k[h:2 * h]] ^ UINT[nb](1 << (8 * nb - 1))
+    return bits
+
+
+def sort_pairs(bits, kt, mode, d, payload):
+    """sort_pairs on the GPU; payload None: the source index. Returns (keys, payload)."""
+    k = dev(bits).view(TORCH[kt])
+    ko, kb = torch.empty_like(k), torch.empty_like(k)
+    po = torch.empty(bits.size, dtype=torch.uint32, device="cuda")
+    pb = torch.empty_like(po)
+    ws = r.Workspace(kt, bits.size, r.LAYOUT_PAIRS)
+    r.sort_pairs(k, ko, kb, None if payload is None else dev(payload), po, pb, ws, d, mode)
+    return host(ko).view(bits.dtype), host(po)
+
+
+def expected_pairs(bits, kt, mode, d, payload):
+    flags = r.decode(kt, mode, d, True)
+    if payload is None:
+        ek, perm = oracle.radixsort(bits, flags, want_payload=True)
+        return ek, perm.astype(np.uint32)
+    return oracle.radixsort(bits, flags, payload=payload)
+
+
+@pytest.mark.parametrize("nb,kt", [(4, r.F32), (8, r.F64), (4, r.I32), (8, r.U64)])
+def test_sample_heavy_values_with_payload(nb, kt, msd):
+    rng = np.random.default_rng(300 + nb + kt)
+    n = 300007
+    bits = heavy_floats(nb, n, rng) if kt in (r.F32, r.F64) else heavy_ints(nb, n, rng)
+    payload = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    for mode in (64, 65, 66, 1):
+        for d in range(4):
+            for p in (None, payload):
+                got = sort_pairs(bits, kt, mode, d, p)
+                exp = expected_pairs(bits, kt, mode, d, p)
+                assert np.array_equal(got[0], exp[0]), (mode, d, p is None)
+                assert np.array_equal(got[1], exp[1]), (mode, d, p is None)
+
+
+@pytest.mark.parametrize("nb,kt", [(4, r.F32), (8, r.F64), (4, r.I32), (8, r.I64)])
+def test_sample_keys_only_all_directions(nb, kt, msd):
+    """Keys-only sorts in every mode, uniform bit patterns and skewed values; the profile shows the
+    per-bucket sort ran."""
+    rng = np.random.default_rng(400 + nb + kt)
+    for n in (1 << 20, 123457):
+        for bits in (rng.integers(0, np.iinfo(UINT[nb]).max, n, dtype=UINT[nb], endpoint=True),
+                     heavy_floats(nb, n, rng) if kt in (r.F32, r.F64) else heavy_ints(nb, n, rng)):
+            a = dev(bits).view(TORCH[kt])
+            o, b = torch.empty_like(a), torch.empty_like(a)
+            ws = r.Workspace(kt, n)
+            for mode in (64, 65, 66):
+                for d in range(4):
+                    r.profile_begin()
+                    r.radixsortcopynoalloc(a, o, b, ws, d, mode)
+                    prof = r.profile_end()
+                    exp, _ = oracle.radixsort(bits, r.decode(kt, mode, d))
+                    assert np.array_equal(host(o).view(bits.dtype), exp), (n, mode, d)
+                    if kt in (r.F32, r.F64) or mode != 64:  # (integer native modes: the plan by top bytes)
+                        assert prof["msd_local"][1] > 0, (n, mode, d)
+
+
+def test_sample_wide_ranges_and_long_runs(msd):
+    """u32 keys at 2^20: one bucket per first-partition segment, key ranges of 2^24 (no room for the
+    position in a 32-bit word), so every bucket takes the LDS LSD passes; then 300 values of about
+    870 copies each: runs far above 96 keys in buckets of a narrow range."""
+    rng = np.random.default_rng(11)
+    n = 1 << 20
+    cases = [rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)]
+    pool = rng.integers(0, 1 << 32, 300, dtype=np.uint64).astype(np.uint32)
+    cases.append(pool[rng.integers(0, 300, n >> 2)])
+    for bits in cases:
+        payload = rng.integers(0, 1 << 32, bits.size, dtype=np.uint64).astype(np.uint32)
+        for d in range(4):
+            got = sort_pairs(bits, r.U32, 64, d, payload)
+            exp = expected_pairs(bits, r.U32, 64, d, payload)
+            assert np.array_equal(got[0], exp[0]) and np.array_equal(got[1], exp[1]), d
+
+
+@pytest.mark.parametrize("nb,kt,mode", [(4, r.U32, 64), (8, r.F64, 64), (4, r.F32, 65), (8, r.I64, 65)])
+def test_sample_buckets_through_hbm(nb, kt, mode, msd):
+    """A mean-bucket target of 2^20 keys leaves one bucket per first-partition segment, about 16K
+    keys each at 2^22: every bucket exceeds the block's 8192 and takes the LSD passes through HBM
+    (in place: the other partition array as the second buffer; copying: the destination)."""
+    prev = r.set_sample_target(1 << 20)
+    try:
+        rng = np.random.default_rng(500 + nb + kt)
+        n = (1 << 22) + 1234
+        bits = heavy_floats(nb, n, rng) if kt in (r.F32, r.F64) else heavy_ints(nb, n, rng)
+        payload = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+        for d in (0, 3):
+            got = sort_pairs(bits, kt, mode, d, payload)
+            exp = expected_pairs(bits, kt, mode, d, payload)
+            assert np.array_equal(got[0], exp[0]) and np.array_equal(got[1], exp[1]), d
+            exp_k, _ = oracle.radixsort(bits, r.decode(kt, mode, d))
+            ws = r.Workspace(kt, n)
+            for move in (False, True):
+                a = dev(bits).view(TORCH[kt])
+                b = torch.empty_like(a)
+                r.radixsortnoalloc(a, b, ws, move, d, mode)
+                assert np.array_equal(host(b if move else a).view(bits.dtype), exp_k), (d, move)
+    finally:
+        r.set_sample_target(prev)
+
+
+def test_sample_target_rejects_negative():
+    with pytest.raises(r.Rsbd8Error):
+        r.set_sample_target(-1)
+
+
+def test_sample_presort_check(msd):
+    """The presort check of a plan: float keys already in order or strictly reversed become one copy;
+    others are sorted."""
+    from radixsortbidi8_amd import planner
+    rng = np.random.default_rng(21)
+    n = 400009
+    srt = np.sort(rng.normal(size=n).astype(np.float32))
+    ws = r.Workspace(r.F32, n)
+    for keys in (srt, srt[::-1].copy(), heavy_floats(4, n, rng).view(np.float32)):
+        a = dev(keys)
+        o, b = torch.empty_like(a), torch.empty_like(a)
+        planner.sort_with_plan(a, o, b, ws, (4, 2, 1), r.decode(r.F32, 64, 0))
+        exp, _ = oracle.radixsort(keys.view(np.uint32), r.decode(r.F32, 64, 0))
+        assert np.array_equal(host(o).view(np.uint32), exp)
