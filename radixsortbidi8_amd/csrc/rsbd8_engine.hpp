@@ -55,7 +55,8 @@ struct Control {
 	uint32_t hyb_over;     // some bucket exceeds the LDS capacity: use the LSD plan
 	uint32_t hyb_queue;    // buckets queued for the LDS LSD passes (long runs of one third digit)
 	uint32_t joint_max;    // largest bucket of the top-16-bit histogram (the exact MSD plan test)
-	uint32_t bucket_count; // sample partition plan: non-empty buckets queued for the per-bucket sort
+	uint32_t bucket_count; // sample partition plan: buckets queued for the smaller per-bucket sort
+	uint32_t bucket_count2;// ... and for the larger one
 };
 
 constexpr size_t kCtrlBytes = 4096;
@@ -67,9 +68,9 @@ constexpr uint32_t kHybBuckets = kRadix * kRadix;  // buckets after the two MSD 
 // (257 words), segment starts and lengths (256 + 256), bucket starts (relative to their segment)
 // and lengths (2 x 65536), the queue of buckets for the LDS LSD passes (65536), the bucket cursors
 // of the second partition (65536), the overflow counts of the top-16-bit histogram (65536, zero
-// between sorts), and the top-digit counts of the first partition's key ranges, then its cursors
-// (8 x 256)
-constexpr size_t kSegTableWords = 257 + 256 + 256 + 5 * kHybBuckets + 8 * 256;
+// between sorts), the top-digit counts of the first partition's key ranges, then its cursors
+// (8 x 256), and a second bucket queue (65536: the sample partition plan's larger buckets)
+constexpr size_t kSegTableWords = 257 + 256 + 256 + 5 * kHybBuckets + 8 * 256 + kHybBuckets;
 // Splitter tables of the sample partition plan (algorithm 4 for keys it cannot bucket by bits):
 // one for the first partition and one per first-partition segment for the second. A table maps a
 // transformed key to a bucket id by the number of its bounds below the key (see split_id):

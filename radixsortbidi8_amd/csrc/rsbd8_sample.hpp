@@ -29,6 +29,11 @@
 
 constexpr uint32_t kSampleMax = 4096;      // sample keys per splitter table
 constexpr uint32_t kBucketRunMax = 96;     // longest run of one partition value the ranking takes
+// With 8-byte keys, buckets of at most kBucketCapA keys take a 256-thread per-bucket sort (three
+// blocks per CU), larger ones the 1024-thread one (8192 keys, one block per CU at that width);
+// modelled at 2^28 keys with the sampling below, about 80% of the buckets fit the smaller one.
+// 4-byte keys take the 1024-thread one only (two blocks per CU).
+constexpr uint32_t kBucketCapA = 256 * 20;
 
 // seeded position in [lo, lo + width) for sample i (a 64-bit mix of i and the salt)
 __device__ __forceinline__ uint64_t sample_pos(uint64_t lo, uint64_t width, uint32_t i, uint32_t salt)
@@ -430,8 +435,8 @@ __device__ void bucket_lsd_global(K *a, K *b, K *dst, uint32_t *pa, uint32_t *pb
 	}
 }
 
-// The per-bucket sort of the sample partition plan: one block per queued non-empty bucket (sg =
-// entry >> 8, id = entry & 255; blocks past the queue exit at once). A bucket's keys (and payload)
+// The per-bucket sort of the sample partition plan: one block per bucket of its queue (sg = entry
+// >> 8, id = entry & 255; blocks past the queue's *queued entries exit at once). A bucket's keys (and payload)
 // are read whole into registers before anything is written, so in place is fine. Each key becomes
 // the word ((sort key - min) << LB) | l, l < CAP = 2^LB its position in the bucket: distinct words
 // whose order is the stable order. They are partitioned by the top PB bits of the key range with
@@ -441,7 +446,7 @@ __device__ void bucket_lsd_global(K *a, K *b, K *dst, uint32_t *pa, uint32_t *pb
 // position in a word, or with a run of more than kBucketRunMax keys is queued for k_bucket_hard.
 template <typename K, bool PAY, int THREADS, int KPT, int PB>
 __global__ __launch_bounds__(THREADS) void k_bucket_sort(Control *ctrl, KeyXform xf, SegTables T,
-	const uint32_t *__restrict__ split)
+	const uint32_t *__restrict__ split, const uint32_t *__restrict__ queue, const uint32_t *queued)
 {
 	using W = typename KeyTraits<K>::W;
 	constexpr uint32_t CAP = THREADS * KPT;
@@ -460,8 +465,8 @@ __global__ __launch_bounds__(THREADS) void k_bucket_sort(Control *ctrl, KeyXform
 	__shared__ W s_min[NW], s_max[NW];
 	__shared__ uint32_t s_big;
 	const PassPlan pp = ctrl->hyb[2];
-	if (!pp.active || blockIdx.x >= ctrl->bucket_count) return;
-	const uint32_t bk = T.queue[blockIdx.x], sg = bk >> 8, id = bk & 255u;
+	if (!pp.active || blockIdx.x >= *queued) return;
+	const uint32_t bk = queue[blockIdx.x], sg = bk >> 8, id = bk & 255u;
 	const uint32_t len = T.bucket_len[bk];
 	const uint32_t start = T.seg_start[sg] + T.bucket_start[bk];
 	const K *src = static_cast<const K *>(pp.src) + start;
@@ -583,7 +588,7 @@ __global__ __launch_bounds__(THREADS) void k_bucket_sort(Control *ctrl, KeyXform
 			const uint32_t lo = s_off[d], hi = s_off[d + 1];
 			uint32_t n_less = 0;
 			for (uint32_t q = lo; q < hi; ++q) n_less += static_cast<uint32_t>(s_t[q] < x);
-			fin[j] = ((static_cast<uint32_t>(x) & (CAP - 1u)) << 16) | (lo + n_less);
+			fin[j] = ((static_cast<uint32_t>(x) & ((1u << LB) - 1u)) << 16) | (lo + n_less);
 		}
 	}
 	__syncthreads();  // every run has been read: s_u takes the final positions

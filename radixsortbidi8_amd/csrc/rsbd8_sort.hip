@@ -55,7 +55,7 @@ constexpr uint32_t kRanges = 8;           // key ranges of the MSD plan's first 
 
 // The MSD plan's tables (see kSegTableWords): tile_start[257], seg_start[256], seg_len[256],
 // bucket_start[65536] (relative to the bucket's segment), bucket_len[65536], queue[65536],
-// cursor[65536], ovf[65536], rcnt[8 * 256].
+// cursor[65536], ovf[65536], rcnt[8 * 256], queue2[65536].
 struct SegTables {
 	uint32_t *tile_start;
 	uint32_t *seg_start;
@@ -66,6 +66,7 @@ struct SegTables {
 	uint32_t *cursor;
 	uint32_t *ovf;
 	uint32_t *rcnt;
+	uint32_t *queue2;
 };
 // the splitter tables (kSplitTables of kSplitTabWords words each): the first partition's, then one
 // per first-partition segment
@@ -98,7 +99,7 @@ __host__ __device__ inline SegTables seg_tables(uint32_t *base)
 {
 	uint32_t *b = base + 257 + 512;
 	return SegTables{base, base + 257, base + 257 + 256, b, b + kHybBuckets, b + 2 * kHybBuckets, b + 3 * kHybBuckets,
-		b + 4 * kHybBuckets, b + 5 * kHybBuckets};
+		b + 4 * kHybBuckets, b + 5 * kHybBuckets, b + 5 * kHybBuckets + 8 * kRadix};
 }
 
 // 0: a real sort is needed; 1: the output is the input as it is; 2: the output is the input
@@ -1142,14 +1143,16 @@ __global__ __launch_bounds__(THREADS) void k_hyb_count(const PassPlan *ppp, uint
 
 // One block per top-digit segment, one thread per second digit: the bucket totals, their
 // exclusive offsets inside the segment, the bucket table, and the tile counts turned into global
-// output offsets. A bucket larger than cap marks the plan for the LSD fallback. With bq, the
-// non-empty buckets are also queued there (ctrl->bucket_count of them; one atomic per block).
+// output offsets. A bucket larger than cap marks the plan for the LSD fallback. With queues (the
+// sample partition plan), the non-empty buckets are also queued: those of at most qcap keys in
+// T.queue (ctrl->bucket_count of them), the others in T.queue2 (ctrl->bucket_count2); one atomic
+// per block and queue.
 __global__ __launch_bounds__(256) void k_hyb_segscan(Control *ctrl, SegTables T, uint32_t *__restrict__ counts, uint32_t cap,
-	uint32_t *__restrict__ bq)
+	int queues, uint32_t qcap)
 {
 	if (!ctrl->hyb[1].active) return;
 	__shared__ uint32_t s_wave[4];
-	__shared__ uint32_t s_qw[4], s_qbase;
+	__shared__ uint32_t s_qw[2][4], s_qbase[2];
 	const uint32_t sg = blockIdx.x, d = threadIdx.x, lane = d & 63u, w = d >> 6;
 	const uint32_t t0 = T.tile_start[sg], t1 = T.tile_start[sg + 1];
 	uint32_t tot = 0;
@@ -1169,18 +1172,22 @@ __global__ __launch_bounds__(256) void k_hyb_segscan(Control *ctrl, SegTables T,
 	T.bucket_start[sg * kRadix + d] = before + incl - tot;
 	T.bucket_len[sg * kRadix + d] = tot;
 	if (tot > cap) atomicOr(&ctrl->hyb_over, 1u);
-	if (bq) {
-		const uint64_t nz = __ballot(tot != 0);
-		if (lane == 0) s_qw[w] = static_cast<uint32_t>(__popcll(nz));
-		__syncthreads();
-		if (d == 0) {
-			const uint32_t all = s_qw[0] + s_qw[1] + s_qw[2] + s_qw[3];
-			s_qbase = all ? atomicAdd(&ctrl->bucket_count, all) : 0u;
+	if (queues) {
+		const int q = tot > qcap ? 1 : 0;
+		const uint64_t m0 = __ballot(tot != 0 && q == 0), m1 = __ballot(q == 1);
+		if (lane == 0) {
+			s_qw[0][w] = static_cast<uint32_t>(__popcll(m0));
+			s_qw[1][w] = static_cast<uint32_t>(__popcll(m1));
 		}
 		__syncthreads();
-		uint32_t at = s_qbase + lanes_below(nz);
-		for (uint32_t v = 0; v < w; ++v) at += s_qw[v];
-		if (tot) bq[at] = sg * kRadix + d;
+		if (d < 2) {
+			const uint32_t all = s_qw[d][0] + s_qw[d][1] + s_qw[d][2] + s_qw[d][3];
+			s_qbase[d] = all ? atomicAdd(d ? &ctrl->bucket_count2 : &ctrl->bucket_count, all) : 0u;
+		}
+		__syncthreads();
+		uint32_t at = s_qbase[q] + lanes_below(q ? m1 : m0);
+		for (uint32_t v = 0; v < w; ++v) at += s_qw[q][v];
+		if (tot) (q ? T.queue2 : T.queue)[at] = sg * kRadix + d;
 	}
 	for (uint32_t tl = t0; tl < t1; tl += 8) {
 		uint32_t c[8];
@@ -1592,6 +1599,7 @@ __device__ __forceinline__ void reset_control(Control *ctrl)
 		ctrl->key_nor = 0;
 		ctrl->joint_max = 0;
 		ctrl->bucket_count = 0;
+		ctrl->bucket_count2 = 0;
 	}
 }
 
@@ -1862,7 +1870,7 @@ hipError_t launch_hybrid(Control *ctrl, const uint32_t *bucket, uint32_t *counts
 		hipLaunchKernelGGL((k_hyb_count<K, 256>), dim3(cgrid), dim3(256), 0, stream, &ctrl->hyb[1], 8u * (P - 2), xf, TILE, m,
 			counts, nullptr);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
-		hipLaunchKernelGGL(k_hyb_segscan, dim3(kRadix), dim3(256), 0, stream, ctrl, T, counts, kLocalCaps[lcfg], nullptr);
+		hipLaunchKernelGGL(k_hyb_segscan, dim3(kRadix), dim3(256), 0, stream, ctrl, T, counts, kLocalCaps[lcfg], 0, 0u);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 		hipLaunchKernelGGL(k_hyb_check, dim3(1), dim3(64), 0, stream, ctrl);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -1915,6 +1923,11 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 	const K *in = static_cast<const K *>(job.in);
 	const uint32_t st = g_sample_target.load();
 	const uint32_t target = st ? st : kSampleTarget;
+	// Two per-bucket sorts (kBucketCapA keys in 256-thread blocks, then the rest) for 8-byte keys,
+	// whose 8192-key blocks hold one per CU: 2^28 f64 N(0,1) 2.97 ms in the bucket sorts against
+	// 3.63 with the large one only; 4-byte keys keep the large one (2^28 f32 + index: 1.89 ms
+	// against 2.17 with both).
+	constexpr bool kTiers = sizeof(K) == 8;
 	hipError_t e;
 	if (job.presort_check) {
 		const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 4095) / 4096, 4u * cus)));
@@ -1972,7 +1985,7 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 		hipLaunchKernelGGL((k_hyb_count<K, 256, true>), dim3(cgrid), dim3(256), 0, stream, &ctrl->hyb[1], 0u, xf, TILE, m, counts,
 			split + kSplitTabWords);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
-		hipLaunchKernelGGL(k_hyb_segscan, dim3(kRadix), dim3(256), 0, stream, ctrl, T, counts, 0xFFFFFFFFu, T.queue);
+		hipLaunchKernelGGL(k_hyb_segscan, dim3(kRadix), dim3(256), 0, stream, ctrl, T, counts, 0xFFFFFFFFu, 1, kTiers ? kBucketCapA : 0u);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 	}
 	{
@@ -1984,10 +1997,19 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 	{
 		ProfScope ps(6, 0, ws, stream);
 		// one block per queued bucket: a segment of len keys has at most 1.5 * ceil(len / target) + 1
-		// non-empty buckets, so 2 n / target + 512 blocks cover every queue (the rest exit at once)
+		// non-empty buckets, so 2 n / target + 512 blocks cover the queue of the smaller ones, and
+		// n / kBucketCapA + 1 the queue of the larger ones (the blocks past a queue exit at once)
+		if constexpr (kTiers) {
+			const uint32_t agrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(std::min<uint64_t>(n, kHybBuckets),
+				2 * n / target + 512)));
+			hipLaunchKernelGGL((k_bucket_sort<K, PAY, 256, kBucketCapA / 256, 10>), dim3(agrid), dim3(256), 0, stream, ctrl, xf, T,
+				split, T.queue, &ctrl->bucket_count);
+			if ((e = hipGetLastError()) != hipSuccess) return e;
+		}
 		const uint32_t bgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(std::min<uint64_t>(n, kHybBuckets),
-			2 * n / target + 512)));
-		hipLaunchKernelGGL((k_bucket_sort<K, PAY, 1024, 8, 11>), dim3(bgrid), dim3(1024), 0, stream, ctrl, xf, T, split);
+			kTiers ? n / kBucketCapA + 1 : 2 * n / target + 512)));
+		hipLaunchKernelGGL((k_bucket_sort<K, PAY, 1024, 8, 11>), dim3(bgrid), dim3(1024), 0, stream, ctrl, xf, T, split, T.queue2,
+			&ctrl->bucket_count2);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 		hipLaunchKernelGGL((k_bucket_hard<K, PAY, 1024, 8>), dim3(2 * cus), dim3(1024), 0, stream, ctrl, xf, T);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -2107,16 +2129,22 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	const int cfg = job.cfg >= 0 ? job.cfg : default_cfg(P, n, job.payload, float_keys);
 	const bool keys_only = !job.payload && !job.keys_final_any;
 	int algo = job.algo ? job.algo : default_algo(P, n, keys_only, float_keys);
-	if constexpr (P == 8) {
-		// The automatic plan takes the sample partition plan where it was measured ahead of the LSD
-		// passes (uniform and N(0,1) keys, MI355X): keys-only sorts of 8-byte floats or under a
-		// transform that is not injective, above the single-block size (f64 N(0,1) at 2^16 / 2^20 /
-		// 2^24 / 2^28: 0.197 / 0.317 / 0.84 / 8.7 ms against 0.207 / 0.336 / 1.03 / 14.0), and 8-byte
-		// keys with a payload from 2^27 (u64 + u32 index at 2^28: 9.7 ms against 19.1; at 2^24 the
-		// LSD passes are ahead, 1.31 against 1.41).
-		if (!job.algo && job.buf && n > kLdsMax &&
-			((keys_only && (float_keys || !xform_injective(xf, P))) || (job.payload && n >= (1ull << 27))))
-			return launch_sample<K>(cfg, job, ws, stream);
+	if (!job.algo) {
+		// The automatic plan and the sample partition plan (measured on MI355X, uniform and N(0,1)
+		// keys, against the LSD passes): 8-byte keys take it from 2^24 when they are floats (f64
+		// N(0,1) keys only at 2^24 / 2^26 / 2^28: 0.81 / 2.25 / 8.1 ms against 1.03 / 3.85 / 14.0;
+		// with a u32 index: 0.92 / 2.57 against 1.30 / 5.17) or keys-only under a transform that is
+		// not injective, and from 2^25 for integers with a payload (u64 + index at 2^25 / 2^27 /
+		// 2^28: 1.77 / 4.48 / 9.2 ms against 2.68 / 9.67 / 19.1; at 2^24 the wide bucket ranges of
+		// uniform 64-bit keys keep it behind, 1.38 against 1.30). 4-byte keys keep the LSD passes
+		// (f32 N(0,1) + index at 2^28: 7.5 ms against 6.45; keys only 6.1 against 5.0).
+		const bool inj = xform_injective(xf, P);
+		if constexpr (P == 8) {
+			if (job.buf && ((keys_only && (float_keys || !inj)) || job.payload) &&
+				n >= (float_keys || keys_only ? (1ull << 24) : (1ull << 25)))
+				return launch_sample<K>(cfg, job, ws, stream);
+		}
+		if (P == 4 && algo == kAlgoHybrid && (float_keys || !inj)) algo = kAlgoCountScan;
 	}
 	// small sorts with a payload: one LDS block as well (up to kLdsPayMax keys)
 	if (!job.algo && job.payload && n <= kLdsPayMax) algo = kAlgoLds;
@@ -2378,8 +2406,9 @@ hipError_t profile_end(double *ms, unsigned *launches, int nkinds)
 // measured on MI355X (uniform keys):
 // - keys-only sorts take the two-level MSD plan from 2^18 keys for 8-byte integer keys (2^20 u64:
 //   223 us against 296 for onesweep; 2^24: 427 against 1065 for count + scan) and from 2^22 for
-//   4-byte integer keys (2^24 u32: 301 us against 411); float keys, whose exponents cluster so that
-//   the plan's bucket estimate usually sends them to the LSD fallback, from 2^26 only;
+//   4-byte integer keys (2^24 u32: 301 us against 411); float keys from 2^24 (algorithm 4 sorts
+//   them with the sample partition plan: 8-byte floats are ahead of the LSD passes from there, and
+//   run_sort keeps 4-byte floats and non-injective 4-byte modes on the LSD passes);
 // - otherwise the count + scan path from 2^22 keys, the onesweep path below (fewer launches);
 // - keys-only sorts of up to kLdsMax keys: one block sorts them in LDS, one launch instead of a dozen
 //   (2^10 / 2^14 u32: 9 / 39 us against 49 / 66 for onesweep; u64: 16 / 85 against 87 / 124; u8 2^14:
@@ -2388,7 +2417,7 @@ int default_algo(int key_bytes, uint64_t n, bool keys_only, bool float_keys)
 {
 	if (keys_only && n <= kLdsMax) return kAlgoLds;
 	if (keys_only && key_bytes >= 4 && local_cfg_for(n) >= 0) {
-		const uint64_t from = float_keys ? (1ull << 26) : (key_bytes >= 8 ? (1ull << 18) : (1ull << 22));
+		const uint64_t from = float_keys ? (1ull << 24) : (key_bytes >= 8 ? (1ull << 18) : (1ull << 22));
 		if (n >= from) return kAlgoHybrid;
 	}
 	return n >= (1u << 22) ? kAlgoCountScan : kAlgoOnesweep;
