@@ -42,18 +42,14 @@ struct Control {
 	PassPlan pass[kMaxPasses];
 	PassPlan pre;          // copy before the first pass (in-place sorts that would overlap)
 	PassPlan fin;          // copy when every digit is uniform (no pass executes)
-	// two-level MSD plan (keys only, 4- and 8-byte keys): hyb[0] partitions by the top digit,
-	// hyb[1] by the second digit inside each top-digit segment, hyb[2] sorts every resulting
-	// bucket in LDS. Its histogram read counts only the top two digits (and finds the uniform
-	// ones among the others from the AND and OR of all keys); when the LSD plan is needed after
-	// all, need_full gates a full histogram read and a second plan.
+	// two-level MSD plans (4- and 8-byte keys): hyb[0] is the first partition, hyb[1] the second
+	// inside each first-partition segment, hyb[2] the per-bucket sort. The plan by the top two
+	// digits reads only the top-16-bit histogram; when the LSD plan is needed after all, need_full
+	// gates a full histogram read and a second plan.
 	PassPlan hyb[3];
-	uint64_t key_or;       // OR of every transformed key
-	uint64_t key_nor;      // OR of every complemented transformed key (the complement of their AND)
 	uint32_t need_full;
 	uint32_t hyb_scan_counter;
-	uint32_t hyb_over;     // some bucket exceeds the LDS capacity: use the LSD plan
-	uint32_t hyb_queue;    // buckets queued for the LDS LSD passes (long runs of one third digit)
+	uint32_t hyb_queue;    // buckets queued for the LDS LSD passes (long runs, or no room for positions)
 	uint32_t joint_max;    // largest bucket of the top-16-bit histogram (the exact MSD plan test)
 	uint32_t bucket_count; // sample partition plan: buckets queued for the smaller per-bucket sort
 	uint32_t bucket_count2;// ... and for the larger one

@@ -16,7 +16,7 @@
 //                   the first stable partition (ranked, so ties keep their order);
 //   k_samp_l2       one block per first-partition segment: its own splitter table from a sample
 //                   of the segment, with as many buckets as its length needs;
-//   k_hyb_count<..., SPL>, k_hyb_segscan, k_scatter<..., SEG, SPL 2>
+//   k_split_count2, k_split_segscan, k_scatter<..., SEG, SPL 2>
 //                   the second stable partition inside every segment;
 //   k_bucket_sort   one block per bucket: orders the bucket by (sort key, position), a partition
 //                   on the top bits of (key - min) and exact ranks within its runs; equality
@@ -289,6 +289,95 @@ __global__ __launch_bounds__(THREADS) void k_split_count1(const K *__restrict__ 
 		__syncthreads();
 	}
 	if (mine) atomicAdd(&hist[t], mine);
+}
+
+// The second count: per-tile counts of the bucket id under the tile's segment table (split +
+// segment * kSplitTabWords) over the tiles of the segment map, into the count rows.
+template <typename K, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_split_count2(const PassPlan *ppp, KeyXform xf, uint32_t tile_keys, SegMap seg,
+	uint32_t *__restrict__ counts, const uint32_t *__restrict__ split)
+{
+	using W = typename KeyTraits<K>::W;
+	const PassPlan pp = *ppp;
+	if (!pp.active) return;
+	const K *__restrict__ src = static_cast<const K *>(pp.src);
+	constexpr int C = count_copies(1);
+	__shared__ uint32_t s_ts[kRadix + 1];
+	__shared__ uint32_t s_t[kRadix * C];
+	__shared__ SplitLds<W> s_tab;
+	for (int i = threadIdx.x; i <= kRadix; i += THREADS) s_ts[i] = seg.tile_start[i];
+	__syncthreads();
+	const uint32_t num = s_ts[kRadix];
+	uint32_t tab_seg = 0xFFFFFFFFu;
+	for (uint32_t tile = blockIdx.x; tile < num; tile += gridDim.x) {
+		for (int i = threadIdx.x; i < kRadix * C; i += THREADS) s_t[i] = 0;
+		uint64_t base;
+		uint32_t len, sg;
+		seg_tile(s_ts, seg, tile_keys, tile, base, len, sg);
+		if (sg != tab_seg) {  // uniform across the block; the last reads of the old table ended at a barrier
+			split_load<W, THREADS>(split + static_cast<size_t>(sg) * kSplitTabWords, s_tab);
+			tab_seg = sg;
+		}
+		__syncthreads();
+		count_split_ids<K, THREADS, C>(src, base, len, xf, s_tab, s_t);
+		__syncthreads();
+		for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kCountStride + i] = copies_sum<C>(s_t, i);
+		__syncthreads();
+	}
+}
+
+// One block per first-partition segment, one thread per bucket id: the bucket totals, their
+// exclusive starts inside the segment (the bucket table), the tile counts turned into global output
+// offsets, and every non-empty bucket queued for a per-bucket sort: those of at most qcap keys in
+// T.queue (ctrl->bucket_count of them), the others in T.queue2 (ctrl->bucket_count2); one atomic per
+// block and queue.
+__global__ __launch_bounds__(256) void k_split_segscan(Control *ctrl, SegTables T, uint32_t *__restrict__ counts, uint32_t qcap)
+{
+	if (!ctrl->hyb[1].active) return;
+	__shared__ uint32_t s_wave[4];
+	__shared__ uint32_t s_qw[2][4], s_qbase[2];
+	const uint32_t sg = blockIdx.x, d = threadIdx.x, lane = d & 63u, w = d >> 6;
+	const uint32_t t0 = T.tile_start[sg], t1 = T.tile_start[sg + 1];
+	uint32_t tot = 0;
+	for (uint32_t tl = t0; tl < t1; tl += 8) {
+		uint32_t c[8];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) c[j] = (tl + j < t1) ? counts[static_cast<uint64_t>(tl + j) * kCountStride + d] : 0u;
+#pragma unroll
+		for (int j = 0; j < 8; ++j) tot += c[j];
+	}
+	const uint32_t incl = wave_inclusive_sum(tot);
+	if (lane == 63) s_wave[w] = incl;
+	const int q = tot > qcap ? 1 : 0;
+	const uint64_t m0 = __ballot(tot != 0 && q == 0), m1 = __ballot(q == 1);
+	if (lane == 0) {
+		s_qw[0][w] = static_cast<uint32_t>(__popcll(m0));
+		s_qw[1][w] = static_cast<uint32_t>(__popcll(m1));
+	}
+	__syncthreads();
+	if (d < 2) {
+		const uint32_t all = s_qw[d][0] + s_qw[d][1] + s_qw[d][2] + s_qw[d][3];
+		s_qbase[d] = all ? atomicAdd(d ? &ctrl->bucket_count2 : &ctrl->bucket_count, all) : 0u;
+	}
+	uint32_t before = 0;
+	for (uint32_t v = 0; v < w; ++v) before += s_wave[v];
+	uint32_t run = T.seg_start[sg] + before + incl - tot;
+	T.bucket_start[sg * kRadix + d] = before + incl - tot;
+	T.bucket_len[sg * kRadix + d] = tot;
+	__syncthreads();
+	uint32_t at = s_qbase[q] + lanes_below(q ? m1 : m0);
+	for (uint32_t v = 0; v < w; ++v) at += s_qw[q][v];
+	if (tot) (q ? T.queue2 : T.queue)[at] = sg * kRadix + d;
+	for (uint32_t tl = t0; tl < t1; tl += 8) {
+		uint32_t c[8];
+#pragma unroll
+		for (int j = 0; j < 8; ++j) c[j] = (tl + j < t1) ? counts[static_cast<uint64_t>(tl + j) * kCountStride + d] : 0u;
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			if (tl + j < t1) counts[static_cast<uint64_t>(tl + j) * kCountStride + d] = run;
+			run += c[j];
+		}
+	}
 }
 
 // One block: the first partition's bucket starts (bucket[0..255]) and segment map, hist[0..255]

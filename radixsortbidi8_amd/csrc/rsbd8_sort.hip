@@ -279,12 +279,13 @@ struct PlanArgs {
 	int reverse;
 	int keys_final_any;
 	int counts0;
-	int hybrid;         // the two-level MSD plan may be chosen (keys only, 4/8-byte keys)
+	int hybrid;         // the read was the two-level MSD plan's (k_hist_joint, k_joint_reduce: the
+	                    // histograms of the top two digits and the exact bucket sizes), so that plan
+	                    // may be chosen; the LSD plan is then incomplete, and its gated full read and
+	                    // plan redo it when the MSD plan is not chosen
 	uint32_t hyb_cap;   // LDS capacity (keys) of the per-bucket sort
 	uint32_t tile2;     // tile size of the segmented second pass
 	uint32_t *segtab;   // the MSD plan's tables
-	int partial;        // the histogram read counted only the top two digits (see k_hist_tiles)
-	int joint;          // ... and the exact bucket sizes (k_hist_joint, k_joint_reduce)
 	const uint32_t *gate;  // non-null: plan only when it reads nonzero (the full plan after a fallback)
 };
 
@@ -292,7 +293,6 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 {
 	__shared__ uint32_t s_wave[4];
 	__shared__ uint32_t s_uniform[kMaxPasses];
-	__shared__ uint32_t s_max[kMaxPasses];
 	if (a.gate && !*a.gate) return;
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	uint32_t top_count = 0, top_start = 0;
@@ -302,14 +302,13 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 		const uint32_t incl = wave_inclusive_sum(c);
 		if (lane == 63) s_wave[w] = incl;
 		const bool full = static_cast<uint64_t>(c) == a.n;
-		if (t == 0) { s_uniform[p] = 0; s_max[p] = 0; }
+		if (t == 0) s_uniform[p] = 0;
 		__syncthreads();
 		uint32_t before = 0;
 		for (uint32_t v = 0; v < w; ++v) before += s_wave[v];
 		bucket[p * kRadix + t] = before + incl - c;
 		if (p == a.passes - 1) { top_count = c; top_start = before + incl - c; }
 		if (full) s_uniform[p] = 1;
-		atomicMax(&s_max[p], c);
 		__syncthreads();
 	}
 	if (a.hybrid) {
@@ -326,25 +325,18 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 		if (t == kRadix - 1) T.tile_start[kRadix] = before + incl;
 		T.seg_start[t] = top_start;
 		T.seg_len[t] = top_count;
-		if (a.joint) {
-			// the first partition's cursors: range x's keys of top digit t follow those of the
-			// ranges before it (a counting table turned into its own prefix, one column per thread)
-			uint32_t run = top_start;
-			for (uint32_t x = 0; x < kRanges; ++x) {
-				const uint32_t c = T.rcnt[x * kRadix + t];
-				T.rcnt[x * kRadix + t] = run;
-				run += c;
-			}
+		// the first partition's cursors: range x's keys of top digit t follow those of the ranges
+		// before it (a counting table turned into its own prefix, one column per thread)
+		uint32_t run = top_start;
+		for (uint32_t x = 0; x < kRanges; ++x) {
+			const uint32_t c = T.rcnt[x * kRadix + t];
+			T.rcnt[x * kRadix + t] = run;
+			run += c;
 		}
 	}
 	if (t != 0) return;
 	PassPlan none{};
 	for (int h = 0; h < 3; ++h) ctrl->hyb[h] = none;
-	if (a.partial) {
-		// digits below the top two: uniform exactly when the AND and the OR of the keys agree on them
-		const uint64_t differ = ctrl->key_or ^ ~ctrl->key_nor;
-		for (int p = 0; p < a.passes - 2; ++p) s_uniform[p] = ((differ >> (8 * p)) & 0xFFu) == 0;
-	}
 	const uint32_t pres = presort_result(ctrl->check, a.reverse);
 	int A = 0;
 	uint32_t mask = 0;
@@ -394,22 +386,14 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 		src = X[slot]; psrc = PX[slot]; rev = 0; iota = 0;
 		++j;
 	}
-	// The two-level MSD plan: both top digits must vary, and the largest bucket of the pair must fit
-	// the per-bucket LDS sort: exactly, from the top-16-bit histogram, or else as estimated from the
-	// two digit histograms, with room to spare (the exact bucket sizes are then checked after the
-	// second count; larger ones fall back to this LSD plan).
+	// The two-level MSD plan: both top digits must vary, and the largest bucket of the top-16-bit
+	// histogram must fit the per-bucket LDS sort.
 	const int P = a.passes;
 	bool msd = a.hybrid && P >= 4 && a.other && !a.payload && !s_uniform[P - 1] && !s_uniform[P - 2];
-	if (msd && a.joint) {
-		msd = ctrl->joint_max <= a.hyb_cap;
-	} else if (msd) {
-		// (the same margin as the host's capacity choice: six standard deviations of a Poisson count)
-		const float est = static_cast<float>(static_cast<double>(s_max[P - 1]) * s_max[P - 2] / static_cast<double>(a.n));
-		msd = est + 6.f * sqrtf(est) + 64.f <= static_cast<float>(a.hyb_cap);
-	}
-	if (!msd && !a.partial) return;
-	// the LSD plan above is either replaced by the MSD plan or, after a partial histogram read,
-	// incomplete (no bucket starts below the top two digits): the gated full read and plan redo it
+	if (msd) msd = ctrl->joint_max <= a.hyb_cap;
+	if (!a.hybrid) return;
+	// the LSD plan above is either replaced by the MSD plan or incomplete (the read counted only the
+	// top two digits): the gated full read and plan redo it
 	for (int p = 0; p < kMaxPasses; ++p) ctrl->pass[p] = none;
 	ctrl->pre = none;
 	ctrl->fin = none;
@@ -623,7 +607,7 @@ __global__ __launch_bounds__(THREADS) void k_onesweep(Control *ctrl, const uint3
 // PARTIAL (the two-level MSD plan's read): only the top two digits' histograms, plus the AND and
 // OR of all transformed keys (a digit is uniform exactly when they agree on it). A non-null gate
 // that reads zero skips the launch (the full read that only a fallback needs).
-template <typename K, int THREADS, int ITEMS, bool PARTIAL = false>
+template <typename K, int THREADS, int ITEMS>
 __global__ __launch_bounds__(THREADS) void k_hist_tiles(const K *__restrict__ in, uint64_t n, KeyXform xf,
 	uint32_t *__restrict__ hist, uint32_t *__restrict__ counts, uint32_t tile_keys, uint32_t num_tiles, int reverse,
 	Control *ctrl, uint32_t count_shift, const uint32_t *gate)
@@ -631,24 +615,20 @@ __global__ __launch_bounds__(THREADS) void k_hist_tiles(const K *__restrict__ in
 	if (gate && !*gate) return;
 	if (presort_result(ctrl->check, reverse)) return;  // the input is already in order
 	constexpr int P = sizeof(K);
-	constexpr int P0 = PARTIAL ? P - 2 : 0;  // first digit counted
-	using W = typename KeyTraits<K>::W;
 	// The top two digits (the skewed ones for floats and clustered integers) and the tile counts
 	// keep C counter copies, the digits below them one; digit p's histogram starts at s_h[hoff(p)].
-	constexpr int NLOW = P - 2 - P0 > 0 ? P - 2 - P0 : 0;
-	constexpr int NTOP = P - P0 - NLOW;
+	constexpr int NLOW = P - 2 > 0 ? P - 2 : 0;
+	constexpr int NTOP = P - NLOW;
 	constexpr int C = count_copies(NTOP + 1, NLOW);
 	constexpr int HWORDS = (NLOW + NTOP * C) * kRadix;
-	auto hoff = [](int p) { return p - P0 < NLOW ? (p - P0) * kRadix : (NLOW + (p - P0 - NLOW) * C) * kRadix; };
-	auto hcop = [](int p) { return p - P0 < NLOW ? 1 : C; };
+	auto hoff = [](int p) { return p < NLOW ? p * kRadix : (NLOW + (p - NLOW) * C) * kRadix; };
+	auto hcop = [](int p) { return p < NLOW ? 1 : C; };
 	__shared__ uint32_t s_h[HWORDS];
 	__shared__ uint32_t s_t[kRadix * C];
-	__shared__ W s_or[THREADS / 64], s_nor[THREADS / 64];
-	W k_or = 0, k_nor = 0;
 	// the digit the tile counts follow is not counted twice: its histogram is the sum of its tile
 	// counts, added in at the end of every tile
-	const int pc = (counts && count_shift % 8u == 0 && static_cast<int>(count_shift / 8u) >= P0 &&
-		static_cast<int>(count_shift / 8u) < P) ? static_cast<int>(count_shift / 8u) : -1;
+	const int pc = (counts && count_shift % 8u == 0 && static_cast<int>(count_shift / 8u) < P) ?
+		static_cast<int>(count_shift / 8u) : -1;
 	for (int i = threadIdx.x; i < HWORDS; i += THREADS) s_h[i] = 0;
 	for (uint32_t tile = blockIdx.x; tile < num_tiles; tile += gridDim.x) {
 		for (int i = threadIdx.x; i < kRadix * C; i += THREADS) s_t[i] = 0;
@@ -660,14 +640,13 @@ __global__ __launch_bounds__(THREADS) void k_hist_tiles(const K *__restrict__ in
 		for_each_key<K, THREADS>(in, lo, len, [&](K x) {
 			const auto k = sort_key<K>(x, xf);
 #pragma unroll
-			for (int p = P0; p < P; ++p) {
+			for (int p = 0; p < P; ++p) {
 				if (p == pc) continue;
 				const uint32_t d = static_cast<uint32_t>(k >> (8 * p)) & 0xFFu;
-				if (p - P0 < NLOW) wave_count<1>(s_h + hoff(p), d);
+				if (p < NLOW) wave_count<1>(s_h + hoff(p), d);
 				else wave_count<C>(s_h + hoff(p), d);
 			}
 			if (counts) wave_count<C>(s_t, static_cast<uint32_t>(k >> count_shift) & 0xFFu);
-			if constexpr (PARTIAL) { k_or |= k; k_nor |= ~k; }
 		});
 		__syncthreads();
 		if (counts)
@@ -679,27 +658,11 @@ __global__ __launch_bounds__(THREADS) void k_hist_tiles(const K *__restrict__ in
 		__syncthreads();
 	}
 	__syncthreads();
-	for (int i = threadIdx.x; i < (P - P0) * kRadix; i += THREADS) {
-		const int p = P0 + (i >> 8);
+	for (int i = threadIdx.x; i < P * kRadix; i += THREADS) {
+		const int p = i >> 8;
 		const uint32_t d = static_cast<uint32_t>(i) & 0xFFu;
 		const uint32_t v = hcop(p) == 1 ? s_h[hoff(p) + d] : copies_sum<C>(s_h + hoff(p), d);
-		if (v) atomicAdd(&hist[P0 * kRadix + i], v);
-	}
-	if constexpr (PARTIAL) {
-#pragma unroll
-		for (int o = 32; o > 0; o >>= 1) {
-			k_or |= __shfl_xor(k_or, o, 64);
-			k_nor |= __shfl_xor(k_nor, o, 64);
-		}
-		const uint32_t w = threadIdx.x >> 6;
-		if ((threadIdx.x & 63u) == 0) { s_or[w] = k_or; s_nor[w] = k_nor; }
-		__syncthreads();
-		if (threadIdx.x == 0) {
-			W a = 0, b = 0;
-			for (int v = 0; v < THREADS / 64; ++v) { a |= s_or[v]; b |= s_nor[v]; }
-			atomicOr(reinterpret_cast<unsigned long long *>(&ctrl->key_or), static_cast<unsigned long long>(a));
-			atomicOr(reinterpret_cast<unsigned long long *>(&ctrl->key_nor), static_cast<unsigned long long>(b));
-		}
+		if (v) atomicAdd(&hist[i], v);
 	}
 }
 
@@ -1101,130 +1064,21 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 // two-level MSD plan: segmented count and scan of the second digit, fallback switch, and the
 // per-bucket LDS sort of the remaining digits
 
-// per-tile counts of the second digit over the tiles of the segment map (SPL: the bucket id under
-// the tile's segment table, split + segment * kSplitTabWords)
-template <typename K, int THREADS, bool SPL = false>
-__global__ __launch_bounds__(THREADS) void k_hyb_count(const PassPlan *ppp, uint32_t shift, KeyXform xf, uint32_t tile_keys,
-	SegMap seg, uint32_t *__restrict__ counts, const uint32_t *__restrict__ split)
-{
-	using W = typename KeyTraits<K>::W;
-	const PassPlan pp = *ppp;
-	if (!pp.active) return;
-	const K *__restrict__ src = static_cast<const K *>(pp.src);
-	constexpr int C = count_copies(1);
-	__shared__ uint32_t s_ts[kRadix + 1];
-	__shared__ uint32_t s_t[kRadix * C];
-	__shared__ std::conditional_t<SPL != 0, SplitLds<W>, uint32_t> s_tab;  // the splitter table (SPL only)
-	for (int i = threadIdx.x; i <= kRadix; i += THREADS) s_ts[i] = seg.tile_start[i];
-	__syncthreads();
-	const uint32_t num = s_ts[kRadix];
-	uint32_t tab_seg = 0xFFFFFFFFu;
-	for (uint32_t tile = blockIdx.x; tile < num; tile += gridDim.x) {
-		for (int i = threadIdx.x; i < kRadix * C; i += THREADS) s_t[i] = 0;
-		uint64_t base;
-		uint32_t len, sg;
-		seg_tile(s_ts, seg, tile_keys, tile, base, len, sg);
-		if constexpr (SPL) {
-			if (sg != tab_seg) {  // uniform across the block; the last reads of the old table ended at a barrier
-				split_load<W, THREADS>(split + static_cast<size_t>(sg) * kSplitTabWords, s_tab);
-				tab_seg = sg;
-			}
-		}
-		__syncthreads();
-		if constexpr (SPL)
-			count_split_ids<K, THREADS, C>(src, base, len, xf, s_tab, s_t);
-		else
-			for_each_key<K, THREADS>(src, base, len, [&](K x) { wave_count<C>(s_t, digit_of<K>(x, xf, shift)); });
-		__syncthreads();
-		for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kCountStride + i] = copies_sum<C>(s_t, i);
-		__syncthreads();
-	}
-}
-
-// One block per top-digit segment, one thread per second digit: the bucket totals, their
-// exclusive offsets inside the segment, the bucket table, and the tile counts turned into global
-// output offsets. A bucket larger than cap marks the plan for the LSD fallback. With queues (the
-// sample partition plan), the non-empty buckets are also queued: those of at most qcap keys in
-// T.queue (ctrl->bucket_count of them), the others in T.queue2 (ctrl->bucket_count2); one atomic
-// per block and queue.
-__global__ __launch_bounds__(256) void k_hyb_segscan(Control *ctrl, SegTables T, uint32_t *__restrict__ counts, uint32_t cap,
-	int queues, uint32_t qcap)
-{
-	if (!ctrl->hyb[1].active) return;
-	__shared__ uint32_t s_wave[4];
-	__shared__ uint32_t s_qw[2][4], s_qbase[2];
-	const uint32_t sg = blockIdx.x, d = threadIdx.x, lane = d & 63u, w = d >> 6;
-	const uint32_t t0 = T.tile_start[sg], t1 = T.tile_start[sg + 1];
-	uint32_t tot = 0;
-	for (uint32_t tl = t0; tl < t1; tl += 8) {
-		uint32_t c[8];
-#pragma unroll
-		for (int j = 0; j < 8; ++j) c[j] = (tl + j < t1) ? counts[static_cast<uint64_t>(tl + j) * kCountStride + d] : 0u;
-#pragma unroll
-		for (int j = 0; j < 8; ++j) tot += c[j];
-	}
-	const uint32_t incl = wave_inclusive_sum(tot);
-	if (lane == 63) s_wave[w] = incl;
-	__syncthreads();
-	uint32_t before = 0;
-	for (uint32_t v = 0; v < w; ++v) before += s_wave[v];
-	uint32_t run = T.seg_start[sg] + before + incl - tot;
-	T.bucket_start[sg * kRadix + d] = before + incl - tot;
-	T.bucket_len[sg * kRadix + d] = tot;
-	if (tot > cap) atomicOr(&ctrl->hyb_over, 1u);
-	if (queues) {
-		const int q = tot > qcap ? 1 : 0;
-		const uint64_t m0 = __ballot(tot != 0 && q == 0), m1 = __ballot(q == 1);
-		if (lane == 0) {
-			s_qw[0][w] = static_cast<uint32_t>(__popcll(m0));
-			s_qw[1][w] = static_cast<uint32_t>(__popcll(m1));
-		}
-		__syncthreads();
-		if (d < 2) {
-			const uint32_t all = s_qw[d][0] + s_qw[d][1] + s_qw[d][2] + s_qw[d][3];
-			s_qbase[d] = all ? atomicAdd(d ? &ctrl->bucket_count2 : &ctrl->bucket_count, all) : 0u;
-		}
-		__syncthreads();
-		uint32_t at = s_qbase[q] + lanes_below(q ? m1 : m0);
-		for (uint32_t v = 0; v < w; ++v) at += s_qw[q][v];
-		if (tot) (q ? T.queue2 : T.queue)[at] = sg * kRadix + d;
-	}
-	for (uint32_t tl = t0; tl < t1; tl += 8) {
-		uint32_t c[8];
-#pragma unroll
-		for (int j = 0; j < 8; ++j) c[j] = (tl + j < t1) ? counts[static_cast<uint64_t>(tl + j) * kCountStride + d] : 0u;
-#pragma unroll
-		for (int j = 0; j < 8; ++j) {
-			if (tl + j < t1) counts[static_cast<uint64_t>(tl + j) * kCountStride + d] = run;
-			run += c[j];
-		}
-	}
-}
-
-// A bucket too large for the LDS sort: switch the rest of the MSD plan off and request the full
-// histogram read and the LSD plan (the input is still intact: only the second pass's scatter and
-// the bucket sort would overwrite arrays the LSD plan reads).
-__global__ void k_hyb_check(Control *ctrl)
-{
-	if (threadIdx.x != 0 || !ctrl->hyb[1].active || !ctrl->hyb_over) return;
-	ctrl->hyb[1].active = 0;
-	ctrl->hyb[2].active = 0;
-	ctrl->need_full = 1;
-}
-
 // The per-bucket sort, one block per bucket (a persistent form that loads the next bucket while
 // sorting the current one measured slower: its registers halve the resident blocks). All keys of a
 // bucket share the top two digits, so the transformed key shifted left by 16 keeps every digit
-// below them. Each key becomes the word (transformed key << 16) | l, with l
-// its position in the bucket: distinct words whose order is the stable order. The words are
-// partitioned by their top PB bits with LDS atomics (in any order), then every word's final
-// position inside its run of equal top bits is the number of smaller words in the run. The raw
-// keys reach their positions through their owners. A bucket with a run longer than kLocalRunMax is
-// queued for k_local_lsd instead. A bucket is read whole before it is written, so source and
-// destination may coincide.
+// below them. Each key becomes the word (transformed key << 16) | l, with l its position in the
+// bucket: distinct words whose order is the stable order. The words are partitioned by their top
+// PB bits with LDS atomics (in any order), then every word's final position inside its run of
+// equal top bits is the number of smaller words in the run. The transform only complements bits
+// (the plan runs for integer keys under an injective transform), so every word gives back its raw
+// key: the bucket index (the top 16 bits), the word's key bits, the transform undone; the ranks go
+// to LDS as the inverse permutation (position -> word) and the output is gathered from the words
+// in order. A bucket with a run longer than kLocalRunMax is queued for k_local_lsd instead. A
+// bucket is read whole before it is written, so source and destination may coincide.
 constexpr uint32_t kLocalRunMax = 96;
 
-template <typename K, int THREADS, int KPT, int PB, bool INJ>
+template <typename K, int THREADS, int KPT, int PB>
 __global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, const uint32_t *__restrict__ seg_start,
 	const uint32_t *__restrict__ bstart, const uint32_t *__restrict__ blen, uint32_t *__restrict__ queue)
 {
@@ -1273,7 +1127,7 @@ __global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, c
 			if (t == 0 && pp.src != pp.dst) dst[0] = key[0];
 		} else if (len > 1) {
 			// the top 16 bits of the sort key, which every key of the bucket shares: the bucket index
-			const W pre = INJ ? static_cast<W>(static_cast<W>(bk) << (8 * P - 16)) : W(0);
+			const W pre = static_cast<W>(static_cast<W>(bk) << (8 * P - 16));
 			uint32_t *s_cur = s_u;
 			for (uint32_t i = t; i < NB; i += THREADS) s_cur[i] = 0;
 			if (t == 0) s_big = 0;
@@ -1319,59 +1173,26 @@ __global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, c
 				}
 				__syncthreads();
 				// the final position of every word (its rank in its run, after the run's start)
-				if constexpr (INJ) {
-					// Equal sort keys are equal keys (the transform only complements bits), so every
-					// word gives back its raw key: the bucket's top 16 bits, the word's key bits, the
-					// transform undone. The ranks go to LDS as the inverse permutation (position ->
-					// word), and the output is gathered from the words in order.
-					uint16_t *s_inv = reinterpret_cast<uint16_t *>(s_u);  // the cursors are dead
+				// Equal sort keys are equal keys (the transform only complements bits), so every
+				// word gives back its raw key: the bucket's top 16 bits, the word's key bits, the
+				// transform undone. The ranks go to LDS as the inverse permutation (position ->
+				// word), and the output is gathered from the words in order.
+				uint16_t *s_inv = reinterpret_cast<uint16_t *>(s_u);  // the cursors are dead
 #pragma unroll
-					for (int j = 0; j < KPT; ++j) {
-						const uint32_t i = j * THREADS + t;
-						if (i < len) {
-							const W x = s_t[i];
-							const uint32_t d = static_cast<uint32_t>(x >> kTop);
-							const uint32_t lo = s_off[d], hi = s_off[d + 1];
-							uint32_t n_less = 0;  // the word itself is in the run and not smaller
-							for (uint32_t q = lo; q < hi; ++q) n_less += static_cast<uint32_t>(s_t[q] < x);
-							s_inv[lo + n_less] = static_cast<uint16_t>(i);
-						}
+				for (int j = 0; j < KPT; ++j) {
+					const uint32_t i = j * THREADS + t;
+					if (i < len) {
+						const W x = s_t[i];
+						const uint32_t d = static_cast<uint32_t>(x >> kTop);
+						const uint32_t lo = s_off[d], hi = s_off[d + 1];
+						uint32_t n_less = 0;  // the word itself is in the run and not smaller
+						for (uint32_t q = lo; q < hi; ++q) n_less += static_cast<uint32_t>(s_t[q] < x);
+						s_inv[lo + n_less] = static_cast<uint16_t>(i);
 					}
-					__syncthreads();
-					for (uint32_t o = t; o < len; o += THREADS)
-						dst[o] = static_cast<K>(((s_t[s_inv[o]] >> 16) | pre) ^ static_cast<W>(xf.E));
-				} else {
-					// the raw keys travel through their owners: position l's final place, then the
-					// owner of l writes its raw key there
-					uint16_t *s_fin = reinterpret_cast<uint16_t *>(s_u);
-					uint32_t fin[KPT];
-#pragma unroll
-					for (int j = 0; j < KPT; ++j) {
-						const uint32_t i = j * THREADS + t;
-						fin[j] = 0xFFFFFFFFu;
-						if (i < len) {
-							const W x = s_t[i];
-							const uint32_t d = static_cast<uint32_t>(x >> kTop);
-							const uint32_t lo = s_off[d], hi = s_off[d + 1];
-							uint32_t n_less = 0;
-							for (uint32_t q = lo; q < hi; ++q) n_less += static_cast<uint32_t>(s_t[q] < x);
-							fin[j] = ((static_cast<uint32_t>(x) & 0xFFFFu) << 16) | (lo + n_less);
-						}
-					}
-					__syncthreads();  // every run has been read: s_u and s_t take the results
-#pragma unroll
-					for (int j = 0; j < KPT; ++j)
-						if (fin[j] != 0xFFFFFFFFu) s_fin[fin[j] >> 16] = static_cast<uint16_t>(fin[j]);
-					__syncthreads();
-					K *s_out = reinterpret_cast<K *>(s_t);
-#pragma unroll
-					for (int j = 0; j < KPT; ++j) {
-						const uint32_t l = w * 64u * KPT + j * 64u + lane;
-						if (l < len) s_out[s_fin[l]] = key[j];
-					}
-					__syncthreads();
-					for (uint32_t i = t; i < len; i += THREADS) dst[i] = s_out[i];
 				}
+				__syncthreads();
+				for (uint32_t o = t; o < len; o += THREADS)
+					dst[o] = static_cast<K>(((s_t[s_inv[o]] >> 16) | pre) ^ static_cast<W>(xf.E));
 			}
 		}
 	}
@@ -1592,11 +1413,8 @@ __device__ __forceinline__ void reset_control(Control *ctrl)
 		ctrl->epoch = ctrl->epoch + 1u;
 		ctrl->check = 0;
 		ctrl->hyb_scan_counter = 0;
-		ctrl->hyb_over = 0;
 		ctrl->hyb_queue = 0;
 		ctrl->need_full = 0;
-		ctrl->key_or = 0;
-		ctrl->key_nor = 0;
 		ctrl->joint_max = 0;
 		ctrl->bucket_count = 0;
 		ctrl->bucket_count2 = 0;
@@ -1789,12 +1607,8 @@ int local_cfg_for(uint64_t n)
 template <typename K, int THREADS, int KPT, int PB>
 hipError_t launch_local_cfg(Control *ctrl, const KeyXform &xf, const SegTables &T, hipStream_t stream)
 {
-	if (xform_injective(xf, sizeof(K)))
-		hipLaunchKernelGGL((k_local<K, THREADS, KPT, PB, true>), dim3(kHybBuckets), dim3(THREADS), 0, stream, ctrl, xf,
-			T.seg_start, T.bucket_start, T.bucket_len, T.queue);
-	else
-		hipLaunchKernelGGL((k_local<K, THREADS, KPT, PB, false>), dim3(kHybBuckets), dim3(THREADS), 0, stream, ctrl, xf,
-			T.seg_start, T.bucket_start, T.bucket_len, T.queue);
+	hipLaunchKernelGGL((k_local<K, THREADS, KPT, PB>), dim3(kHybBuckets), dim3(THREADS), 0, stream, ctrl, xf, T.seg_start,
+		T.bucket_start, T.bucket_len, T.queue);
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess) return e;
 	const uint32_t lgrid = static_cast<uint32_t>(4 * devinfo().cus);
@@ -1817,68 +1631,28 @@ hipError_t launch_local(int lcfg, Control *ctrl, const KeyXform &xf, const SegTa
 	}
 }
 
-// The two-level MSD plan's launches after k_plan (every one exits at once when the plan did not
-// choose it; k_hyb_check switches the rest off when a bucket is too large). With joint (the read
-// was k_hist_joint: the bucket sizes are known and the plan checked them), the second partition
-// takes its output ranges from the bucket cursors and needs no count of its own.
+// The two-level MSD plan's launches after k_plan (each exits at once when the plan did not choose
+// it). The read was k_hist_joint: the bucket sizes are known and the plan checked them, so both
+// partitions take their output ranges from cursors (the range cursors, then the bucket cursors)
+// and need no count or scan of their own.
 template <typename K, int THREADS, int KPT>
-hipError_t launch_hybrid(Control *ctrl, const uint32_t *bucket, uint32_t *counts, uint64_t *lookback, const SegTables &T,
-	const KeyXform &xf, uint64_t n, uint32_t tiles, int lcfg, bool joint, void *ws, hipStream_t stream)
+hipError_t launch_hybrid(Control *ctrl, const SegTables &T, const KeyXform &xf, uint64_t n, uint32_t tiles, int lcfg,
+	void *ws, hipStream_t stream)
 {
 	constexpr int P = sizeof(K);
-	constexpr uint32_t TILE = THREADS * KPT;
-	const int cus = devinfo().cus;
-	const bool inj = xform_injective(xf, P);
-	hipError_t e;
 	const uint32_t tiles2 = tiles + kRadix;  // bound: one partial tile per segment
 	const SegMap m{T.tile_start, T.seg_start, T.seg_len};
-	if (joint && inj) {
-		// both partitions take their output ranges from cursors: no count or scan launches
-		{
-			ProfScope ps(5, 0, ws, stream);
-			e = launch_scatter<K, false, THREADS, KPT, false, true, true>(&ctrl->hyb[0], 8u * (P - 1), xf, n, tiles, nullptr,
-				SegMap{}, stream, T.rcnt);
-			if (e != hipSuccess) return e;
-		}
-		{
-			ProfScope ps(5, 1, ws, stream);
-			e = launch_scatter<K, false, THREADS, KPT, true, true, true>(&ctrl->hyb[1], 8u * (P - 2), xf, n, tiles2, nullptr, m,
-				stream, T.cursor);
-			if (e != hipSuccess) return e;
-		}
-		ProfScope ps(6, 0, ws, stream);
-		return launch_local<K>(lcfg, ctrl, xf, T, stream);
-	}
-	{
-		ProfScope ps(7, 0, ws, stream);
-		const uint32_t nchunks = (tiles + kScanChunk - 1) / kScanChunk;
-		const uint32_t sgrid = std::max<uint32_t>(1u, std::min<uint32_t>(nchunks, 4u * cus));
-		hipLaunchKernelGGL(k_scan, dim3(sgrid), dim3(256), 0, stream, ctrl, &ctrl->hyb[0], &ctrl->hyb_scan_counter,
-			bucket + (P - 1) * kRadix, kMaxPasses, counts, lookback, tiles);
-		if ((e = hipGetLastError()) != hipSuccess) return e;
-	}
+	hipError_t e;
 	{
 		ProfScope ps(5, 0, ws, stream);
-		e = inj ? launch_scatter<K, false, THREADS, KPT, false, true>(&ctrl->hyb[0], 8u * (P - 1), xf, n, tiles, counts, SegMap{},
-		                                                              stream)
-		        : launch_scatter<K, false, THREADS, KPT>(&ctrl->hyb[0], 8u * (P - 1), xf, n, tiles, counts, SegMap{}, stream);
+		e = launch_scatter<K, false, THREADS, KPT, false, true, true>(&ctrl->hyb[0], 8u * (P - 1), xf, n, tiles, nullptr,
+			SegMap{}, stream, T.rcnt);
 		if (e != hipSuccess) return e;
 	}
 	{
-		ProfScope ps(7, 1, ws, stream);
-		const uint32_t cgrid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles2, 8u * cus));
-		hipLaunchKernelGGL((k_hyb_count<K, 256>), dim3(cgrid), dim3(256), 0, stream, &ctrl->hyb[1], 8u * (P - 2), xf, TILE, m,
-			counts, nullptr);
-		if ((e = hipGetLastError()) != hipSuccess) return e;
-		hipLaunchKernelGGL(k_hyb_segscan, dim3(kRadix), dim3(256), 0, stream, ctrl, T, counts, kLocalCaps[lcfg], 0, 0u);
-		if ((e = hipGetLastError()) != hipSuccess) return e;
-		hipLaunchKernelGGL(k_hyb_check, dim3(1), dim3(64), 0, stream, ctrl);
-		if ((e = hipGetLastError()) != hipSuccess) return e;
-	}
-	{
 		ProfScope ps(5, 1, ws, stream);
-		e = inj ? launch_scatter<K, false, THREADS, KPT, true, true>(&ctrl->hyb[1], 8u * (P - 2), xf, n, tiles2, counts, m, stream)
-		        : launch_scatter<K, false, THREADS, KPT, true>(&ctrl->hyb[1], 8u * (P - 2), xf, n, tiles2, counts, m, stream);
+		e = launch_scatter<K, false, THREADS, KPT, true, true, true>(&ctrl->hyb[1], 8u * (P - 2), xf, n, tiles2, nullptr, m,
+			stream, T.cursor);
 		if (e != hipSuccess) return e;
 	}
 	ProfScope ps(6, 0, ws, stream);
@@ -1886,16 +1660,16 @@ hipError_t launch_hybrid(Control *ctrl, const uint32_t *bucket, uint32_t *counts
 }
 
 template <typename K>
-hipError_t launch_hybrid_cfg(int cfg, Control *ctrl, const uint32_t *bucket, uint32_t *counts, uint64_t *lookback,
-	const SegTables &T, const KeyXform &xf, uint64_t n, uint32_t tiles, int lcfg, bool joint, void *ws, hipStream_t stream)
+hipError_t launch_hybrid_cfg(int cfg, Control *ctrl, const SegTables &T, const KeyXform &xf, uint64_t n, uint32_t tiles,
+	int lcfg, void *ws, hipStream_t stream)
 {
 	switch (cfg) {
-	case 0: return launch_hybrid<K, 256, 8>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, joint, ws, stream);
-	case 1: return launch_hybrid<K, 256, 16>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, joint, ws, stream);
-	case 2: return launch_hybrid<K, 512, 8>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, joint, ws, stream);
-	case 3: return launch_hybrid<K, 1024, 12>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, joint, ws, stream);
-	case 4: return launch_hybrid<K, 256, 24>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, joint, ws, stream);
-	default: return launch_hybrid<K, 1024, 16>(ctrl, bucket, counts, lookback, T, xf, n, tiles, lcfg, joint, ws, stream);
+	case 0: return launch_hybrid<K, 256, 8>(ctrl, T, xf, n, tiles, lcfg, ws, stream);
+	case 1: return launch_hybrid<K, 256, 16>(ctrl, T, xf, n, tiles, lcfg, ws, stream);
+	case 2: return launch_hybrid<K, 512, 8>(ctrl, T, xf, n, tiles, lcfg, ws, stream);
+	case 3: return launch_hybrid<K, 1024, 12>(ctrl, T, xf, n, tiles, lcfg, ws, stream);
+	case 4: return launch_hybrid<K, 256, 24>(ctrl, T, xf, n, tiles, lcfg, ws, stream);
+	default: return launch_hybrid<K, 1024, 16>(ctrl, T, xf, n, tiles, lcfg, ws, stream);
 	}
 }
 
@@ -1982,10 +1756,10 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 		hipLaunchKernelGGL((k_samp_l2<K, 1024>), dim3(kRadix), dim3(1024), 0, stream, ctrl, xf, T, split, target);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 		const uint32_t cgrid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles2, 8u * cus));
-		hipLaunchKernelGGL((k_hyb_count<K, 256, true>), dim3(cgrid), dim3(256), 0, stream, &ctrl->hyb[1], 0u, xf, TILE, m, counts,
+		hipLaunchKernelGGL((k_split_count2<K, 256>), dim3(cgrid), dim3(256), 0, stream, &ctrl->hyb[1], xf, TILE, m, counts,
 			split + kSplitTabWords);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
-		hipLaunchKernelGGL(k_hyb_segscan, dim3(kRadix), dim3(256), 0, stream, ctrl, T, counts, 0xFFFFFFFFu, 1, kTiers ? kBucketCapA : 0u);
+		hipLaunchKernelGGL(k_split_segscan, dim3(kRadix), dim3(256), 0, stream, ctrl, T, counts, kTiers ? kBucketCapA : 0u);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 	}
 	{
@@ -2212,10 +1986,9 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	// count + scan plan, the top digit for the MSD plan
 	const bool counts0 = algo == kAlgoCountScan && !fill;
 	const bool count_tiles = counts0 || hybrid;
-	// The exact bucket sizes (k_hist_joint): integer keys under an injective transform. The read
-	// leaves one 65536-word slice per block in the array the first partition writes (X1, never the
-	// source), so the number of blocks is bounded by that array's size.
-	const bool joint = hybrid && P >= 4 && !float_keys && inj_xf && joint_slices >= kRanges;
+	// The MSD plan's read (k_hist_joint) leaves one 65536-word slice per block in the array the first
+	// partition writes (X1, never the source), so the number of blocks is bounded by that array's
+	// size (joint_slices).
 	uint32_t *joint_part = static_cast<uint32_t *>(job.buf != job.in ? job.buf : job.out);
 	if (job.presort_check && !fill) {
 		const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 4095) / 4096, 4u * cus)));
@@ -2227,27 +2000,17 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 		// four blocks per CU: measured 0.44 ms for the 2^28 u64 first read against 0.48 with eight
 		const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(hybrid ? htiles : tiles, 4u * cus));
 		if (hybrid) {
-			// The MSD plan needs only the top two digits' histograms. Float keys, whose exponents
-			// cluster, usually fail its bucket estimate; for them the read counts every digit, so the
-			// LSD plan it falls back to needs no second read.
+			// The MSD plan needs only the histograms of the top two digits and the exact bucket
+			// sizes: the histogram of the top 16 bits (integer keys under an injective transform).
 			if constexpr (P >= 4) {
-				if (joint) {
-					// the key ranges: whole first-partition tiles, as its blocks deal them to the XCDs
-					const uint64_t range_keys = static_cast<uint64_t>((htiles + kRanges - 1) / kRanges) * htile_keys;
-					hipLaunchKernelGGL((k_hist_joint<K, 1024>), dim3(joint_slices), dim3(1024), 0, stream,
-						static_cast<const K *>(job.in), n, xf, range_keys, (job.flags & kRev) ? 1 : 0, ctrl, joint_part,
-						seg_tables(segtab_of(ws)).ovf);
-					if ((e = hipGetLastError()) != hipSuccess) return e;
-					hipLaunchKernelGGL((k_joint_reduce<P>), dim3(kRadix), dim3(256), 0, stream, ctrl, (job.flags & kRev) ? 1 : 0,
-						joint_part, joint_slices, seg_tables(segtab_of(ws)), hist);
-				} else if (!float_keys)
-					hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS, true>), dim3(grid), dim3(256), 0, stream,
-						static_cast<const K *>(job.in), n, xf, hist, counts, htile_keys, htiles, (job.flags & kRev) ? 1 : 0, ctrl,
-						8u * (P - 1), nullptr);
-				else
-					hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS, false>), dim3(grid), dim3(256), 0, stream,
-						static_cast<const K *>(job.in), n, xf, hist, counts, htile_keys, htiles, (job.flags & kRev) ? 1 : 0, ctrl,
-						8u * (P - 1), nullptr);
+				// the key ranges: whole first-partition tiles, as its blocks deal them to the XCDs
+				const uint64_t range_keys = static_cast<uint64_t>((htiles + kRanges - 1) / kRanges) * htile_keys;
+				hipLaunchKernelGGL((k_hist_joint<K, 1024>), dim3(joint_slices), dim3(1024), 0, stream,
+					static_cast<const K *>(job.in), n, xf, range_keys, (job.flags & kRev) ? 1 : 0, ctrl, joint_part,
+					seg_tables(segtab_of(ws)).ovf);
+				if ((e = hipGetLastError()) != hipSuccess) return e;
+				hipLaunchKernelGGL((k_joint_reduce<P>), dim3(kRadix), dim3(256), 0, stream, ctrl, (job.flags & kRev) ? 1 : 0,
+					joint_part, joint_slices, seg_tables(segtab_of(ws)), hist);
 			}
 		} else if (count_tiles) {
 			hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS>), dim3(grid), dim3(256), 0, stream, static_cast<const K *>(job.in), n,
@@ -2273,8 +2036,6 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	pa.hyb_cap = hybrid ? kLocalCaps[lcfg] : 0;
 	pa.tile2 = htile_keys;
 	pa.segtab = segtab_of(ws);
-	pa.partial = (hybrid && !float_keys) ? 1 : 0;
-	pa.joint = joint ? 1 : 0;
 	pa.gate = nullptr;
 
 	hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, stream, pa, ctrl, hist, bucket);
@@ -2287,24 +2048,18 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 			// launches (each of the two sets exits at once unless it was chosen), so that the
 			// fallback's launches, about 30 dispatches, cost no time after the bucket sort. The
 			// stream waits for the side stream before the sort ends.
-			if (joint && (e = fork_side(stream, &ls)) != hipSuccess) return e;
-			e = launch_hybrid_cfg<K>(hcfg, ctrl, bucket, counts, lookback, seg_tables(segtab_of(ws)), xf, n, htiles, lcfg, joint,
-				ws, stream);
+			if ((e = fork_side(stream, &ls)) != hipSuccess) return e;
+			e = launch_hybrid_cfg<K>(hcfg, ctrl, seg_tables(segtab_of(ws)), xf, n, htiles, lcfg, ws, stream);
 			if (e != hipSuccess) return e;
 			// The LSD plan as the fallback: a full histogram read and a second plan, both gated on
-			// need_full, then the launches below (inactive unless planned). Count + scan passes,
-			// except for 8-byte float keys, whose first read already counted every digit: there the
-			// onesweep passes, which need no count read, are ahead (f64 from N(0, 1), 2^26 / 2^28:
-			// 3.21 / 12.2 ms against 3.87 / 14.0), while for 4-byte keys the two are within 6%.
-			algo = (P == 8 && float_keys) ? kAlgoOnesweep : kAlgoCountScan;
+			// need_full, then count + scan passes (inactive unless planned).
+			algo = kAlgoCountScan;
 			const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, 8u * cus));
 			hipLaunchKernelGGL((k_hist_tiles<K, 256, HITEMS>), dim3(grid), dim3(256), 0, ls, static_cast<const K *>(job.in), n,
 				xf, hist, counts, tile_keys, tiles, (job.flags & kRev) ? 1 : 0, ctrl, 0u, &ctrl->need_full);
 			if ((e = hipGetLastError()) != hipSuccess) return e;
 			pa.counts0 = 1;
 			pa.hybrid = 0;
-			pa.partial = 0;
-			pa.joint = 0;
 			pa.gate = &ctrl->need_full;
 			hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, ls, pa, ctrl, hist, bucket);
 			if ((e = hipGetLastError()) != hipSuccess) return e;
