@@ -65,8 +65,9 @@ constexpr uint32_t kHybBuckets = kRadix * kRadix;  // buckets after the two MSD 
 // and lengths (2 x 65536), the queue of buckets for the LDS LSD passes (65536), the bucket cursors
 // of the second partition (65536), the overflow counts of the top-16-bit histogram (65536, zero
 // between sorts), the top-digit counts of the first partition's key ranges, then its cursors
-// (8 x 256), and a second bucket queue (65536: the sample partition plan's larger buckets)
-constexpr size_t kSegTableWords = 257 + 256 + 256 + 5 * kHybBuckets + 8 * 256 + kHybBuckets;
+// (8 x 256), and (16-byte aligned) the sample partition plan's bucket queues: three of 65536
+// four-word entries (its smaller buckets, its larger ones, and those its LDS sorts pass on)
+constexpr size_t kSegTableWords = 257 + 256 + 256 + 5 * kHybBuckets + 8 * 256 + 3 + 3 * 4 * kHybBuckets;
 // Splitter tables of the sample partition plan (algorithm 4 for keys it cannot bucket by bits):
 // one for the first partition and one per first-partition segment for the second. A table maps a
 // transformed key to a bucket id by the number of its bounds below the key (see split_id):

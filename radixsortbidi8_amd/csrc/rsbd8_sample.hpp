@@ -328,16 +328,20 @@ __global__ __launch_bounds__(THREADS) void k_split_count2(const PassPlan *ppp, K
 
 // One block per first-partition segment, one thread per bucket id: the bucket totals, their
 // exclusive starts inside the segment (the bucket table), the tile counts turned into global output
-// offsets, and every non-empty bucket queued for a per-bucket sort: those of at most qcap keys in
-// T.queue (ctrl->bucket_count of them), the others in T.queue2 (ctrl->bucket_count2); one atomic per
-// block and queue.
-__global__ __launch_bounds__(256) void k_split_segscan(Control *ctrl, SegTables T, uint32_t *__restrict__ counts, uint32_t qcap)
+// offsets, and every non-empty bucket queued for a per-bucket sort as the entry {absolute start,
+// length, bucket, equal keys} (equal: the bucket is an equality bucket of the segment's table at
+// split_l2 + segment * kSplitTabWords): those of at most qcap keys that are not equality buckets in
+// T.bq[0, ...) (ctrl->bucket_count of them), the others in T.bq[65536, ...) (ctrl->bucket_count2);
+// one atomic per block and queue.
+__global__ __launch_bounds__(256) void k_split_segscan(Control *ctrl, SegTables T, uint32_t *__restrict__ counts,
+	const uint32_t *__restrict__ split_l2, uint32_t qcap)
 {
 	if (!ctrl->hyb[1].active) return;
 	__shared__ uint32_t s_wave[4];
 	__shared__ uint32_t s_qw[2][4], s_qbase[2];
 	const uint32_t sg = blockIdx.x, d = threadIdx.x, lane = d & 63u, w = d >> 6;
 	const uint32_t t0 = T.tile_start[sg], t1 = T.tile_start[sg + 1];
+	const uint32_t eq = (split_l2[static_cast<size_t>(sg) * kSplitTabWords + kSplitEqWord + (d >> 5)] >> (d & 31u)) & 1u;
 	uint32_t tot = 0;
 	for (uint32_t tl = t0; tl < t1; tl += 8) {
 		uint32_t c[8];
@@ -348,8 +352,8 @@ __global__ __launch_bounds__(256) void k_split_segscan(Control *ctrl, SegTables 
 	}
 	const uint32_t incl = wave_inclusive_sum(tot);
 	if (lane == 63) s_wave[w] = incl;
-	const int q = tot > qcap ? 1 : 0;
-	const uint64_t m0 = __ballot(tot != 0 && q == 0), m1 = __ballot(q == 1);
+	const int q = (tot > qcap || eq) ? 1 : 0;
+	const uint64_t m0 = __ballot(tot != 0 && q == 0), m1 = __ballot(tot != 0 && q == 1);
 	if (lane == 0) {
 		s_qw[0][w] = static_cast<uint32_t>(__popcll(m0));
 		s_qw[1][w] = static_cast<uint32_t>(__popcll(m1));
@@ -367,7 +371,7 @@ __global__ __launch_bounds__(256) void k_split_segscan(Control *ctrl, SegTables 
 	__syncthreads();
 	uint32_t at = s_qbase[q] + lanes_below(q ? m1 : m0);
 	for (uint32_t v = 0; v < w; ++v) at += s_qw[q][v];
-	if (tot) (q ? T.queue2 : T.queue)[at] = sg * kRadix + d;
+	if (tot) T.bq[(q ? kHybBuckets : 0u) + at] = make_uint4(run, tot, sg * kRadix + d, eq);
 	for (uint32_t tl = t0; tl < t1; tl += 8) {
 		uint32_t c[8];
 #pragma unroll
@@ -535,7 +539,7 @@ __device__ void bucket_lsd_global(K *a, K *b, K *dst, uint32_t *pa, uint32_t *pb
 // position in a word, or with a run of more than kBucketRunMax keys is queued for k_bucket_hard.
 template <typename K, bool PAY, int THREADS, int KPT, int PB>
 __global__ __launch_bounds__(THREADS) void k_bucket_sort(Control *ctrl, KeyXform xf, SegTables T,
-	const uint32_t *__restrict__ split, const uint32_t *__restrict__ queue, const uint32_t *queued)
+	const uint4 *__restrict__ queue, const uint32_t *queued)
 {
 	using W = typename KeyTraits<K>::W;
 	constexpr uint32_t CAP = THREADS * KPT;
@@ -555,22 +559,20 @@ __global__ __launch_bounds__(THREADS) void k_bucket_sort(Control *ctrl, KeyXform
 	__shared__ uint32_t s_big;
 	const PassPlan pp = ctrl->hyb[2];
 	if (!pp.active || blockIdx.x >= *queued) return;
-	const uint32_t bk = queue[blockIdx.x], sg = bk >> 8, id = bk & 255u;
-	const uint32_t len = T.bucket_len[bk];
-	const uint32_t start = T.seg_start[sg] + T.bucket_start[bk];
+	const uint4 entry = queue[blockIdx.x];
+	const uint32_t start = entry.x, len = entry.y;
 	const K *src = static_cast<const K *>(pp.src) + start;
 	K *dst = static_cast<K *>(pp.dst) + start;
 	const uint32_t *psrc = PAY ? pp.psrc + start : nullptr;
 	uint32_t *pdst = PAY ? pp.pdst + start : nullptr;
-	const uint32_t *tab = split + static_cast<size_t>(1 + sg) * kSplitTabWords;
-	if (len == 1 || ((tab[kSplitEqWord + (id >> 5)] >> (id & 31u)) & 1u)) {
+	if (len == 1 || entry.w) {
 		// equal keys: the partitions kept them in order
 		block_copy<K, PAY, THREADS>(src, dst, psrc, pdst, len);
 		return;
 	}
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	if (len > CAP) {
-		if (t == 0) T.cursor[atomicAdd(&ctrl->hyb_queue, 1u)] = bk;
+		if (t == 0) T.bq[2 * kHybBuckets + atomicAdd(&ctrl->hyb_queue, 1u)] = entry;
 		return;
 	}
 	K key[KPT];
@@ -620,7 +622,7 @@ __global__ __launch_bounds__(THREADS) void k_bucket_sort(Control *ctrl, KeyXform
 	}
 	const int hb = sizeof(W) == 8 ? 63 - __clzll(static_cast<long long>(diff)) : 31 - __clz(static_cast<int>(diff));
 	if (hb >= WB - LB) {  // (key - min) and the position do not fit one word
-		if (t == 0) T.cursor[atomicAdd(&ctrl->hyb_queue, 1u)] = bk;
+		if (t == 0) T.bq[2 * kHybBuckets + atomicAdd(&ctrl->hyb_queue, 1u)] = entry;
 		return;
 	}
 	const uint32_t sh = hb + 1 >= PB ? static_cast<uint32_t>(hb + 1 - PB) : 0u;
@@ -643,7 +645,7 @@ __global__ __launch_bounds__(THREADS) void k_bucket_sort(Control *ctrl, KeyXform
 	if (big) s_big = 1;
 	__syncthreads();
 	if (s_big) {
-		if (t == 0) T.cursor[atomicAdd(&ctrl->hyb_queue, 1u)] = bk;
+		if (t == 0) T.bq[2 * kHybBuckets + atomicAdd(&ctrl->hyb_queue, 1u)] = entry;
 		return;
 	}
 	uint32_t run = incl - sum;
@@ -707,7 +709,7 @@ __global__ __launch_bounds__(THREADS) void k_bucket_sort(Control *ctrl, KeyXform
 	}
 }
 
-// The buckets k_bucket_sort queued (T.cursor[i], i < ctrl->hyb_queue), persistent blocks: a bucket
+// The buckets k_bucket_sort passed on (T.bq[131072 + i], i < ctrl->hyb_queue), persistent blocks: a bucket
 // that fits the block takes stable LDS LSD passes over the digits of (sort key - min) that vary
 // (block ranking, payload alongside); a longer one takes the same passes through HBM, CAP keys at
 // a time, between its range and the range of the other partition array (free since the second
@@ -732,9 +734,8 @@ __global__ __launch_bounds__(THREADS) void k_bucket_hard(const Control *ctrl, Ke
 	uint32_t *s_p = reinterpret_cast<uint32_t *>(s_k);
 	for (uint32_t qi = blockIdx.x; qi < queued; qi += gridDim.x) {
 		__syncthreads();  // the previous bucket's last reads of the shared arrays are done
-		const uint32_t bk = T.cursor[qi], sg = bk >> 8;
-		const uint32_t len = T.bucket_len[bk];
-		const uint32_t start = T.seg_start[sg] + T.bucket_start[bk];
+		const uint4 entry = T.bq[2 * kHybBuckets + qi];
+		const uint32_t start = entry.x, len = entry.y;
 		const K *src = static_cast<const K *>(pp.src) + start;
 		K *dst = static_cast<K *>(pp.dst) + start;
 		const uint32_t *psrc = PAY ? pp.psrc + start : nullptr;

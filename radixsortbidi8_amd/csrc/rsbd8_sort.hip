@@ -55,7 +55,7 @@ constexpr uint32_t kRanges = 8;           // key ranges of the MSD plan's first 
 
 // The MSD plan's tables (see kSegTableWords): tile_start[257], seg_start[256], seg_len[256],
 // bucket_start[65536] (relative to the bucket's segment), bucket_len[65536], queue[65536],
-// cursor[65536], ovf[65536], rcnt[8 * 256], queue2[65536].
+// cursor[65536], ovf[65536], rcnt[8 * 256], bq[3 * 65536].
 struct SegTables {
 	uint32_t *tile_start;
 	uint32_t *seg_start;
@@ -66,7 +66,10 @@ struct SegTables {
 	uint32_t *cursor;
 	uint32_t *ovf;
 	uint32_t *rcnt;
-	uint32_t *queue2;
+	// the sample partition plan's bucket queues, entries {absolute start, length, bucket (segment *
+	// 256 + id), equal keys}: [0, 65536) the buckets of at most kBucketCapA keys, [65536, 131072)
+	// the others and the equality buckets, [131072, 196608) those the LDS sorts pass on
+	uint4 *bq;
 };
 // the splitter tables (kSplitTables of kSplitTabWords words each): the first partition's, then one
 // per first-partition segment
@@ -98,8 +101,10 @@ __device__ __forceinline__ void split_load(const uint32_t *__restrict__ tab, Spl
 __host__ __device__ inline SegTables seg_tables(uint32_t *base)
 {
 	uint32_t *b = base + 257 + 512;
+	uint32_t *q = b + 5 * kHybBuckets + 8 * kRadix;
+	q += (4 - (reinterpret_cast<uintptr_t>(q) / 4) % 4) % 4;  // 16-byte entries (the base is 256-byte aligned)
 	return SegTables{base, base + 257, base + 257 + 256, b, b + kHybBuckets, b + 2 * kHybBuckets, b + 3 * kHybBuckets,
-		b + 4 * kHybBuckets, b + 5 * kHybBuckets, b + 5 * kHybBuckets + 8 * kRadix};
+		b + 4 * kHybBuckets, b + 5 * kHybBuckets, reinterpret_cast<uint4 *>(q)};
 }
 
 // 0: a real sort is needed; 1: the output is the input as it is; 2: the output is the input
@@ -1759,7 +1764,8 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 		hipLaunchKernelGGL((k_split_count2<K, 256>), dim3(cgrid), dim3(256), 0, stream, &ctrl->hyb[1], xf, TILE, m, counts,
 			split + kSplitTabWords);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
-		hipLaunchKernelGGL(k_split_segscan, dim3(kRadix), dim3(256), 0, stream, ctrl, T, counts, kTiers ? kBucketCapA : 0u);
+		hipLaunchKernelGGL(k_split_segscan, dim3(kRadix), dim3(256), 0, stream, ctrl, T, counts, split + kSplitTabWords,
+			kTiers ? kBucketCapA : 0u);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 	}
 	{
@@ -1777,13 +1783,13 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 			const uint32_t agrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(std::min<uint64_t>(n, kHybBuckets),
 				2 * n / target + 512)));
 			hipLaunchKernelGGL((k_bucket_sort<K, PAY, 256, kBucketCapA / 256, 10>), dim3(agrid), dim3(256), 0, stream, ctrl, xf, T,
-				split, T.queue, &ctrl->bucket_count);
+				T.bq, &ctrl->bucket_count);
 			if ((e = hipGetLastError()) != hipSuccess) return e;
 		}
 		const uint32_t bgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(std::min<uint64_t>(n, kHybBuckets),
 			kTiers ? n / kBucketCapA + 1 : 2 * n / target + 512)));
-		hipLaunchKernelGGL((k_bucket_sort<K, PAY, 1024, 8, 11>), dim3(bgrid), dim3(1024), 0, stream, ctrl, xf, T, split, T.queue2,
-			&ctrl->bucket_count2);
+		hipLaunchKernelGGL((k_bucket_sort<K, PAY, 1024, 8, 11>), dim3(bgrid), dim3(1024), 0, stream, ctrl, xf, T,
+			T.bq + kHybBuckets, &ctrl->bucket_count2);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 		hipLaunchKernelGGL((k_bucket_hard<K, PAY, 1024, 8>), dim3(2 * cus), dim3(1024), 0, stream, ctrl, xf, T);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
