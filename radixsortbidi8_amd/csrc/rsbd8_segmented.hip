@@ -1001,12 +1001,13 @@ hipError_t launch_lsd_t(const void *src, void *dst, const uint64_t *istart, cons
 	return hipGetLastError();
 }
 
-// capacity tiers: 0 = 256x4 (1024 keys), 1 = 256x16 (4096), 2 = 512x16 (8192), partition bits chosen
+// capacity tiers: 0 = 256x4 (1024 keys), 1 = 512x8 (4096), 2 = 512x16 (8192), partition bits chosen
 // for about four keys per partition (two in the 4096 tier, whose counters take 16-bit halves: its
-// LDS, 29 KB, still lets four blocks share a CU). Measured on the C4 batch scaled to 2^28 keys, the
-// 4096 tier's sort takes 1.30-1.34 ms whether partitions hold four, two or one key (with one, 33 KB
-// of LDS leaves three blocks per CU); its counters put it at about 95 vector instructions per key,
-// so that it is bound by instruction issue rather than by memory.
+// LDS, 29 KB, lets four blocks of eight waves share a CU). Measured on 65,536 int32 segments of
+// log-uniform lengths scaled to 2^28 keys, the 4096 tier's sort took 1.30-1.34 ms with 256x16
+// whether partitions held four, two or one key (about 95 vector instructions per key); 512x8, with
+// twice the waves per CU to hide the LDS latency of its phases, takes the whole graph from 2.36 to
+// 2.25 ms. 512x2 for the 1024 tier measured slower (2.38 ms).
 template <typename K>
 hipError_t launch_sort_cfg(int cfg, const void *src, void *dst, const uint64_t *istart, const uint32_t *ilen,
 	const uint32_t *count, uint32_t count_value, const KeyXform &xf, int reverse, uint32_t *queue, uint32_t *qcount,
@@ -1014,7 +1015,7 @@ hipError_t launch_sort_cfg(int cfg, const void *src, void *dst, const uint64_t *
 {
 	switch (cfg) {
 	case 0: return launch_sort_t<K, 256, 4, 8>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase, s);
-	case 1: return launch_sort_t<K, 256, 16, 11>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase, s);
+	case 1: return launch_sort_t<K, 512, 8, 11>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase, s);
 	default: return launch_sort_t<K, 512, 16, 11>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase, s);
 	}
 }
