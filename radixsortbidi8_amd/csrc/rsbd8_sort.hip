@@ -1707,10 +1707,10 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 	const K *in = static_cast<const K *>(job.in);
 	const uint32_t st = g_sample_target.load();
 	const uint32_t target = st ? st : kSampleTarget;
-	// Two per-bucket sorts (kBucketCapA keys in 256-thread blocks, then the rest) for 8-byte keys,
-	// whose 8192-key blocks hold one per CU: 2^28 f64 N(0,1) 2.97 ms in the bucket sorts against
-	// 3.63 with the large one only; 4-byte keys keep the large one (2^28 f32 + index: 1.89 ms
-	// against 2.17 with both).
+	// Two per-bucket sorts (kBucketCapA keys in 512-thread blocks, then the rest) for 8-byte keys,
+	// whose 8192-key blocks hold one per CU (2^28 f64 N(0,1): 2.97 ms in the bucket sorts against
+	// 3.63 with the large one only, when the smaller blocks had 256 threads; 2.2 ms with 512); 4-byte
+	// keys keep the large one (2^28 f32 + index: 1.89 ms against 2.17 with both).
 	constexpr bool kTiers = sizeof(K) == 8;
 	hipError_t e;
 	if (job.presort_check) {
@@ -1806,16 +1806,19 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 	return hipGetLastError();
 }
 
-// The partitions take 512x8 tiles whatever the configuration: with the splitter table in LDS the
-// 1024x12 tiles of the LSD passes spill registers (measured at 2^28 f32 with a payload: 2.3 ms per
-// partition against 1.75).
+// The partitions take 1024x8 tiles (512x8 for 8-byte keys without a payload, whose 1024x8 tiles
+// take 104 KB of LDS, one block per CU: 2^28 f64 N(0,1) 7.48 ms against 7.26), whatever the
+// configuration: with the splitter table in LDS, 1024x12 and larger tiles spill registers. Against
+// 512x8, the larger tiles halve the tile count rows the counting reads write and the scans read
+// (2^28 f32 with an index payload: 7.13 -> 6.50 ms; u64 with a payload: 8.27 -> 7.44 ms).
 template <typename K>
 hipError_t launch_sample(int cfg, const SortJob &job, void *ws, hipStream_t stream)
 {
 	(void)cfg;
 	const KeyXform xf = make_xform(8 * sizeof(K), job.flags);
-	if (job.payload) return launch_sample_cfg<K, true, 512, 8>(job, ws, xf, stream);
-	return launch_sample_cfg<K, false, 512, 8>(job, ws, xf, stream);
+	if (job.payload) return launch_sample_cfg<K, true, 1024, 8>(job, ws, xf, stream);
+	if constexpr (sizeof(K) == 8) return launch_sample_cfg<K, false, 512, 8>(job, ws, xf, stream);
+	else return launch_sample_cfg<K, false, 1024, 8>(job, ws, xf, stream);
 }
 
 // One side stream per device (default priority, not synchronising with the null stream) and the
