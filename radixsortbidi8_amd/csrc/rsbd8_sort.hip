@@ -1919,17 +1919,20 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	int algo = job.algo ? job.algo : default_algo(P, n, keys_only, float_keys);
 	if (!job.algo) {
 		// The automatic plan and the sample partition plan (measured on MI355X, uniform and N(0,1)
-		// keys, against the LSD passes): 8-byte keys take it from 2^24 when they are floats (f64
-		// N(0,1) keys only at 2^24 / 2^26 / 2^28: 0.81 / 2.25 / 8.1 ms against 1.03 / 3.85 / 14.0;
-		// with a u32 index: 0.92 / 2.57 against 1.30 / 5.17) or keys-only under a transform that is
-		// not injective, and from 2^25 for integers with a payload (u64 + index at 2^25 / 2^27 /
-		// 2^28: 1.77 / 4.48 / 9.2 ms against 2.68 / 9.67 / 19.1; at 2^24 the wide bucket ranges of
-		// uniform 64-bit keys keep it behind, 1.38 against 1.30). 4-byte keys keep the LSD passes
-		// (f32 N(0,1) + index at 2^28: 7.5 ms against 6.45; keys only 6.1 against 5.0).
+		// keys, against the LSD passes): 8-byte keys take it from 2^22 when they are floats (f64
+		// N(0,1) keys only at 2^21 / 2^22 / 2^24 / 2^26 / 2^28: 0.37 / 0.43 / 0.75 / 2.12 / 7.3 ms
+		// against 0.35 / 0.45 / 1.03 / 3.85 / 14.0; with a u32 index: 0.40 / 0.45 / 0.84 / 2.30
+		// against 0.37 / 0.50 / 1.29 / 5.17), from 2^24 when keys-only under a transform that is not
+		// injective (i64 absolute values at 2^23 / 2^24: 0.75 / 0.80 ms against 0.62 / 1.02), and
+		// from 2^25 for integers with a payload (u64 + index at 2^24 / 2^25 / 2^27 / 2^28: 1.35 /
+		// 1.66 / 3.76 / 7.4 ms against 1.30 / 2.68 / 9.68 / 19.1: below 2^25 the wide bucket ranges
+		// of uniform 64-bit keys keep it behind). 4-byte keys keep the LSD passes (f32 N(0,1) +
+		// index at 2^26 / 2^28: 1.82 / 6.49 ms against 1.76 / 6.48; keys only 1.59 / 5.61 against
+		// 1.34 / 5.01).
 		const bool inj = xform_injective(xf, P);
 		if constexpr (P == 8) {
 			if (job.buf && ((keys_only && (float_keys || !inj)) || job.payload) &&
-				n >= (float_keys || keys_only ? (1ull << 24) : (1ull << 25)))
+				n >= (float_keys ? (1ull << 22) : keys_only ? (1ull << 24) : (1ull << 25)))
 				return launch_sample<K>(cfg, job, ws, stream);
 		}
 		if (P == 4 && algo == kAlgoHybrid && (float_keys || !inj)) algo = kAlgoCountScan;
@@ -2175,9 +2178,10 @@ hipError_t profile_end(double *ms, unsigned *launches, int nkinds)
 // measured on MI355X (uniform keys):
 // - keys-only sorts take the two-level MSD plan from 2^18 keys for 8-byte integer keys (2^20 u64:
 //   223 us against 296 for onesweep; 2^24: 427 against 1065 for count + scan) and from 2^22 for
-//   4-byte integer keys (2^24 u32: 301 us against 411); float keys from 2^24 (algorithm 4 sorts
-//   them with the sample partition plan: 8-byte floats are ahead of the LSD passes from there, and
-//   run_sort keeps 4-byte floats and non-injective 4-byte modes on the LSD passes);
+//   4-byte integer keys (2^24 u32: 301 us against 411); 8-byte float keys from 2^22 and 4-byte
+//   ones from 2^24 (algorithm 4 sorts them with the sample partition plan: 8-byte floats are ahead
+//   of the LSD passes from 2^22, and run_sort keeps 4-byte floats and non-injective 4-byte modes on
+//   the LSD passes);
 // - otherwise the count + scan path from 2^22 keys, the onesweep path below (fewer launches);
 // - keys-only sorts of up to kLdsMax keys: one block sorts them in LDS, one launch instead of a dozen
 //   (2^10 / 2^14 u32: 9 / 39 us against 49 / 66 for onesweep; u64: 16 / 85 against 87 / 124; u8 2^14:
@@ -2186,7 +2190,8 @@ int default_algo(int key_bytes, uint64_t n, bool keys_only, bool float_keys)
 {
 	if (keys_only && n <= kLdsMax) return kAlgoLds;
 	if (keys_only && key_bytes >= 4 && local_cfg_for(n) >= 0) {
-		const uint64_t from = float_keys ? (1ull << 24) : (key_bytes >= 8 ? (1ull << 18) : (1ull << 22));
+		const uint64_t from = float_keys ? (key_bytes >= 8 ? (1ull << 22) : (1ull << 24))
+		                                 : (key_bytes >= 8 ? (1ull << 18) : (1ull << 22));
 		if (n >= from) return kAlgoHybrid;
 	}
 	return n >= (1u << 22) ? kAlgoCountScan : kAlgoOnesweep;
