@@ -422,22 +422,35 @@ __device__ __forceinline__ void for_each_key(const K *__restrict__ p, uint64_t l
 	for (uint32_t i = nvec * VEC + threadIdx.x; i < body; i += THREADS) f(qa[i]);
 }
 
+// bit 0 for +0.0, bit 1 for -0.0 (the raw key has no bit set below its top one), else 0
+template <typename K>
+__device__ __forceinline__ uint32_t zero_sign_bit(K raw)
+{
+	constexpr int B = 8 * static_cast<int>(sizeof(K));
+	return (raw << 1) == K(0) ? (1u << static_cast<uint32_t>(raw >> (B - 1))) : 0u;
+}
+
 // Counts the splitter bucket id of every key of the physical range [lo, lo + len) of p into s_t
 // (C counter copies), reading as for_each_key does; the ids of all the vectors a thread holds are
 // searched before any is counted, so that the searches' LDS loads overlap instead of each waiting
-// behind the previous key's counter update.
-template <typename K, int THREADS, int C>
-__device__ __forceinline__ void count_split_ids(const K *__restrict__ p, uint64_t lo, uint32_t len, const KeyXform &xf,
+// behind the previous key's counter update. With ZCHK, returns the zero_sign_bit of the thread's
+// keys ORed together (else 0).
+template <typename K, int THREADS, int C, bool ZCHK = false>
+__device__ __forceinline__ uint32_t count_split_ids(const K *__restrict__ p, uint64_t lo, uint32_t len, const KeyXform &xf,
 	const SplitLds<typename KeyTraits<K>::W> &tb, uint32_t *s_t)
 {
 	using W = typename KeyTraits<K>::W;
 	constexpr uint32_t VEC = 16 / sizeof(K);
-	constexpr int INFLIGHT = 8;
+	constexpr int INFLIGHT = ZCHK ? 4 : 8;
 	const K *q = p + lo;
+	uint32_t zf = 0;
 	const uint32_t mis = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(q) & 15u) / static_cast<uint32_t>(sizeof(K));
 	uint32_t head = mis ? VEC - mis : 0u;
 	if (head > len) head = len;
-	if (threadIdx.x < head) wave_count<C>(s_t, split_id<W>(tb, sort_key<K>(q[threadIdx.x], xf)));
+	if (threadIdx.x < head) {
+		wave_count<C>(s_t, split_id<W>(tb, sort_key<K>(q[threadIdx.x], xf)));
+		if constexpr (ZCHK) zf |= zero_sign_bit<K>(q[threadIdx.x]);
+	}
 	const K *qa = q + head;
 	const uint32_t body = len - head;
 	const uint32_t nvec = body / VEC;
@@ -448,6 +461,17 @@ __device__ __forceinline__ void count_split_ids(const K *__restrict__ p, uint64_
 		for (int jj = 0; jj < INFLIGHT; ++jj) {
 			const uint32_t idx = v + jj * THREADS;
 			u[jj] = idx < nvec ? qv[idx] : make_uint4(0, 0, 0, 0);
+		}
+		if constexpr (ZCHK) {
+#pragma unroll
+			for (int jj = 0; jj < INFLIGHT; ++jj) {
+				K ks[VEC];
+				__builtin_memcpy(ks, &u[jj], 16);
+				uint32_t vz = 0;
+#pragma unroll
+				for (uint32_t e = 0; e < VEC; ++e) vz |= zero_sign_bit<K>(ks[e]);
+				zf |= v + jj * THREADS < nvec ? vz : 0u;  // (the vectors past nvec are zero-filled)
+			}
 		}
 		uint32_t id[INFLIGHT][VEC];
 #pragma unroll
@@ -464,8 +488,11 @@ __device__ __forceinline__ void count_split_ids(const K *__restrict__ p, uint64_
 			for (uint32_t e = 0; e < VEC; ++e) wave_count<C>(s_t, id[jj][e]);
 		}
 	}
-	for (uint32_t i = nvec * VEC + threadIdx.x; i < body; i += THREADS)
+	for (uint32_t i = nvec * VEC + threadIdx.x; i < body; i += THREADS) {
 		wave_count<C>(s_t, split_id<W>(tb, sort_key<K>(qa[i], xf)));
+		if constexpr (ZCHK) zf |= zero_sign_bit<K>(qa[i]);
+	}
+	return zf;
 }
 
 } // namespace rsbd8hip

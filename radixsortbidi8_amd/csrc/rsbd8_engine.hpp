@@ -53,6 +53,8 @@ struct Control {
 	uint32_t joint_max;    // largest bucket of the top-16-bit histogram (the exact MSD plan test)
 	uint32_t bucket_count; // sample partition plan: buckets queued for the smaller per-bucket sort
 	uint32_t bucket_count2;// ... and for the larger one
+	uint32_t zero_signs;   // sample partition plan's first read: bit 0 a +0.0 key seen, bit 1 a -0.0 key
+	uint32_t group_parts;  // sample partition plan: 1 when its partitions group (unstable), 0 when they rank
 };
 
 constexpr size_t kCtrlBytes = 4096;

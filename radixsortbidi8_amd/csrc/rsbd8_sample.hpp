@@ -260,11 +260,12 @@ __global__ __launch_bounds__(THREADS) void k_samp_l2(const Control *ctrl, KeyXfo
 }
 
 // The first read: per-tile counts of the first bucket id (tiles in the logical order of the first
-// partition) and the bucket histogram (hist[0..255], zero on entry).
-template <typename K, int THREADS>
+// partition) and the bucket histogram (hist[0..255], zero on entry); with ZCHK, the signs of the
+// zero keys seen into ctrl->zero_signs (zero_sign_bit).
+template <typename K, int THREADS, bool ZCHK>
 __global__ __launch_bounds__(THREADS) void k_split_count1(const K *__restrict__ in, uint64_t n, KeyXform xf,
 	uint32_t *__restrict__ hist, uint32_t *__restrict__ counts, uint32_t tile_keys, uint32_t num_tiles, int reverse,
-	const Control *ctrl, const uint32_t *__restrict__ split, const uint32_t *gate)
+	Control *ctrl, const uint32_t *__restrict__ split, const uint32_t *gate)
 {
 	static_assert(THREADS == kRadix, "one thread per bucket id");
 	using W = typename KeyTraits<K>::W;
@@ -276,13 +277,14 @@ __global__ __launch_bounds__(THREADS) void k_split_count1(const K *__restrict__ 
 	split_load<W, THREADS>(split, s_tab);
 	const uint32_t t = threadIdx.x;
 	uint32_t mine = 0;  // this block's count of bucket id t
+	uint32_t zf = 0;
 	for (uint32_t tile = blockIdx.x; tile < num_tiles; tile += gridDim.x) {
 		for (int i = t; i < kRadix * C; i += THREADS) s_t[i] = 0;
 		__syncthreads();
 		const uint64_t base = static_cast<uint64_t>(tile) * tile_keys;
 		const uint32_t len = (n - base) < tile_keys ? static_cast<uint32_t>(n - base) : tile_keys;
 		const uint64_t lo = reverse ? n - base - len : base;
-		count_split_ids<K, THREADS, C>(in, lo, len, xf, s_tab, s_t);
+		zf |= count_split_ids<K, THREADS, C, ZCHK>(in, lo, len, xf, s_tab, s_t);
 		__syncthreads();
 		const uint32_t v = copies_sum<C>(s_t, t);
 		counts[static_cast<uint64_t>(tile) * kCountStride + t] = v;
@@ -290,6 +292,10 @@ __global__ __launch_bounds__(THREADS) void k_split_count1(const K *__restrict__ 
 		__syncthreads();
 	}
 	if (mine) atomicAdd(&hist[t], mine);
+	if constexpr (ZCHK) {
+		const uint32_t wz = (__ballot((zf & 1u) != 0) ? 1u : 0u) | (__ballot((zf & 2u) != 0) ? 2u : 0u);
+		if ((threadIdx.x & 63u) == 0 && wz) atomicOr(&ctrl->zero_signs, wz);
+	}
 }
 
 // The second count: per-tile counts of the bucket id under the tile's segment table (split +
@@ -388,6 +394,7 @@ __global__ __launch_bounds__(256) void k_split_segscan(Control *ctrl, SegTables 
 // One block: the first partition's bucket starts (bucket[0..255]) and segment map, hist[0..255]
 // zeroed for the next sort, and the pass plans: hyb[0] input -> X1, hyb[1] X1 -> X2, hyb[2] X2 ->
 // the output, the payload arrays alongside. An input already in order becomes the final copy.
+// ctrl->group_parts selects the partitions' form (see PlanArgs::group).
 __global__ __launch_bounds__(256) void k_plan_sample(PlanArgs a, Control *ctrl, uint32_t *hist, uint32_t *bucket)
 {
 	__shared__ uint32_t s_wave[4], s_tw[4];
@@ -423,6 +430,7 @@ __global__ __launch_bounds__(256) void k_plan_sample(PlanArgs a, Control *ctrl, 
 	ctrl->counts_ready = 0;
 	ctrl->active_mask = 0;
 	ctrl->num_active = 0;
+	ctrl->group_parts = (a.group == 1 || (a.group == 2 && ctrl->zero_signs != 3u)) ? 1u : 0u;
 	const uint32_t pres = presort_result(ctrl->check, a.reverse);
 	if (pres) {
 		PassPlan &f = ctrl->fin;

@@ -292,6 +292,9 @@ struct PlanArgs {
 	uint32_t tile2;     // tile size of the segmented second pass
 	uint32_t *segtab;   // the MSD plan's tables
 	const uint32_t *gate;  // non-null: plan only when it reads nonzero (the full plan after a fallback)
+	int group;          // sample partition plan, keys only: equal sort keys are equal keys (1), or are
+	                    // unless both +0.0 and -0.0 occur (2), so the partitions may group instead of
+	                    // rank; 0: they rank
 };
 
 __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_t *hist, uint32_t *bucket)
@@ -893,19 +896,24 @@ __device__ __forceinline__ void seg_tile(const uint32_t *s_ts, const SegMap &m, 
 // With SPL the digit is not a byte of the sort key but its bucket id under a splitter table (the
 // sample partition plan): SPL 1, the table at split for every tile; SPL 2 (with SEG), the table of
 // the tile's segment, split + segment * kSplitTabWords. The digits are staged beside the keys, so
-// the stores do not search again.
+// the stores do not search again. With GROUP and SPL (keys only, tile offsets from counts) the
+// per-bucket sort that follows orders each bucket by the whole sort key, and the sample plan takes
+// these passes only when equal sort keys are equal keys. A non-null gsel selects between the two
+// forms of a pass launched side by side: the grouping form runs when *gsel is 1, the ranking form
+// when it is 0.
 template <typename K, bool PAY, int THREADS, int KPT, bool SEG = false, bool GROUP = false, bool CUR = false, int SPL = 0>
 __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32_t shift, KeyXform xf, uint64_t n,
 	uint32_t num_tiles, const uint32_t *__restrict__ offsets, SegMap seg, uint32_t *__restrict__ cursor,
-	const uint32_t *__restrict__ split)
+	const uint32_t *__restrict__ split, const uint32_t *gsel)
 {
 	static_assert(!CUR || GROUP, "cursors are for the grouping passes");
-	static_assert(!(SPL && GROUP) && (SPL != 2 || SEG), "splitter digits: ranked passes; per-segment tables on segment tiles");
+	static_assert(!(SPL && CUR) && (SPL != 2 || SEG), "splitter digits: counted passes; per-segment tables on segment tiles");
 	using W = typename KeyTraits<K>::W;
 	constexpr int TILE = THREADS * KPT;
 	constexpr int NW = THREADS / 64;
 	const PassPlan pp = *ppp;
 	if (!pp.active) return;
+	if (gsel && *gsel != (GROUP ? 1u : 0u)) return;
 	const K *__restrict__ src = static_cast<const K *>(pp.src);
 	K *__restrict__ dst = static_cast<K *>(pp.dst);
 	__shared__ uint32_t s_whist[NW * kRadix];
@@ -1008,6 +1016,11 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 #pragma unroll
 			for (int j = 0; j < KPT; ++j)
 				if (pos[j] != 0xFFFFFFFFu) s_keys[pos[j]] = keys[j];
+			if constexpr (SPL) {
+#pragma unroll
+				for (int j = 0; j < KPT; ++j)
+					if (pos[j] != 0xFFFFFFFFu) s_dig[pos[j]] = static_cast<uint8_t>(dig[j]);
+			}
 		} else {
 			block_rank<THREADS, KPT>(dig, pos, s_whist, s_digoff, s_wsum, &total, s_peer);
 			if (t < kRadix) s_glob[t] = offsets[static_cast<uint64_t>(tile) * kCountStride + t] - s_digoff[t];
@@ -1423,6 +1436,7 @@ __device__ __forceinline__ void reset_control(Control *ctrl)
 		ctrl->joint_max = 0;
 		ctrl->bucket_count = 0;
 		ctrl->bucket_count2 = 0;
+		ctrl->zero_signs = 0;
 	}
 }
 
@@ -1553,7 +1567,8 @@ hipError_t launch_pass_cfg(int cfg, Control *ctrl, const uint32_t *bucket, uint6
 
 template <typename K, bool PAY, int THREADS, int KPT, bool SEG = false, bool GROUP = false, bool CUR = false, int SPL = 0>
 hipError_t launch_scatter(const PassPlan *pp, uint32_t shift, const KeyXform &xf, uint64_t n, uint32_t tiles,
-	const uint32_t *offsets, SegMap seg, hipStream_t stream, uint32_t *cursor = nullptr, const uint32_t *split = nullptr)
+	const uint32_t *offsets, SegMap seg, hipStream_t stream, uint32_t *cursor = nullptr, const uint32_t *split = nullptr,
+	const uint32_t *gsel = nullptr)
 {
 	static int occ = -1;
 	if (occ < 0) {
@@ -1567,7 +1582,7 @@ hipError_t launch_scatter(const PassPlan *pp, uint32_t shift, const KeyXform &xf
 	const uint32_t want = static_cast<uint32_t>(occ * devinfo().cus * 2 + 7) / 8;
 	const uint32_t grid = 8u * std::max<uint32_t>(1u, std::min<uint32_t>(per, want));
 	hipLaunchKernelGGL((k_scatter<K, PAY, THREADS, KPT, SEG, GROUP, CUR, SPL>), dim3(grid), dim3(THREADS), 0, stream, pp, shift, xf,
-		n, tiles, offsets, seg, cursor, split);
+		n, tiles, offsets, seg, cursor, split, gsel);
 	return hipGetLastError();
 }
 
@@ -1692,6 +1707,22 @@ constexpr uint32_t kSampleTarget = 4096;
 std::atomic<uint32_t> g_sample_target{0};
 
 // The sample partition plan (rsbd8_sample.hpp) with THREADS x KPT partition tiles.
+// Whether the sample partition plan may group instead of rank (PlanArgs::group): keys-only sorts
+// whose transform is one to one (unsigned, signed, the tiered absolute values and the special
+// float order: equal sort keys are equal keys, so no order among them can be seen), and the float
+// order, where +0.0 and -0.0 alone share a sort key (the first read checks that they do not both
+// occur); never with a payload, nor for the absolute-value modes.
+int sample_group_class(const SortJob &job)
+{
+	if (job.payload || job.keys_final_any) return 0;
+	switch (job.flags & 7u) {
+	case kAbs | kSign:
+	case kAbs | kSign | kFltp: return 0;
+	case kSign | kFltp: return 2;
+	default: return 1;
+	}
+}
+
 template <typename K, bool PAY, int THREADS, int KPT>
 hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, hipStream_t stream)
 {
@@ -1712,6 +1743,7 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 	// 3.63 with the large one only, when the smaller blocks had 256 threads; 2.2 ms with 512); 4-byte
 	// keys keep the large one (2^28 f32 + index: 1.89 ms against 2.17 with both).
 	constexpr bool kTiers = sizeof(K) == 8;
+	const int group = sample_group_class(job);
 	hipError_t e;
 	if (job.presort_check) {
 		const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 4095) / 4096, 4u * cus)));
@@ -1728,8 +1760,12 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 		hipLaunchKernelGGL((k_samp_l1<K, 1024>), dim3(1), dim3(1024), 0, stream, ctrl, rev, split, nullptr);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 		const uint32_t grid = std::max<uint32_t>(1u, std::min<uint32_t>(tiles, 4u * cus));
-		hipLaunchKernelGGL((k_split_count1<K, 256>), dim3(grid), dim3(256), 0, stream, in, n, xf, hist, counts, TILE, tiles, rev,
-			ctrl, split, nullptr);
+		if (group == 2)
+			hipLaunchKernelGGL((k_split_count1<K, 256, true>), dim3(grid), dim3(256), 0, stream, in, n, xf, hist, counts, TILE, tiles,
+				rev, ctrl, split, nullptr);
+		else
+			hipLaunchKernelGGL((k_split_count1<K, 256, false>), dim3(grid), dim3(256), 0, stream, in, n, xf, hist, counts, TILE,
+				tiles, rev, ctrl, split, nullptr);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 	}
 	PlanArgs pa{};
@@ -1743,6 +1779,7 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 	pa.tile2 = TILE;
 	pa.segtab = segtab_of(ws);
 	pa.gate = nullptr;
+	pa.group = group;
 	hipLaunchKernelGGL(k_plan_sample, dim3(1), dim3(256), 0, stream, pa, ctrl, hist, bucket);
 	if ((e = hipGetLastError()) != hipSuccess) return e;
 	{
@@ -1753,11 +1790,24 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 			kMaxPasses, counts, lookback, tiles);
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 	}
+	// the partitions' forms: ranked (group 0), grouping (group 1), or both, each gated on the plan's
+	// choice (group 2; 2^28 f64 N(0,1) keys: 6.98 ms grouped against 7.50 ranked, f32 4.80 against
+	// 5.61)
+	const uint32_t *gsel = group == 2 ? &ctrl->group_parts : nullptr;
 	{
 		ProfScope ps(5, 0, ws, stream);
-		e = launch_scatter<K, PAY, THREADS, KPT, false, false, false, 1>(&ctrl->hyb[0], 0, xf, n, tiles, counts, SegMap{}, stream,
-			nullptr, split);
-		if (e != hipSuccess) return e;
+		if constexpr (!PAY) {
+			if (group) {
+				e = launch_scatter<K, false, THREADS, KPT, false, true, false, 1>(&ctrl->hyb[0], 0, xf, n, tiles, counts, SegMap{},
+					stream, nullptr, split, gsel);
+				if (e != hipSuccess) return e;
+			}
+		}
+		if (group != 1) {
+			e = launch_scatter<K, PAY, THREADS, KPT, false, false, false, 1>(&ctrl->hyb[0], 0, xf, n, tiles, counts, SegMap{},
+				stream, nullptr, split, gsel);
+			if (e != hipSuccess) return e;
+		}
 	}
 	const uint32_t tiles2 = tiles + kRadix;  // bound: one partial tile per segment
 	const SegMap m{T.tile_start, T.seg_start, T.seg_len};
@@ -1775,9 +1825,18 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 	}
 	{
 		ProfScope ps(5, 1, ws, stream);
-		e = launch_scatter<K, PAY, THREADS, KPT, true, false, false, 2>(&ctrl->hyb[1], 0, xf, n, tiles2, counts, m, stream,
-			nullptr, split + kSplitTabWords);
-		if (e != hipSuccess) return e;
+		if constexpr (!PAY) {
+			if (group) {
+				e = launch_scatter<K, false, THREADS, KPT, true, true, false, 2>(&ctrl->hyb[1], 0, xf, n, tiles2, counts, m, stream,
+					nullptr, split + kSplitTabWords, gsel);
+				if (e != hipSuccess) return e;
+			}
+		}
+		if (group != 1) {
+			e = launch_scatter<K, PAY, THREADS, KPT, true, false, false, 2>(&ctrl->hyb[1], 0, xf, n, tiles2, counts, m, stream,
+				nullptr, split + kSplitTabWords, gsel);
+			if (e != hipSuccess) return e;
+		}
 	}
 	{
 		ProfScope ps(6, 0, ws, stream);
@@ -1806,19 +1865,18 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 	return hipGetLastError();
 }
 
-// The partitions take 1024x8 tiles (512x8 for 8-byte keys without a payload, whose 1024x8 tiles
-// take 104 KB of LDS, one block per CU: 2^28 f64 N(0,1) 7.48 ms against 7.26), whatever the
-// configuration: with the splitter table in LDS, 1024x12 and larger tiles spill registers. Against
-// 512x8, the larger tiles halve the tile count rows the counting reads write and the scans read
-// (2^28 f32 with an index payload: 7.13 -> 6.50 ms; u64 with a payload: 8.27 -> 7.44 ms).
+// The partitions take 1024x8 tiles whatever the configuration: with the splitter table in LDS,
+// 1024x12 and larger tiles spill registers. Against 512x8, the larger tiles halve the tile count
+// rows the counting reads write and the scans read (2^28 f32 with an index payload: 7.13 -> 6.50
+// ms; u64 with a payload: 8.27 -> 7.44 ms; f64 N(0,1) keys, grouped: 7.33 -> 6.98 ms, though its
+// ranked form, 104 KB of LDS per block, goes from 7.33 to 7.50).
 template <typename K>
 hipError_t launch_sample(int cfg, const SortJob &job, void *ws, hipStream_t stream)
 {
 	(void)cfg;
 	const KeyXform xf = make_xform(8 * sizeof(K), job.flags);
 	if (job.payload) return launch_sample_cfg<K, true, 1024, 8>(job, ws, xf, stream);
-	if constexpr (sizeof(K) == 8) return launch_sample_cfg<K, false, 512, 8>(job, ws, xf, stream);
-	else return launch_sample_cfg<K, false, 1024, 8>(job, ws, xf, stream);
+	return launch_sample_cfg<K, false, 1024, 8>(job, ws, xf, stream);
 }
 
 // One side stream per device (default priority, not synchronising with the null stream) and the
@@ -1926,13 +1984,19 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 		// injective (i64 absolute values at 2^23 / 2^24: 0.75 / 0.80 ms against 0.62 / 1.02), and
 		// from 2^25 for integers with a payload (u64 + index at 2^24 / 2^25 / 2^27 / 2^28: 1.35 /
 		// 1.66 / 3.76 / 7.4 ms against 1.30 / 2.68 / 9.68 / 19.1: below 2^25 the wide bucket ranges
-		// of uniform 64-bit keys keep it behind). 4-byte keys keep the LSD passes (f32 N(0,1) +
-		// index at 2^26 / 2^28: 1.82 / 6.49 ms against 1.76 / 6.48; keys only 1.59 / 5.61 against
-		// 1.34 / 5.01).
+		// of uniform 64-bit keys keep it behind). 4-byte keys take it from 2^27 when they are floats
+		// sorted keys-only under a transform whose equal sort keys are equal keys (grouping
+		// partitions; f32 N(0,1) at 2^26 / 2^27 / 2^28: 1.42 / 2.57 / 4.80 ms against 1.35 / 2.59 /
+		// 5.02), else keep the LSD passes (f32 N(0,1) + index at 2^26 / 2^28: 1.82 / 6.49 ms against
+		// 1.76 / 6.48; i32 absolute values at 2^28: 5.29 against 4.95).
 		const bool inj = xform_injective(xf, P);
 		if constexpr (P == 8) {
 			if (job.buf && ((keys_only && (float_keys || !inj)) || job.payload) &&
 				n >= (float_keys ? (1ull << 22) : keys_only ? (1ull << 24) : (1ull << 25)))
+				return launch_sample<K>(cfg, job, ws, stream);
+		}
+		if constexpr (P == 4) {
+			if (job.buf && keys_only && float_keys && sample_group_class(job) != 0 && n >= (1ull << 27))
 				return launch_sample<K>(cfg, job, ws, stream);
 		}
 		if (P == 4 && algo == kAlgoHybrid && (float_keys || !inj)) algo = kAlgoCountScan;
@@ -2054,6 +2118,7 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	pa.tile2 = htile_keys;
 	pa.segtab = segtab_of(ws);
 	pa.gate = nullptr;
+	pa.group = 0;
 
 	hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, stream, pa, ctrl, hist, bucket);
 	if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -2180,8 +2245,8 @@ hipError_t profile_end(double *ms, unsigned *launches, int nkinds)
 //   223 us against 296 for onesweep; 2^24: 427 against 1065 for count + scan) and from 2^22 for
 //   4-byte integer keys (2^24 u32: 301 us against 411); 8-byte float keys from 2^22 and 4-byte
 //   ones from 2^24 (algorithm 4 sorts them with the sample partition plan: 8-byte floats are ahead
-//   of the LSD passes from 2^22, and run_sort keeps 4-byte floats and non-injective 4-byte modes on
-//   the LSD passes);
+//   of the LSD passes from 2^22; run_sort keeps 4-byte floats below 2^27 and non-injective 4-byte
+//   modes on the LSD passes);
 // - otherwise the count + scan path from 2^22 keys, the onesweep path below (fewer launches);
 // - keys-only sorts of up to kLdsMax keys: one block sorts them in LDS, one launch instead of a dozen
 //   (2^10 / 2^14 u32: 9 / 39 us against 49 / 66 for onesweep; u64: 16 / 85 against 87 / 124; u8 2^14:

@@ -9,7 +9,9 @@ bucket) against the oracle, bit for bit, keys and payload, through each of its p
   one partition value: stable LDS LSD passes (k_bucket_hard);
 * buckets above the LDS capacity (8192 keys), forced with a large mean-bucket target: stable LSD
   passes through HBM, in place and into the buffer;
-* the presort check, in-place sorts and sorts into the buffer.
+* the presort check, in-place sorts and sorts into the buffer;
+* grouping (unstable) partitions for keys-only sorts whose equal sort keys are equal keys, and
+  the ranked ones when +0.0 and -0.0 both occur under the float order.
 Stability is observed through payloads and through modes where distinct bit patterns rank equal.
 """
 import numpy as np
@@ -169,6 +171,54 @@ def test_sample_buckets_through_hbm(nb, kt, mode, msd):
                 assert np.array_equal(host(b if move else a).view(bits.dtype), exp_k), (d, move)
     finally:
         r.set_sample_target(prev)
+
+
+@pytest.mark.parametrize("nb,kt", [(4, r.F32), (8, r.F64)])
+def test_sample_grouping_partitions_floats(nb, kt, msd):
+    """Keys-only float sorts whose equal sort keys are equal keys take grouping (unstable)
+    partitions: the one-to-one float modes always, the float order (native) unless both +0.0 and
+    -0.0 occur. Zeros of one sign only (grouped), of both signs (ranked: their input order is
+    observable and must survive), N(0,1) without zeros, in every direction, bit for bit."""
+    rng = np.random.default_rng(600 + nb)
+    n = (1 << 20) + 4321
+    base = rng.normal(size=n).astype(FLOAT[nb]).view(UINT[nb])
+    base[base << 1 == 0] = UINT[nb](1)  # no zeros in the base data
+    neg = UINT[nb](1 << (8 * nb - 1))
+    u = rng.random(n)
+    plus, minus, both = base.copy(), base.copy(), base.copy()
+    plus[u < 0.03] = 0
+    minus[u < 0.03] = neg
+    both[u < 0.02] = 0
+    both[(u >= 0.02) & (u < 0.04)] = neg
+    a, o, b = None, None, None
+    ws = r.Workspace(kt, n)
+    for name, bits in (("normal", base), ("+0", plus), ("-0", minus), ("+-0", both)):
+        a = dev(bits).view(TORCH[kt])
+        o, b = torch.empty_like(a), torch.empty_like(a)
+        for mode in (64, r.sortingmode.forcefloatingp, r.sortingmode.specialfloatingp,
+                     r.sortingmode.forcetieredabsfloatingp):
+            for d in range(4):
+                r.radixsortcopynoalloc(a, o, b, ws, d, mode)
+                exp, _ = oracle.radixsort(bits, r.decode(kt, mode, d))
+                assert np.array_equal(host(o).view(bits.dtype), exp), (name, int(mode), d)
+
+
+@pytest.mark.parametrize("nb,kt", [(4, r.I32), (8, r.I64)])
+def test_sample_grouping_partitions_tiered_abs(nb, kt, msd):
+    """Integer keys-only sorts in the tiered absolute-value mode (one to one, so grouped) with
+    heavy values, in place and copying."""
+    rng = np.random.default_rng(700 + nb)
+    n = (1 << 20) + 99
+    bits = heavy_ints(nb, n, rng)
+    ws = r.Workspace(kt, n)
+    for d in range(4):
+        exp, _ = oracle.radixsort(bits, r.decode(kt, r.sortingmode.forcetieredabssigned, d))
+        a = dev(bits).view(TORCH[kt])
+        o, b = torch.empty_like(a), torch.empty_like(a)
+        r.radixsortcopynoalloc(a, o, b, ws, d, r.sortingmode.forcetieredabssigned)
+        assert np.array_equal(host(o).view(bits.dtype), exp), d
+        r.radixsortnoalloc(a, b, ws, False, d, r.sortingmode.forcetieredabssigned)
+        assert np.array_equal(host(a).view(bits.dtype), exp), d
 
 
 def test_sample_target_rejects_negative():
