@@ -526,6 +526,18 @@ def collect(requests, reps: int = 3, log=None):
     return X, T, M
 
 
+def request_weights(T, M, power: float):
+    """Per-request training weights: (best time / median best time) ** power, clipped to [1/16, 16]
+    (T holds log2 microseconds, M marks the timed plans). A squared error in log2 time costs the
+    same relative time at every size, but a stream's total is its large requests'; power 0 weighs
+    every request alike."""
+    best = np.where(M > 0, T, np.inf).min(axis=1)
+    best = np.where(np.isfinite(best), best, np.nan)
+    med = np.nanmedian(best)
+    w = np.exp2(power * (np.nan_to_num(best, nan=med) - med))
+    return np.clip(w, 1.0 / 16, 16.0).astype(np.float32)
+
+
 def selection_overhead_ms(selector: Selector, reps: int = 50) -> float:
     """Median wall time of rsbd8_hip_select_plan (sampling, network, the choice's round trip) on a
     2^20-key array: the fixed cost a learned choice must earn back."""
