@@ -1984,19 +1984,17 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 		// injective (i64 absolute values at 2^23 / 2^24: 0.75 / 0.80 ms against 0.62 / 1.02), and
 		// from 2^25 for integers with a payload (u64 + index at 2^24 / 2^25 / 2^27 / 2^28: 1.35 /
 		// 1.66 / 3.76 / 7.4 ms against 1.30 / 2.68 / 9.68 / 19.1: below 2^25 the wide bucket ranges
-		// of uniform 64-bit keys keep it behind). 4-byte keys take it from 2^27 when they are floats
-		// sorted keys-only under a transform whose equal sort keys are equal keys (grouping
-		// partitions; f32 N(0,1) at 2^26 / 2^27 / 2^28: 1.42 / 2.57 / 4.80 ms against 1.35 / 2.59 /
-		// 5.02), else keep the LSD passes (f32 N(0,1) + index at 2^26 / 2^28: 1.82 / 6.49 ms against
-		// 1.76 / 6.48; i32 absolute values at 2^28: 5.29 against 4.95).
+		// of uniform 64-bit keys keep it behind). 4-byte keys keep the LSD passes: f32 keys-only
+		// with grouping partitions are ahead only from 2^27 and only by a little (N(0,1) at 2^26 /
+		// 2^27 / 2^28: 1.42 / 2.57 / 4.80 ms against 1.35 / 2.59 / 5.02), while with both +0.0 and
+		// -0.0 among them the ranked partitions fall behind (5.62 against 5.02 ms at 2^28), which
+		// the plan cannot know before it has read the keys; with a payload the two are level (f32
+		// N(0,1) + index at 2^26 / 2^28: 1.82 / 6.49 ms against 1.76 / 6.48), and the absolute-value
+		// modes behind (i32 at 2^28: 5.29 against 4.95). Algorithm 4 still takes the sample plan.
 		const bool inj = xform_injective(xf, P);
 		if constexpr (P == 8) {
 			if (job.buf && ((keys_only && (float_keys || !inj)) || job.payload) &&
 				n >= (float_keys ? (1ull << 22) : keys_only ? (1ull << 24) : (1ull << 25)))
-				return launch_sample<K>(cfg, job, ws, stream);
-		}
-		if constexpr (P == 4) {
-			if (job.buf && keys_only && float_keys && sample_group_class(job) != 0 && n >= (1ull << 27))
 				return launch_sample<K>(cfg, job, ws, stream);
 		}
 		if (P == 4 && algo == kAlgoHybrid && (float_keys || !inj)) algo = kAlgoCountScan;
@@ -2245,8 +2243,8 @@ hipError_t profile_end(double *ms, unsigned *launches, int nkinds)
 //   223 us against 296 for onesweep; 2^24: 427 against 1065 for count + scan) and from 2^22 for
 //   4-byte integer keys (2^24 u32: 301 us against 411); 8-byte float keys from 2^22 and 4-byte
 //   ones from 2^24 (algorithm 4 sorts them with the sample partition plan: 8-byte floats are ahead
-//   of the LSD passes from 2^22; run_sort keeps 4-byte floats below 2^27 and non-injective 4-byte
-//   modes on the LSD passes);
+//   of the LSD passes from 2^22; run_sort keeps 4-byte floats and non-injective 4-byte modes on
+//   the LSD passes);
 // - otherwise the count + scan path from 2^22 keys, the onesweep path below (fewer launches);
 // - keys-only sorts of up to kLdsMax keys: one block sorts them in LDS, one launch instead of a dozen
 //   (2^10 / 2^14 u32: 9 / 39 us against 49 / 66 for onesweep; u64: 16 / 85 against 87 / 124; u8 2^14:
