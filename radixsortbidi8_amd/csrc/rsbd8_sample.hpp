@@ -559,13 +559,15 @@ __global__ __launch_bounds__(THREADS) void k_bucket_sort(Control *ctrl, KeyXform
 	constexpr uint32_t PER = NB / THREADS;
 	static_assert(CAP <= (1u << LB) && CAP < 65536, "positions and run starts are held in 16 bits");
 	static_assert(NB % THREADS == 0 && PB <= 16, "partition buckets per thread");
-	constexpr uint32_t kUnionWords = NB > CAP / 2 ? NB : CAP / 2;
+	// s_u: the NB cursors (after the scatter: the run ends) and, above them, the block's minimum and
+	// maximum, wave sums and overflow flag; then the final positions (u16) over all of it. Nothing
+	// else in LDS, so that two 1024x8 blocks of 8-byte keys share a CU (80 KiB each).
+	constexpr uint32_t kSmallWords = NW * (2 * sizeof(W) / 4 + 1) + 1;
+	constexpr uint32_t kUnionWords = NB + kSmallWords > CAP / 2 ? NB + kSmallWords : CAP / 2;
 	__shared__ __align__(16) W s_t[CAP];        // words, then the raw keys and the payload in final order
-	__shared__ uint32_t s_u[kUnionWords];       // cursors, then final positions (u16)
-	__shared__ uint16_t s_off[NB + 1];
-	__shared__ uint32_t s_wsum[NW];
-	__shared__ W s_min[NW], s_max[NW];
-	__shared__ uint32_t s_big;
+	__shared__ __align__(16) uint32_t s_u[kUnionWords];
+	W *s_min = reinterpret_cast<W *>(s_u + NB), *s_max = s_min + NW;
+	uint32_t *s_wsum = reinterpret_cast<uint32_t *>(s_max + NW), *s_big = s_wsum + NW;
 	const PassPlan pp = ctrl->hyb[2];
 	if (!pp.active || blockIdx.x >= *queued) return;
 	const uint4 entry = queue[blockIdx.x];
@@ -598,7 +600,7 @@ __global__ __launch_bounds__(THREADS) void k_bucket_sort(Control *ctrl, KeyXform
 			mx = max(mx, k);
 		}
 	}
-	if (t == 0) s_big = 0;
+	if (t == 0) *s_big = 0;
 	for (uint32_t i = t; i < NB; i += THREADS) s_u[i] = 0;
 #pragma unroll
 	for (int o = 32; o > 0; o >>= 1) {
@@ -651,9 +653,9 @@ __global__ __launch_bounds__(THREADS) void k_bucket_sort(Control *ctrl, KeyXform
 	}
 	const uint32_t incl = wave_inclusive_sum(sum);
 	if (lane == 63) s_wsum[w] = incl;
-	if (big) s_big = 1;
+	if (big) *s_big = 1;
 	__syncthreads();
-	if (s_big) {
+	if (*s_big) {
 		if (t == 0) T.bq[2 * kHybBuckets + atomicAdd(&ctrl->hyb_queue, 1u)] = entry;
 		return;
 	}
@@ -661,11 +663,9 @@ __global__ __launch_bounds__(THREADS) void k_bucket_sort(Control *ctrl, KeyXform
 	for (uint32_t v = 0; v < w; ++v) run += s_wsum[v];
 #pragma unroll
 	for (uint32_t k = 0; k < PER; ++k) {
-		s_off[t * PER + k] = static_cast<uint16_t>(run);
 		s_cur[t * PER + k] = run;
 		run += c[k];
 	}
-	if (t == THREADS - 1) s_off[NB] = static_cast<uint16_t>(run);
 	__syncthreads();
 #pragma unroll
 	for (int j = 0; j < KPT; ++j) {
@@ -676,7 +676,8 @@ __global__ __launch_bounds__(THREADS) void k_bucket_sort(Control *ctrl, KeyXform
 		}
 	}
 	__syncthreads();
-	// the final place of every word: its run's start plus the smaller words of its run
+	// the final place of every word: its run's start (the previous run's end) plus the smaller words
+	// of its run
 	uint32_t fin[KPT];
 #pragma unroll
 	for (int j = 0; j < KPT; ++j) {
@@ -685,7 +686,7 @@ __global__ __launch_bounds__(THREADS) void k_bucket_sort(Control *ctrl, KeyXform
 		if (i < len) {
 			const W x = s_t[i];
 			const uint32_t d = static_cast<uint32_t>((x >> LB) >> sh);
-			const uint32_t lo = s_off[d], hi = s_off[d + 1];
+			const uint32_t lo = d ? s_cur[d - 1] : 0u, hi = s_cur[d];
 			uint32_t n_less = 0;
 			for (uint32_t q = lo; q < hi; ++q) n_less += static_cast<uint32_t>(s_t[q] < x);
 			fin[j] = ((static_cast<uint32_t>(x) & ((1u << LB) - 1u)) << 16) | (lo + n_less);

@@ -1846,7 +1846,7 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 		if constexpr (kTiers) {
 			const uint32_t agrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(std::min<uint64_t>(n, kHybBuckets),
 				2 * n / target + 512)));
-			hipLaunchKernelGGL((k_bucket_sort<K, PAY, 512, kBucketCapA / 512, 10>), dim3(agrid), dim3(512), 0, stream, ctrl, xf, T,
+			hipLaunchKernelGGL((k_bucket_sort<K, PAY, 512, kBucketCapA / 512, 11>), dim3(agrid), dim3(512), 0, stream, ctrl, xf, T,
 				T.bq, &ctrl->bucket_count);
 			if ((e = hipGetLastError()) != hipSuccess) return e;
 		}
