@@ -1110,12 +1110,14 @@ __global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, c
 	static_assert(NB % THREADS == 0 && PB <= 16, "partition buckets per thread");
 	using W = typename KeyTraits<K>::W;
 	static_assert(sizeof(W) == sizeof(K), "transformed keys share the key width");
-	constexpr uint32_t kUnionWords = (NB > CAP / 2) ? NB : CAP / 2;
+	// s_u: the NB partition counters, then cursors (after the scatter: the run ends, so that a run
+	// starts where the previous one ends), the wave sums and the overflow flag above them; then the
+	// final positions (u16) over all of it
+	constexpr uint32_t kSmallWords = THREADS / 64 + 1;
+	constexpr uint32_t kUnionWords = (NB + kSmallWords > CAP / 2) ? NB + kSmallWords : CAP / 2;
 	__shared__ __align__(16) W s_t[CAP];          // packed words, then the raw keys in final order
-	__shared__ uint32_t s_u[kUnionWords];         // partition counters/cursors, then final positions (u16)
-	__shared__ uint16_t s_off[NB + 1];            // run starts (positions < CAP <= 65536)
-	__shared__ uint32_t s_wsum[THREADS / 64];
-	__shared__ uint32_t s_big;
+	__shared__ uint32_t s_u[kUnionWords];
+	uint32_t *s_wsum = s_u + NB, *s_big = s_wsum + THREADS / 64;
 	const PassPlan pp = ctrl->hyb[2];
 	if (!pp.active) return;
 	const K *__restrict__ src_all = static_cast<const K *>(pp.src);
@@ -1148,7 +1150,7 @@ __global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, c
 			const W pre = static_cast<W>(static_cast<W>(bk) << (8 * P - 16));
 			uint32_t *s_cur = s_u;
 			for (uint32_t i = t; i < NB; i += THREADS) s_cur[i] = 0;
-			if (t == 0) s_big = 0;
+			if (t == 0) *s_big = 0;
 			__syncthreads();
 #pragma unroll
 			for (int j = 0; j < KPT; ++j) {
@@ -1166,19 +1168,17 @@ __global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, c
 			}
 			const uint32_t incl = wave_inclusive_sum(sum);
 			if (lane == 63) s_wsum[w] = incl;
-			if (big) s_big = 1;
+			if (big) *s_big = 1;
 			__syncthreads();
 			uint32_t run = incl - sum;
 			for (uint32_t v = 0; v < w; ++v) run += s_wsum[v];
 #pragma unroll
 			for (uint32_t k = 0; k < PER; ++k) {
-				s_off[t * PER + k] = static_cast<uint16_t>(run);
 				s_cur[t * PER + k] = run;
 				run += c[k];
 			}
-			if (t == THREADS - 1) s_off[NB] = static_cast<uint16_t>(run);
 			__syncthreads();
-			if (s_big) {
+			if (*s_big) {
 				if (t == 0) queue[atomicAdd(&ctrl->hyb_queue, 1u)] = bk;
 			} else {
 #pragma unroll
@@ -1195,19 +1195,25 @@ __global__ __launch_bounds__(THREADS) void k_local(Control *ctrl, KeyXform xf, c
 				// word gives back its raw key: the bucket's top 16 bits, the word's key bits, the
 				// transform undone. The ranks go to LDS as the inverse permutation (position ->
 				// word), and the output is gathered from the words in order.
-				uint16_t *s_inv = reinterpret_cast<uint16_t *>(s_u);  // the cursors are dead
+				uint32_t fin[KPT];  // (position << 16) | word index
 #pragma unroll
 				for (int j = 0; j < KPT; ++j) {
 					const uint32_t i = j * THREADS + t;
+					fin[j] = 0xFFFFFFFFu;
 					if (i < len) {
 						const W x = s_t[i];
 						const uint32_t d = static_cast<uint32_t>(x >> kTop);
-						const uint32_t lo = s_off[d], hi = s_off[d + 1];
+						const uint32_t lo = d ? s_cur[d - 1] : 0u, hi = s_cur[d];
 						uint32_t n_less = 0;  // the word itself is in the run and not smaller
 						for (uint32_t q = lo; q < hi; ++q) n_less += static_cast<uint32_t>(s_t[q] < x);
-						s_inv[lo + n_less] = static_cast<uint16_t>(i);
+						fin[j] = ((lo + n_less) << 16) | i;
 					}
 				}
+				__syncthreads();  // every run bound has been read: s_u takes the inverse permutation
+				uint16_t *s_inv = reinterpret_cast<uint16_t *>(s_u);
+#pragma unroll
+				for (int j = 0; j < KPT; ++j)
+					if (fin[j] != 0xFFFFFFFFu) s_inv[fin[j] >> 16] = static_cast<uint16_t>(fin[j]);
 				__syncthreads();
 				for (uint32_t o = t; o < len; o += THREADS)
 					dst[o] = static_cast<K>(((s_t[s_inv[o]] >> 16) | pre) ^ static_cast<W>(xf.E));
@@ -1641,7 +1647,8 @@ hipError_t launch_local_cfg(Control *ctrl, const KeyXform &xf, const SegTables &
 }
 
 // capacity and partition width per configuration (kLocalCaps): one to three keys per partition
-// run at the mean bucket size each capacity is chosen for
+// run at the mean bucket size each capacity is chosen for (12 bits for 512x9: its LDS, with the run
+// bounds read from the cursors, still lets three blocks share a CU; 0.83 against 0.85 ms at 2^28 u64)
 template <typename K>
 hipError_t launch_local(int lcfg, Control *ctrl, const KeyXform &xf, const SegTables &T, hipStream_t stream)
 {
@@ -1650,7 +1657,7 @@ hipError_t launch_local(int lcfg, Control *ctrl, const KeyXform &xf, const SegTa
 	case 0: return launch_local_cfg<K, 256, 4, 9>(ctrl, xf, T, stream);
 	case 1: return launch_local_cfg<K, 256, 8, 10>(ctrl, xf, T, stream);
 	case 2: return launch_local_cfg<K, kWide ? 512 : 256, kWide ? 6 : 12, 10>(ctrl, xf, T, stream);
-	case 3: return launch_local_cfg<K, 512, 9, 11>(ctrl, xf, T, stream);
+	case 3: return launch_local_cfg<K, 512, 9, 12>(ctrl, xf, T, stream);
 	case 4: return launch_local_cfg<K, 256, 19, 11>(ctrl, xf, T, stream);
 	default: return launch_local_cfg<K, 512, 16, 11>(ctrl, xf, T, stream);
 	}
