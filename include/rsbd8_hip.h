@@ -84,7 +84,7 @@ int rsbd8_hip_radixsortcopynoalloc(int keytype, unsigned flags, size_t count, vo
 /* No counterpart in the reference. For callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc): rsbd8_hip_radixsortcopynoalloc
  * without the workspace argument. Its workspace is a file-scope device array of the library, one
- * per GPU, sized for RSBD8_HIP_STATIC_MAX_COUNT keys (about 26 MB, held by every GPU the library
+ * per GPU, sized for RSBD8_HIP_STATIC_MAX_COUNT keys (33.4 MB, held by every GPU the library
  * runs on until the process ends); a larger count returns RSBD8_HIP_EINVAL. The call allocates
  * nothing (the first call on a GPU looks the array's address up once). input, output and buffer
  * are the caller's device arrays as in rsbd8_hip_radixsortcopynoalloc. Every call on one GPU

@@ -1,27 +1,31 @@
 // rsbd8_sample.hpp -- the sample partition plan of the gfx950 sort (included by rsbd8_sort.hip
 // inside its internal namespace, after the kernels it shares).
 //
-// Two stable partitions and one per-bucket LDS sort, like the two-level MSD plan, but the buckets
-// come from splitters drawn from the keys instead of from the top two bytes of the sort key, so
-// that keys whose bytes cluster (floats, whose exponents crowd into a few top digits; skewed
-// integers) still fall into buckets an LDS block can hold, and so that keys with a payload and
-// modes whose ties must keep their source order take three moves through HBM instead of one per
-// varying byte. The reference has no such plan: its scatter passes (Radixsortbidi8.hpp H:31649-
-// 31982) visit every varying digit; the output is the same stable order, so the plan is exact.
-//   k_samp_l1       one block: samples the input (stratified, seeded positions), sorts the
-//                   sample in LDS and writes the first partition's splitter table;
+// Two partitions and one per-bucket LDS sort, like the two-level MSD plan, but the buckets come
+// from splitters drawn from the keys instead of from the top two bytes of the sort key, so that
+// keys whose bytes cluster (floats, whose exponents crowd into a few top digits; skewed integers)
+// still fall into buckets an LDS block can hold, and so that keys with a payload and modes whose
+// ties must keep their source order take three moves through HBM instead of one per varying byte.
+// The reference has no such plan: its scatter passes (Radixsortbidi8.hpp H:31649-31982) visit
+// every varying digit; the output is the same stable order, so the plan is exact.
+//   k_samp_chunk    one block per input chunk: a stratified sample at seeded positions, sorted
+//                   in LDS, reduced to regular samples;
+//   k_samp_l1       one block: sorts the regular samples and writes the first partition's
+//                   splitter table;
 //   k_split_count1  per-tile counts of the first bucket id and its histogram (the first read);
 //   k_plan_sample   bucket starts, the segment map of the second partition, the pass plans;
 //   k_scan, k_scatter<..., SPL 1>
-//                   the first stable partition (ranked, so ties keep their order);
+//                   the first partition (ranked, so ties keep their order, or grouped when
+//                   equal sort keys are equal keys);
 //   k_samp_l2       one block per first-partition segment: its own splitter table from a sample
 //                   of the segment, with as many buckets as its length needs;
 //   k_split_count2, k_split_segscan, k_scatter<..., SEG, SPL 2>
-//                   the second stable partition inside every segment;
+//                   the second partition inside every segment;
 //   k_bucket_sort   one block per bucket: orders the bucket by (sort key, position), a partition
 //                   on the top bits of (key - min) and exact ranks within its runs; equality
-//                   buckets are already in order; a bucket above the LDS capacity sorts with
-//                   stable LSD passes through HBM in its own block.
+//                   buckets are already in order;
+//   k_bucket_hard   the buckets k_bucket_sort passes on: stable LSD passes in LDS, or through
+//                   HBM above the LDS capacity.
 // Splitter tables: a sorted sample, candidates at the ranks (j + 1) * S / f, j < f - 1, and their
 // distinct values as bounds. A value that takes two or more candidate slots (at least about one
 // bucket's worth of keys) gets an equality bucket of its own, so heavy repeats never overflow a
@@ -31,7 +35,7 @@ constexpr uint32_t kSampleMax = 4096;      // sample keys per splitter table
 constexpr uint32_t kBucketRunMax = 96;     // longest run of one partition value the ranking takes
 // With 8-byte keys, buckets of at most kBucketCapA keys take a 512-thread per-bucket sort (three
 // blocks of eight waves per CU: 2^28 N(0,1) doubles sort 0.8 ms faster than with 256 threads),
-// larger ones the 1024-thread one (8192 keys, one block per CU at that width); modelled at 2^28
+// larger ones the 1024-thread one (8192 keys, two blocks per CU at that width); modelled at 2^28
 // keys with the sampling below, about 80% of the buckets fit the smaller one. 4-byte keys take
 // the 1024-thread one only (two blocks per CU).
 constexpr uint32_t kBucketCapA = 512 * 10;
