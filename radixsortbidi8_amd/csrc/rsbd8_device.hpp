@@ -80,20 +80,27 @@ __device__ __forceinline__ uint32_t digit_of(K raw, const KeyXform &t, uint32_t 
 // c covers the keys lo + (c << sh) ... lo + ((c + 1) << sh) - 1 and holds the number of bounds below
 // its first key (bits 0-7) and how many more lie below its last key (bits 8-15).
 constexpr uint32_t kSplitCells = 4096;
+constexpr uint32_t kSplitSubs = 16;      // sub-tables, for the first cells found with eight bounds or more
+constexpr uint32_t kSplitSubCells = 16;  // sub-cells per sub-table
 template <typename W>
 struct SplitLds {
 	W b[256];
 	uint32_t info[256];
 	uint16_t cell[kSplitCells];
+	uint16_t sub[kSplitSubs * kSplitSubCells];
 	W lo, hi;
 	uint32_t sh;
+	uint32_t nsub;  // sub-tables in use
 };
 
 // Bucket id of the transformed key k: t = the number of bounds below k (0 at or below lo, all of
 // them above hi, else from the key's cell, refined by a binary search over the few bounds that
 // fall inside the cell), then the open bucket below bound t, or bound t's equality bucket when k
-// equals it. Ids grow with the key.
-template <typename W>
+// equals it. Ids grow with the key. A cell holding eight bounds or more (span field 0xFF) points
+// to a sub-table of 16 sub-cells (index in its low byte), which take the search's place; SUB false
+// is for tables with no sub-table (tb.nsub 0: keys spread over their range), whose lookups then
+// take no second cell read (callers branch once on tb.nsub).
+template <typename W, bool SUB = true>
 __device__ __forceinline__ uint32_t split_id(const SplitLds<W> &tb, W k)
 {
 	uint32_t t;
@@ -102,7 +109,10 @@ __device__ __forceinline__ uint32_t split_id(const SplitLds<W> &tb, W k)
 	} else if (k > tb.hi) {
 		t = tb.info[255] >> 16;  // the number of real bounds
 	} else {
-		const uint32_t e = tb.cell[static_cast<uint32_t>((k - tb.lo) >> tb.sh)];
+		const W off = k - tb.lo;
+		uint32_t e = tb.cell[static_cast<uint32_t>(off >> tb.sh)];
+		if (SUB && (e >> 8) == 0xFFu)
+			e = tb.sub[(e & 0xFFu) * kSplitSubCells + (static_cast<uint32_t>(off >> (tb.sh - 4u)) & (kSplitSubCells - 1u))];
 		t = e & 0xFFu;
 		uint32_t span = e >> 8;
 		while (span) {
@@ -434,8 +444,8 @@ __device__ __forceinline__ uint32_t zero_sign_bit(K raw)
 // (C counter copies), reading as for_each_key does; the ids of all the vectors a thread holds are
 // searched before any is counted, so that the searches' LDS loads overlap instead of each waiting
 // behind the previous key's counter update. With ZCHK, returns the zero_sign_bit of the thread's
-// keys ORed together (else 0).
-template <typename K, int THREADS, int C, bool ZCHK = false>
+// keys ORed together (else 0). SUB as for split_id.
+template <typename K, int THREADS, int C, bool ZCHK = false, bool SUB = true>
 __device__ __forceinline__ uint32_t count_split_ids(const K *__restrict__ p, uint64_t lo, uint32_t len, const KeyXform &xf,
 	const SplitLds<typename KeyTraits<K>::W> &tb, uint32_t *s_t)
 {
@@ -448,7 +458,7 @@ __device__ __forceinline__ uint32_t count_split_ids(const K *__restrict__ p, uin
 	uint32_t head = mis ? VEC - mis : 0u;
 	if (head > len) head = len;
 	if (threadIdx.x < head) {
-		wave_count<C>(s_t, split_id<W>(tb, sort_key<K>(q[threadIdx.x], xf)));
+		wave_count<C>(s_t, split_id<W, SUB>(tb, sort_key<K>(q[threadIdx.x], xf)));
 		if constexpr (ZCHK) zf |= zero_sign_bit<K>(q[threadIdx.x]);
 	}
 	const K *qa = q + head;
@@ -479,7 +489,7 @@ __device__ __forceinline__ uint32_t count_split_ids(const K *__restrict__ p, uin
 			K ks[VEC];
 			__builtin_memcpy(ks, &u[jj], 16);
 #pragma unroll
-			for (uint32_t e = 0; e < VEC; ++e) id[jj][e] = split_id<W>(tb, sort_key<K>(ks[e], xf));
+			for (uint32_t e = 0; e < VEC; ++e) id[jj][e] = split_id<W, SUB>(tb, sort_key<K>(ks[e], xf));
 		}
 #pragma unroll
 		for (int jj = 0; jj < INFLIGHT; ++jj) {
@@ -489,7 +499,7 @@ __device__ __forceinline__ uint32_t count_split_ids(const K *__restrict__ p, uin
 		}
 	}
 	for (uint32_t i = nvec * VEC + threadIdx.x; i < body; i += THREADS) {
-		wave_count<C>(s_t, split_id<W>(tb, sort_key<K>(qa[i], xf)));
+		wave_count<C>(s_t, split_id<W, SUB>(tb, sort_key<K>(qa[i], xf)));
 		if constexpr (ZCHK) zf |= zero_sign_bit<K>(qa[i]);
 	}
 	return zf;

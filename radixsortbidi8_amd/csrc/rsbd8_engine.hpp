@@ -75,12 +75,14 @@ constexpr size_t kSegTableWords = 257 + 256 + 256 + 5 * kHybBuckets + 8 * 256 + 
 // transformed key to a bucket id by the number of its bounds below the key (see split_id):
 // bounds as 256 u64 (unused ones all-ones), 256 info words (open bucket id in bits 0-7, bit 8: the
 // bound has an equality bucket of its own, whose id is the open id + 1), the number of real bounds,
-// the number of ids, an 8-word mask of the equality buckets, then the cell table that finds a key's
-// place among the bounds: the lowest and highest bound (u64 each), the cell shift, and
-// kSplitCells u16 cells (see SplitLds).
+// the number of ids, an 8-word mask of the equality buckets, the number of sub-tables in use, then
+// the cell table that finds a key's place among the bounds: the lowest and highest bound (u64
+// each), the cell shift, kSplitCells u16 cells and kSplitSubs sub-tables of kSplitSubCells u16
+// sub-cells (see SplitLds, split_id).
 constexpr uint32_t kSplitBoundsWord = 0, kSplitInfoWord = 512, kSplitCountWord = 768, kSplitIdsWord = 769,
-	kSplitEqWord = 770, kSplitLoWord = 784, kSplitHiWord = 786, kSplitShWord = 788, kSplitCellWord = 792;
-constexpr size_t kSplitTabWords = kSplitCellWord + kSplitCells / 2 + 8;
+	kSplitEqWord = 770, kSplitNsubWord = 778, kSplitLoWord = 784, kSplitHiWord = 786, kSplitShWord = 788,
+	kSplitCellWord = 792, kSplitSubWord = kSplitCellWord + kSplitCells / 2;
+constexpr size_t kSplitTabWords = kSplitSubWord + kSplitSubs * kSplitSubCells / 2 + 8;
 constexpr size_t kSplitTables = 1 + kRadix;
 static_assert(sizeof(Control) <= kCtrlBytes, "control block too large");
 

@@ -78,6 +78,7 @@ struct SplitScratch {
 	uint32_t x[kRadix]; // per bound: open id, equality flag in bit 8
 	uint32_t eq[8];
 	uint32_t wsum[4];
+	uint32_t nsub;      // sub-tables handed out
 };
 
 // Writes the splitter table tab for f buckets from the sorted sample s[0, S) (S >= 1), or, with
@@ -92,6 +93,7 @@ __device__ void build_split_table(const W *s, uint32_t S, uint32_t f, bool all_e
 	const uint32_t nc = (all_equal || f < 2) ? 0u : (f > kRadix ? kRadix - 1 : f - 1);
 	if (t < kRadix) sc.c[t] = t < nc ? s[static_cast<uint32_t>((static_cast<uint64_t>(t + 1) * S) / (nc + 1))] : W(0);
 	if (t < 8) sc.eq[t] = 0;
+	if (t == 0) sc.nsub = 0;
 	__syncthreads();
 	uint32_t first = 0, heavy = 0;
 	if (t < nc) {
@@ -154,18 +156,39 @@ __device__ void build_split_table(const W *s, uint32_t S, uint32_t f, bool all_e
 		}
 		return a;
 	};
+	// cells and sub-cells: (bounds below the cell's first key) | (bounds inside it) << 8. A cell with
+	// eight bounds or more gets a sub-table of 16 sub-cells, while there are any left and the cells
+	// are at least 16 keys wide, and the span field 0xFF with the sub-table's index (a cell that
+	// holds all 255 bounds, the one span that could read as 0xFF, is then the only such cell and
+	// gets the first). The first table of N(0,1) doubles has 10 such cells (8-41 bounds, a binary
+	// search of up to six steps each), that of N(0,1) floats none (at most 7 bounds per cell: the
+	// second read a sub-table costs is worth no more than the three steps it saves), that of keys
+	// spread over their range none.
 	uint16_t *cells = reinterpret_cast<uint16_t *>(tab + kSplitCellWord);
+	uint16_t *subs = reinterpret_cast<uint16_t *>(tab + kSplitSubWord);
+	auto cell_of = [&](uint64_t off, uint64_t wmask) -> uint32_t {
+		if (!m || off > static_cast<uint64_t>(range)) return 0u;
+		const uint64_t end = static_cast<uint64_t>(range) - off < wmask ? static_cast<uint64_t>(range) : off + wmask;
+		const uint32_t t0 = below(lo + static_cast<W>(off)), t1 = below(lo + static_cast<W>(end));
+		return t0 | ((t1 - t0) << 8);
+	};
 	const uint64_t mask = (1ull << sh) - 1ull;
 	for (uint32_t c = t; c < kSplitCells; c += THREADS) {
-		uint32_t e = 0;
 		const uint64_t off = static_cast<uint64_t>(c) << sh;
-		if (m && off <= static_cast<uint64_t>(range)) {
-			const uint64_t end = static_cast<uint64_t>(range) - off < mask ? static_cast<uint64_t>(range) : off + mask;
-			const uint32_t t0 = below(lo + static_cast<W>(off)), t1 = below(lo + static_cast<W>(end));
-			e = t0 | ((t1 - t0) << 8);
+		uint32_t e = cell_of(off, mask);
+		if ((e >> 8) >= 8u && sh >= 4u) {
+			const uint32_t si = atomicAdd(&sc.nsub, 1u);
+			if (si < kSplitSubs) {
+				for (uint32_t q = 0; q < kSplitSubCells; ++q)
+					subs[si * kSplitSubCells + q] =
+						static_cast<uint16_t>(cell_of(off + (static_cast<uint64_t>(q) << (sh - 4u)), (mask >> 4)));
+				e = si | (0xFFu << 8);
+			}
 		}
 		cells[c] = static_cast<uint16_t>(e);
 	}
+	__syncthreads();
+	if (t == 0) tab[kSplitNsubWord] = min(sc.nsub, kSplitSubs);
 }
 
 // Gathers up to SMAX transformed keys of src[lo, lo + len) at seeded stratified positions (all of
@@ -288,7 +311,10 @@ __global__ __launch_bounds__(THREADS) void k_split_count1(const K *__restrict__ 
 		const uint64_t base = static_cast<uint64_t>(tile) * tile_keys;
 		const uint32_t len = (n - base) < tile_keys ? static_cast<uint32_t>(n - base) : tile_keys;
 		const uint64_t lo = reverse ? n - base - len : base;
-		zf |= count_split_ids<K, THREADS, C, ZCHK>(in, lo, len, xf, s_tab, s_t);
+		if (__builtin_amdgcn_readfirstlane(s_tab.nsub))
+			zf |= count_split_ids<K, THREADS, C, ZCHK, true>(in, lo, len, xf, s_tab, s_t);
+		else
+			zf |= count_split_ids<K, THREADS, C, ZCHK, false>(in, lo, len, xf, s_tab, s_t);
 		__syncthreads();
 		const uint32_t v = copies_sum<C>(s_t, t);
 		counts[static_cast<uint64_t>(tile) * kCountStride + t] = v;
@@ -330,7 +356,10 @@ __global__ __launch_bounds__(THREADS) void k_split_count2(const PassPlan *ppp, K
 			tab_seg = sg;
 		}
 		__syncthreads();
-		count_split_ids<K, THREADS, C>(src, base, len, xf, s_tab, s_t);
+		if (__builtin_amdgcn_readfirstlane(s_tab.nsub))
+			count_split_ids<K, THREADS, C, false, true>(src, base, len, xf, s_tab, s_t);
+		else
+			count_split_ids<K, THREADS, C, false, false>(src, base, len, xf, s_tab, s_t);
 		__syncthreads();
 		for (int i = threadIdx.x; i < kRadix; i += THREADS) counts[static_cast<uint64_t>(tile) * kCountStride + i] = copies_sum<C>(s_t, i);
 		__syncthreads();

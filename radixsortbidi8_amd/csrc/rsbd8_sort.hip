@@ -88,13 +88,18 @@ __device__ __forceinline__ void split_load(const uint32_t *__restrict__ tab, Spl
 		tb.b[i] = static_cast<W>(b[i]);
 		tb.info[i] = tab[kSplitInfoWord + i];
 	}
-	const uint32_t *c = tab + kSplitCellWord;
+	const uint32_t *c = tab + kSplitCellWord;  // the cells, then the sub-tables in use
 	uint32_t *dc = reinterpret_cast<uint32_t *>(tb.cell);
+	uint32_t *ds = reinterpret_cast<uint32_t *>(tb.sub);
+	const uint32_t nsub = min(tab[kSplitNsubWord], kSplitSubs);
+	const int nsub_words = static_cast<int>(nsub * (kSplitSubCells / 2));
 	for (int i = threadIdx.x; i < static_cast<int>(kSplitCells / 2); i += THREADS) dc[i] = c[i];
+	for (int i = threadIdx.x; i < nsub_words; i += THREADS) ds[i] = c[kSplitCells / 2 + i];
 	if (threadIdx.x == 0) {
 		tb.lo = static_cast<W>(*reinterpret_cast<const uint64_t *>(tab + kSplitLoWord));
 		tb.hi = static_cast<W>(*reinterpret_cast<const uint64_t *>(tab + kSplitHiWord));
 		tb.sh = tab[kSplitShWord];
+		tb.nsub = nsub;
 	}
 }
 
@@ -988,11 +993,26 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 				tab_seg = sgm;
 			}
 		}
+		if constexpr (SPL) {
+			if (__builtin_amdgcn_readfirstlane(s_tab.nsub)) {
 #pragma unroll
-		for (int j = 0; j < KPT; ++j) {
-			const uint32_t l = w * 64u * KPT + j * 64u + lane;
-			if constexpr (SPL) dig[j] = l < valid ? split_id<W>(s_tab, sort_key<K>(keys[j], xf)) : 0xFFu;
-			else dig[j] = l < valid ? digit_of<K>(keys[j], xf, shift) : 0xFFu;
+				for (int j = 0; j < KPT; ++j) {
+					const uint32_t l = w * 64u * KPT + j * 64u + lane;
+					dig[j] = l < valid ? split_id<W, true>(s_tab, sort_key<K>(keys[j], xf)) : 0xFFu;
+				}
+			} else {
+#pragma unroll
+				for (int j = 0; j < KPT; ++j) {
+					const uint32_t l = w * 64u * KPT + j * 64u + lane;
+					dig[j] = l < valid ? split_id<W, false>(s_tab, sort_key<K>(keys[j], xf)) : 0xFFu;
+				}
+			}
+		} else {
+#pragma unroll
+			for (int j = 0; j < KPT; ++j) {
+				const uint32_t l = w * 64u * KPT + j * 64u + lane;
+				dig[j] = l < valid ? digit_of<K>(keys[j], xf, shift) : 0xFFu;
+			}
 		}
 		if constexpr (GROUP) {
 			static_assert(!PAY, "grouping without order is for keys only");
