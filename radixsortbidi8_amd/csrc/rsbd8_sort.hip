@@ -1892,8 +1892,9 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 	return hipGetLastError();
 }
 
-// The partitions take 1024x8 tiles whatever the configuration: with the splitter table in LDS,
-// 1024x12 and larger tiles spill registers. Against 512x8, the larger tiles halve the tile count
+// The partitions take 1024x8 tiles whatever the configuration (but see below for 8-byte keys that
+// may group): with the splitter table in LDS, the ranked form spills registers in 1024x12 and
+// larger tiles. Against 512x8, the larger tiles halve the tile count
 // rows the counting reads write and the scans read (2^28 f32 with an index payload: 7.13 -> 6.50
 // ms; u64 with a payload: 8.27 -> 7.44 ms; f64 N(0,1) keys, grouped: 7.33 -> 6.98 ms, though its
 // ranked form, 104 KB of LDS per block, goes from 7.33 to 7.50).
@@ -1903,6 +1904,12 @@ hipError_t launch_sample(int cfg, const SortJob &job, void *ws, hipStream_t stre
 	(void)cfg;
 	const KeyXform xf = make_xform(8 * sizeof(K), job.flags);
 	if (job.payload) return launch_sample_cfg<K, true, 1024, 8>(job, ws, xf, stream);
+	if constexpr (sizeof(K) == 8) {
+		// keys-only 8-byte sorts that may group take 1024x12 tiles, whose grouping form fits its
+		// registers (2^28 f64 N(0,1): 6.13 ms against 6.35), though the ranked form, run when +0.0
+		// and -0.0 both occur, then spills (7.17 against 6.91); 4-byte keys gain nothing from them
+		if (sample_group_class(job) != 0) return launch_sample_cfg<K, false, 1024, 12>(job, ws, xf, stream);
+	}
 	return launch_sample_cfg<K, false, 1024, 8>(job, ws, xf, stream);
 }
 
