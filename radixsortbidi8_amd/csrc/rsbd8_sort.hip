@@ -1636,11 +1636,12 @@ hipError_t launch_scatter_cfg(int cfg, Control *ctrl, int pass, const KeyXform &
 }
 
 // Capacities (keys) of the per-bucket LDS sort: 256x4, 256x8, 256x12 (512x6 for 8-byte keys), 512x9,
-// 256x19, 512x16. Eight waves per block where the LDS allows three blocks per CU: the block's
+// 256x19, 512x16, 1024x12 (2^29 keys: u64 7.3 ms, u32 4.2, against 27.1 and 9.2 for the LSD
+// passes). Eight waves per block where the LDS allows three blocks per CU: the block's
 // phases between barriers are short, and more waves per block hide more of their LDS latency
 // (512x9 sorts 2^28 u64 keys' buckets in 0.84 ms against 1.17 ms for 256x19).
-constexpr int kNumLocalCfgs = 6;
-constexpr uint32_t kLocalCaps[kNumLocalCfgs] = {1024, 2048, 3072, 4608, 4864, 8192};
+constexpr int kNumLocalCfgs = 7;
+constexpr uint32_t kLocalCaps[kNumLocalCfgs] = {1024, 2048, 3072, 4608, 4864, 8192, 12288};
 
 // the smallest capacity that holds the largest of 65536 buckets of uniformly spread keys
 // (mean + 6 standard deviations + slack); -1 when none does
@@ -1679,7 +1680,8 @@ hipError_t launch_local(int lcfg, Control *ctrl, const KeyXform &xf, const SegTa
 	case 2: return launch_local_cfg<K, kWide ? 512 : 256, kWide ? 6 : 12, 10>(ctrl, xf, T, stream);
 	case 3: return launch_local_cfg<K, 512, 9, 12>(ctrl, xf, T, stream);
 	case 4: return launch_local_cfg<K, 256, 19, 11>(ctrl, xf, T, stream);
-	default: return launch_local_cfg<K, 512, 16, 11>(ctrl, xf, T, stream);
+	case 5: return launch_local_cfg<K, 512, 16, 11>(ctrl, xf, T, stream);
+	default: return launch_local_cfg<K, 1024, 12, 12>(ctrl, xf, T, stream);
 	}
 }
 
@@ -1879,8 +1881,17 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 		}
 		const uint32_t bgrid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(std::min<uint64_t>(n, kHybBuckets),
 			kTiers ? n / kBucketCapA + 1 : 2 * n / target + 512)));
-		hipLaunchKernelGGL((k_bucket_sort<K, PAY, 1024, 8, 11>), dim3(bgrid), dim3(1024), 0, stream, ctrl, xf, T,
-			T.bq + kHybBuckets, &ctrl->bucket_count2);
+		// above 2^28 keys the second level's 256 buckets per segment average 8192 keys or more: the
+		// larger buckets then take 1024x12 (8-byte) or 1024x16 (4-byte) blocks, one per CU, rather
+		// than LSD passes through HBM (2^29 u64 keys: 26.2 ms with 8192-key blocks)
+		if (n > (1ull << 28)) {
+			constexpr int KB = sizeof(K) == 8 ? 12 : 16;
+			hipLaunchKernelGGL((k_bucket_sort<K, PAY, 1024, KB, 12>), dim3(bgrid), dim3(1024), 0, stream, ctrl, xf, T,
+				T.bq + kHybBuckets, &ctrl->bucket_count2);
+		} else {
+			hipLaunchKernelGGL((k_bucket_sort<K, PAY, 1024, 8, 11>), dim3(bgrid), dim3(1024), 0, stream, ctrl, xf, T,
+				T.bq + kHybBuckets, &ctrl->bucket_count2);
+		}
 		if ((e = hipGetLastError()) != hipSuccess) return e;
 		hipLaunchKernelGGL((k_bucket_hard<K, PAY, 1024, 8>), dim3(2 * cus), dim3(1024), 0, stream, ctrl, xf, T);
 		if ((e = hipGetLastError()) != hipSuccess) return e;

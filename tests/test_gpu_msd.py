@@ -160,3 +160,27 @@ def test_msd_default_plan_at_2p26():
     oracle.radixsort_mt(bits, 0, 8, exp)
     assert used
     assert np.array_equal(got, exp)
+
+
+def test_msd_2p29_takes_the_12288_key_tier():
+    """2^29 uniform u64 and u32 keys: 8192 keys per bucket on average, so the exact plan needs its
+    12288-key tier (1024x12 blocks); checked against torch.sort of the same keys."""
+    gen = torch.Generator(device="cuda").manual_seed(529)
+    n = 1 << 29
+    for dt, it in ((torch.uint64, torch.int64), (torch.uint32, torch.int32)):
+        lo, hi = (-2**63, 2**63 - 1) if it == torch.int64 else (-2**31, 2**31 - 1)
+        k = torch.randint(lo, hi, (n,), dtype=it, device="cuda", generator=gen).view(dt)
+        kt = r.keytype_of(k.dtype)
+        ws = r.Workspace(kt, n)
+        o, b = torch.empty_like(k), torch.empty_like(k)
+        r.profile_begin()
+        r.radixsortcopynoalloc(k, o, b, ws, 0, 64)
+        prof = r.profile_end()
+        assert prof["msd_local"][1] > 0
+        sk = k.view(it).to(torch.int64)
+        sk = sk ^ (-(1 << 63)) if it == torch.int64 else sk & 0xFFFFFFFF
+        ref = torch.sort(sk).values
+        got = o.view(it).to(torch.int64)
+        got = got ^ (-(1 << 63)) if it == torch.int64 else got & 0xFFFFFFFF
+        assert torch.equal(got, ref)
+        del k, o, b, ws, sk, ref, got

@@ -221,6 +221,28 @@ def test_sample_grouping_partitions_tiered_abs(nb, kt, msd):
         assert np.array_equal(host(a).view(bits.dtype), exp), d
 
 
+def test_sample_2p29_pairs_take_the_large_bucket_blocks():
+    """2^29 u64 keys with their source index: the second level's buckets average 8192 keys, so the
+    larger ones take the 1024x12 per-bucket blocks; keys and payload checked against a stable
+    torch.sort (the payload is the stable permutation)."""
+    gen = torch.Generator(device="cuda").manual_seed(729)
+    n = 1 << 29
+    k = torch.randint(-2**63, 2**63 - 1, (n,), dtype=torch.int64, device="cuda", generator=gen)
+    k[: n // 64] = k[n // 2: n // 2 + n // 64]  # duplicates: stability is observable
+    ku = k.view(torch.uint64)
+    ws = r.Workspace(r.U64, n, r.LAYOUT_PAIRS)
+    ko, kb = torch.empty_like(ku), torch.empty_like(ku)
+    po = torch.empty(n, dtype=torch.uint32, device="cuda")
+    pb = torch.empty_like(po)
+    r.profile_begin()
+    r.sort_pairs(ku, ko, kb, None, po, pb, ws, 0, 64)
+    prof = r.profile_end()
+    assert prof["msd_local"][1] > 0
+    ref_v, ref_i = torch.sort(k ^ (-(1 << 63)), stable=True)
+    assert torch.equal(ko.view(torch.int64) ^ (-(1 << 63)), ref_v)
+    assert torch.equal(po.view(torch.int32).to(torch.int64) & 0xFFFFFFFF, ref_i)
+
+
 def test_sample_target_rejects_negative():
     with pytest.raises(r.Rsbd8Error):
         r.set_sample_target(-1)
