@@ -198,7 +198,8 @@ int rsbd8_hip_set_sample_target(int keys);
  * compile-time thresholds, H:12948-13152): algo 0 automatic, 1 onesweep, 2 count + scan + scatter,
  * 3 a single block sorting everything in LDS (keys only, count <= 16384; otherwise automatic),
  * 4 the two-level MSD plan (as for rsbd8_hip_set_algorithm); tile_cfg -1 automatic or 0..4 as for rsbd8_hip_set_tile_config; presort_check first tests
- * whether the input is already in order or strictly reversed (then the sort is one copy). */
+ * whether the input is already in order or strictly reversed (then the sort is one copy), or in
+ * reverse order with ties (then one pass moves the runs of equal keys, each in its own order). */
 typedef struct rsbd8_hip_plan {
 	int algo;
 	int tile_cfg;

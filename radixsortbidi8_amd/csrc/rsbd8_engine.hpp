@@ -55,7 +55,12 @@ struct Control {
 	uint32_t bucket_count2;// ... and for the larger one
 	uint32_t zero_signs;   // sample partition plan's first read: bit 0 a +0.0 key seen, bit 1 a -0.0 key
 	uint32_t group_parts;  // sample partition plan: 1 when its partitions group (unstable), 0 when they rank
+	// presort check: some wave saw an ascent and a descent (the others may stop): word 0 of 16 cache
+	// lines, block b setting line b % 16 (stores to one word from every wave of the grid queue on one
+	// L2 channel)
+	alignas(128) uint32_t check_stop[16][32];
 };
+constexpr uint32_t kStopLines = 16;
 
 constexpr size_t kCtrlBytes = 4096;
 constexpr size_t kHistBytes = kMaxPasses * kRadix * sizeof(uint32_t);

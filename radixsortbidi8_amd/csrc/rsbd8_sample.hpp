@@ -465,6 +465,7 @@ __global__ __launch_bounds__(256) void k_plan_sample(PlanArgs a, Control *ctrl, 
 	ctrl->num_active = 0;
 	ctrl->group_parts = (a.group == 1 || (a.group == 2 && ctrl->zero_signs != 3u)) ? 1u : 0u;
 	const uint32_t pres = presort_result(ctrl->check, a.reverse);
+	if (pres == 3u) return;  // the run reversal (k_runs_copy) writes the result
 	if (pres) {
 		PassPlan &f = ctrl->fin;
 		f.src = a.src; f.dst = a.fin; f.psrc = a.psrc; f.pdst = a.pfin;

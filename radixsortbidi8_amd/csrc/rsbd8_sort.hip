@@ -114,12 +114,16 @@ __host__ __device__ inline SegTables seg_tables(uint32_t *base)
 
 // 0: a real sort is needed; 1: the output is the input as it is; 2: the output is the input
 // reversed (physical order). For isrevorder the logical input is the physical one read backwards.
+// The presort check's verdict (check: bit 0 ran, bits 1-3 an ascent, a tie, a descent seen between
+// physical neighbours, bit 4 the sort may take the run reversal): 1 the input is in order (a copy),
+// 2 it is strictly in reverse order (a reversed copy), 3 it is in reverse order with ties (the runs
+// of equal keys reversed as blocks, each in its own order: k_runs_*), 0 it needs sorting.
 __device__ __forceinline__ uint32_t presort_result(uint32_t check, int reverse)
 {
 	if (!(check & 1u)) return 0;
-	const bool lt = check & 2u, eq = check & 4u, gt = check & 8u;
-	if (!reverse) return !gt ? 1u : ((!lt && !eq) ? 2u : 0u);
-	return !lt ? 2u : ((!gt && !eq) ? 1u : 0u);
+	const bool lt = check & 2u, eq = check & 4u, gt = check & 8u, runs = check & 16u;
+	if (!reverse) return !gt ? 1u : (!lt ? (!eq ? 2u : (runs ? 3u : 0u)) : 0u);
+	return !lt ? 2u : (!gt ? (!eq ? 1u : (runs ? 3u : 0u)) : 0u);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -155,30 +159,407 @@ __global__ __launch_bounds__(THREADS) void k_hist(const K *__restrict__ in, uint
 
 
 // ---------------------------------------------------------------------------------------------
-// presort check: is the input already in order, or strictly in reverse order?
+// presort check: is the input already in order, or in reverse order?
+
+// The sort key as an ordered value: sort_key leaves bits above a 1- or 2-byte key's width undefined
+// (the tiered modes shift the sign bit up there), which the digit passes never read but an order
+// comparison would.
+template <typename K>
+__device__ __forceinline__ typename KeyTraits<K>::W order_key(K raw, const KeyXform &xf)
+{
+	using W = typename KeyTraits<K>::W;
+	constexpr W mask = sizeof(K) >= sizeof(W) ? ~W(0) : static_cast<W>((W(1) << (8 * sizeof(K))) - 1u);
+	return sort_key<K>(raw, xf) & mask;
+}
+
+// The run reversal (presort verdict 3: the input, read in its logical order, never ascends but has
+// ties). The stable order is then the runs of equal keys in reverse order, each run in its own
+// order: a key at physical position j of the run [a, b] goes to n - 1 - b + (j - a), or, when the
+// input is read back to front, to a + (b - j). Tiles of kRunTile keys: the check records every
+// wave's last run start and first run end (a wave of the check reads a contiguous quarter of a
+// tile), k_runs_reduce and k_runs_carry carry them across tiles (the start of the run that enters
+// a tile, the end of the run that leaves it), k_runs_copy places every key. The words live in the
+// workspace's count rows, planar (word k of tile c at k * tiles + c, see run_word): eight wave
+// words (start, end), the two carries, then two words per block of the carry kernels.
+constexpr uint32_t kRunTile = 4096;
+constexpr uint32_t kRunWaves = 4;          // waves of the check's blocks
+constexpr uint32_t kRunWaveKeys = kRunTile / kRunWaves;
+constexpr uint32_t kRunCarry = 2 * kRunWaves;
+constexpr uint32_t kRunAgg = kRunCarry + 2;
+constexpr uint32_t kRunBlock = 1024;       // tiles per block of the carry kernels
+
+constexpr uint32_t kCheckSlots = 4096;   // the check's per-wave verdict words come first (at most 1024 blocks)
+
+__device__ __forceinline__ uint32_t *count_word(uint32_t *counts, uint32_t v)
+{
+	return counts + static_cast<uint64_t>(v >> 8) * kCountStride + (v & 255u);
+}
+__device__ __forceinline__ uint32_t *run_word(uint32_t *counts, uint32_t v) { return count_word(counts, kCheckSlots + v); }
+
+template <typename K>
+__device__ __forceinline__ K lane0_of(K v)
+{
+	if constexpr (sizeof(K) == 8) {
+		const uint64_t u = static_cast<uint64_t>(v);
+		const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u));
+		const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(u >> 32));
+		return static_cast<K>((static_cast<uint64_t>(hi) << 32) | lo);
+	} else {
+		return static_cast<K>(__builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v)));
+	}
+}
+
+// Tiles of kRunTile keys, blocks taking tiles in grid stride. Wave w reads the contiguous quarter w
+// of a tile, key i of lane l at offset i * 64 + l (coalesced): the key after it is lane l + 1's, or
+// lane 0's key i + 1, and for the wave's last key the first key after the quarter, loaded with the
+// others (as are the key before the quarter and the check word, so every iteration waits for memory
+// once). A wave stops once an ascent and a descent were seen anywhere (then no shortcut is possible,
+// and no run word is needed), and publishes only what it had not published yet. With runs set,
+// every wave records the last run start and the first run end of its quarter (a run starts right
+// after an end), as bit masks over its keys: bounds kept in the loop by conditional updates lost the
+// first iteration's update in the gfx950 build, so the run kernels use masks and selects only.
+template <typename K, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_check(const K *__restrict__ in, uint64_t n, KeyXform xf, Control *ctrl,
+	uint32_t *__restrict__ words, int runs, uint32_t tiles)
+{
+	constexpr uint32_t PER = kRunWaveKeys / 64;
+	static_assert(THREADS / 64 == kRunWaves && PER * 64 == kRunWaveKeys, "a wave reads a quarter tile");
+	if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&ctrl->check, runs ? 17u : 1u);
+	if (n < 2) return;
+	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+	const uint64_t pairs = n - 1;
+	const uint32_t last = static_cast<uint32_t>(n - 1);
+	uint32_t *slot = count_word(words, blockIdx.x * kRunWaves + w);
+	uint32_t seen = 0;  // what this wave saw (wave-uniform), stored to its slot when it ends
+	uint32_t it = 0;
+	for (uint64_t tb = static_cast<uint64_t>(blockIdx.x) * kRunTile; tb < n; tb += static_cast<uint64_t>(gridDim.x) * kRunTile, ++it) {
+		const uint64_t base = tb + w * kRunWaveKeys;
+		// another wave's stop, every 16 tiles (loads this CU's cache may serve a little late)
+		uint32_t stop = 0;
+		if ((it & 15u) == 0) {
+			const uint32_t v = lane < kStopLines
+				? __hip_atomic_load(&ctrl->check_stop[lane][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0u;
+			stop = __ballot(v != 0) ? 1u : 0u;
+		}
+		K k[PER];
+#pragma unroll
+		for (uint32_t i = 0; i < PER; ++i) {
+			const uint64_t j = base + i * 64 + lane;
+			k[i] = j < n ? in[j] : K(0);
+		}
+		const uint64_t qend = min(n, base + kRunWaveKeys);
+		K tail = K(0), head = K(0);  // the key after the quarter (lane 63), the key before it (lane 0)
+		if (lane == 63 && qend < n) tail = in[qend];
+		if (lane == 0 && base > 0 && base < n) head = in[base - 1];
+		uint32_t mine = 0, endm = 0, stm = 0;
+#pragma unroll
+		for (uint32_t i = 0; i < PER; ++i) {
+			const uint64_t j = base + i * 64 + lane;
+			// lane 0's key i + 1, read by the whole wave (the first active lane must be lane 0)
+			const K down = __shfl_down(k[i], 1, 64), first = lane0_of<K>(k[i + 1 < PER ? i + 1 : i]);
+			const K nx = lane < 63 ? down : (i + 1 < PER ? first : tail);
+			const auto a = order_key<K>(k[i], xf), b = order_key<K>(nx, xf);
+			const uint32_t r = (a < b ? 2u : 0u) | (a == b ? 4u : 0u) | (a > b ? 8u : 0u);
+			mine |= j < pairs ? r : 0u;
+			const bool e = (j < n) & ((j + 1 == n) | (a != b));
+			endm |= static_cast<uint32_t>(e) << i;
+			stm |= static_cast<uint32_t>(e & (j + 1 < qend)) << i;
+		}
+		seen |= (__ballot((mine & 2u) != 0) ? 2u : 0u) | (__ballot((mine & 4u) != 0) ? 4u : 0u) |
+			(__ballot((mine & 8u) != 0) ? 8u : 0u);
+		if (runs) {
+			uint32_t en = endm ? static_cast<uint32_t>(base + (__ffs(endm) - 1u) * 64 + lane) : last;
+			uint32_t st = stm ? static_cast<uint32_t>(base + (31u - __clz(stm)) * 64 + lane + 1u) : 0u;
+			if (lane == 0 && base > 0 && base < n) {  // the quarter's first key starts a run when it differs from the one before
+				const bool s0 = order_key<K>(head, xf) != order_key<K>(k[0], xf);
+				st = s0 ? max(st, static_cast<uint32_t>(base)) : st;
+			}
+#pragma unroll
+			for (int o = 32; o > 0; o >>= 1) {
+				st = max(st, static_cast<uint32_t>(__shfl_xor(st, o, 64)));
+				en = min(en, static_cast<uint32_t>(__shfl_xor(en, o, 64)));
+			}
+			const uint32_t tile = static_cast<uint32_t>(tb / kRunTile);
+			if (lane == 0) {
+				*run_word(words, (2 * w) * tiles + tile) = st;
+				*run_word(words, (2 * w + 1) * tiles + tile) = en;
+			}
+		}
+		if ((seen & 10u) == 10u) {  // no shortcut is possible
+			if (lane == 0 && !stop)
+				__hip_atomic_store(&ctrl->check_stop[blockIdx.x % kStopLines][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			break;
+		}
+		if (stop) break;
+	}
+	if (lane == 0) *slot = seen;
+}
+
+// The verdict: every wave's slot of the check, or'd into the control block (one atomic).
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_check_fold(Control *ctrl, uint32_t *__restrict__ words, uint32_t slots)
+{
+	__shared__ uint32_t s_w[THREADS / 64];
+	uint32_t v = 0;
+	for (uint32_t q = threadIdx.x; q < slots; q += THREADS) v |= *count_word(words, q);
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v |= static_cast<uint32_t>(__shfl_xor(v, o, 64));
+	if ((threadIdx.x & 63u) == 0) s_w[threadIdx.x >> 6] = v;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int q = 1; q < THREADS / 64; ++q) v |= s_w[q];
+		if (v) atomicOr(&ctrl->check, v);
+	}
+}
+
+// Block-wide maximum / minimum over the 1024 tiles of a carry block, with every thread's value
+// combined with the threads before it (INCL), or the reduction only.
+template <int THREADS, bool MAX>
+__device__ __forceinline__ uint32_t block_scan_mm(uint32_t v, uint32_t *s_w, uint32_t &total)
+{
+	constexpr int NW = THREADS / 64;
+	const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+	uint32_t incl = v;
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+		const uint32_t u = __shfl_up(incl, o, 64);
+		incl = lane >= static_cast<uint32_t>(o) ? (MAX ? max(incl, u) : min(incl, u)) : incl;
+	}
+	if (lane == 63) s_w[w] = incl;
+	__syncthreads();
+	uint32_t before = MAX ? 0u : ~0u, all = before;
+	for (int q = 0; q < NW; ++q) {
+		const uint32_t x = s_w[q];
+		before = q < static_cast<int>(w) ? (MAX ? max(before, x) : min(before, x)) : before;
+		all = MAX ? max(all, x) : min(all, x);
+	}
+	__syncthreads();
+	total = all;
+	return MAX ? max(before, incl) : min(before, incl);
+}
+
+// Every carry block's largest run start and smallest run end (its tiles' wave words combined).
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_runs_reduce(const Control *ctrl, int reverse, uint64_t n, uint32_t *__restrict__ counts,
+	uint32_t tiles)
+{
+	if (presort_result(ctrl->check, reverse) != 3u) return;
+	__shared__ uint32_t s_w[THREADS / 64];
+	const uint32_t c = blockIdx.x * THREADS + threadIdx.x;
+	const uint32_t last = static_cast<uint32_t>(n - 1);
+	uint32_t st = 0, en = last;
+	if (c < tiles) {
+#pragma unroll
+		for (uint32_t q = 0; q < kRunWaves; ++q) {
+			st = max(st, *run_word(counts, (2 * q) * tiles + c));
+			en = min(en, *run_word(counts, (2 * q + 1) * tiles + c));
+		}
+	}
+	uint32_t smax, emin;
+	block_scan_mm<THREADS, true>(st, s_w, smax);
+	block_scan_mm<THREADS, false>(en, s_w, emin);
+	if (threadIdx.x == 0) {
+		*run_word(counts, kRunAgg * tiles + blockIdx.x) = smax;
+		*run_word(counts, kRunAgg * tiles + gridDim.x + blockIdx.x) = emin;
+	}
+}
+
+// carry_start of tile c (word kRunCarry) = the largest run start of the tiles before c (0: position
+// 0 starts a run), carry_end (word kRunCarry + 1) = the smallest run end of the tiles after c (n - 1
+// ends one): the blocks before and after from the block words, then a block-wide scan each way.
+template <int THREADS>
+__global__ __launch_bounds__(THREADS) void k_runs_carry(const Control *ctrl, int reverse, uint64_t n, uint32_t *__restrict__ counts,
+	uint32_t tiles)
+{
+	if (presort_result(ctrl->check, reverse) != 3u) return;
+	__shared__ uint32_t s_w[THREADS / 64];
+	const uint32_t t = threadIdx.x, b = blockIdx.x, nb = gridDim.x;
+	const uint32_t c = b * THREADS + t;
+	const uint32_t last = static_cast<uint32_t>(n - 1);
+	// the other blocks' words (nb <= THREADS * 4: up to four per thread)
+	uint32_t pmax = 0, smin = last;
+	for (uint32_t q = t; q < nb; q += THREADS) {
+		const uint32_t sa = *run_word(counts, kRunAgg * tiles + q), ea = *run_word(counts, kRunAgg * tiles + nb + q);
+		pmax = q < b ? max(pmax, sa) : pmax;
+		smin = q > b ? min(smin, ea) : smin;
+	}
+	uint32_t bmax, bmin;
+	block_scan_mm<THREADS, true>(pmax, s_w, bmax);
+	block_scan_mm<THREADS, false>(smin, s_w, bmin);
+	uint32_t st = 0, en = last;
+	if (c < tiles) {
+#pragma unroll
+		for (uint32_t q = 0; q < kRunWaves; ++q) {
+			st = max(st, *run_word(counts, (2 * q) * tiles + c));
+			en = min(en, *run_word(counts, (2 * q + 1) * tiles + c));
+		}
+	}
+	uint32_t unused;
+	const uint32_t incl = block_scan_mm<THREADS, true>(st, s_w, unused);
+	const uint32_t lane = t & 63u;
+	// exclusive: the inclusive value of the thread before (a shuffle, or the wave before's total)
+	__shared__ uint32_t s_last[THREADS / 64];
+	if (lane == 63) s_last[t >> 6] = incl;
+	__syncthreads();
+	const uint32_t upv = __shfl_up(incl, 1, 64);
+	const uint32_t prev = lane ? upv : ((t >> 6) ? s_last[(t >> 6) - 1] : 0u);
+	// the suffix minimum: the same scan over the block's tiles in reverse order
+	const uint32_t rc = b * THREADS + (THREADS - 1 - t);
+	uint32_t ren = last;
+	if (rc < tiles) {
+#pragma unroll
+		for (uint32_t q = 0; q < kRunWaves; ++q) ren = min(ren, *run_word(counts, (2 * q + 1) * tiles + rc));
+	}
+	const uint32_t rincl = block_scan_mm<THREADS, false>(ren, s_w, unused);
+	__shared__ uint32_t s_rlast[THREADS / 64];
+	if (lane == 63) s_rlast[t >> 6] = rincl;
+	__syncthreads();
+	const uint32_t rupv = __shfl_up(rincl, 1, 64);
+	const uint32_t rprev = lane ? rupv : ((t >> 6) ? s_rlast[(t >> 6) - 1] : last);
+	if (c < tiles) *run_word(counts, kRunCarry * tiles + c) = max(bmax, prev);
+	if (rc < tiles) *run_word(counts, (kRunCarry + 1) * tiles + rc) = min(bmin, rprev);
+}
+
+__device__ __forceinline__ uint32_t pad16(uint32_t x) { return x + (x >> 4); }
 
 template <typename K, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_check(const K *__restrict__ in, uint64_t n, KeyXform xf, Control *ctrl)
+__global__ __launch_bounds__(THREADS) void k_runs_copy(const K *__restrict__ in, K *__restrict__ out, uint64_t n, KeyXform xf,
+	const Control *ctrl, int reverse, uint32_t *__restrict__ counts, uint32_t tiles)
 {
-	if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&ctrl->check, 1u);
-	if (n < 2) return;
-	const uint64_t pairs = n - 1;
-	const uint64_t chunk = (pairs + gridDim.x - 1) / gridDim.x;
-	const uint64_t lo = static_cast<uint64_t>(blockIdx.x) * chunk;
-	const uint64_t hi = min(pairs, lo + chunk);
-	uint32_t mine = 0;
-	uint64_t it = 0;
-	for (uint64_t j = lo + threadIdx.x; j < hi; j += THREADS, ++it) {
-		if ((it & 63u) == 0) {
-			// stop once both an ascent and a descent were seen anywhere: no shortcut is possible
-			const uint32_t seen = __hip_atomic_load(&ctrl->check, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) | mine;
-			if ((seen & 10u) == 10u) break;
+	if (presort_result(ctrl->check, reverse) != 3u) return;
+	constexpr uint32_t PER = kRunTile / THREADS;
+	constexpr int NW = THREADS / 64;
+	// the tile, with the key before it and the key after it; then the output order. Index x lives at
+	// x + x / 16 (pad): a thread's 16 consecutive keys and the threads' 16-key strides fall in
+	// different banks.
+	__shared__ K s_k[kRunTile + 2 + (kRunTile + 2) / 16];
+	__shared__ uint32_t s_wmax[NW], s_wmin[NW];
+	__shared__ uint32_t s_run[4];   // the runs of the tile's first and last key: a0, b0, a1, b1
+	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+	const uint32_t last = static_cast<uint32_t>(n - 1);
+	const uint32_t tile = blockIdx.x;
+	const uint64_t base = static_cast<uint64_t>(tile) * kRunTile;
+	const uint32_t tcount = static_cast<uint32_t>(min<uint64_t>(n - base, kRunTile));  // keys in the tile
+	{
+		// coalesced, every load in flight at once: s_k[i] = in[base - 1 + i]
+		constexpr uint32_t LOADS = (kRunTile + 2 + THREADS - 1) / THREADS;
+		K v[LOADS];
+#pragma unroll
+		for (uint32_t q = 0; q < LOADS; ++q) {
+			const uint32_t i = q * THREADS + t;
+			const uint64_t j = base + i;  // physical position + 1
+			v[q] = (i < kRunTile + 2 && j >= 1 && j - 1 < n) ? in[j - 1] : K(0);
 		}
-		const auto a = sort_key<K>(in[j], xf), b = sort_key<K>(in[j + 1], xf);
-		mine |= a < b ? 2u : (a == b ? 4u : 8u);
-		if ((it & 63u) == 63u && mine) { atomicOr(&ctrl->check, mine); }
+#pragma unroll
+		for (uint32_t q = 0; q < LOADS; ++q) {
+			const uint32_t i = q * THREADS + t;
+			if (i < kRunTile + 2) s_k[pad16(i)] = v[q];
+		}
 	}
-	if (mine) atomicOr(&ctrl->check, mine);
+	const uint32_t carry_st = *run_word(counts, kRunCarry * tiles + tile);
+	const uint32_t carry_end = *run_word(counts, (kRunCarry + 1) * tiles + tile);
+	__syncthreads();
+	const uint64_t j0 = base + t * PER;
+	// bit i: position j0 + i starts / ends a run, from the sort keys of positions j0 - 1 .. j0 + PER
+	// (s_k[t * PER + i] holds position j0 - 1 + i); position 0 starts a run, n - 1 ends one, and
+	// positions from n on do neither
+	using W = typename KeyTraits<K>::W;
+	W sk[PER + 2];
+#pragma unroll
+	for (uint32_t i = 0; i < PER + 2; ++i) sk[i] = order_key<K>(s_k[pad16(t * PER + i)], xf);
+	uint32_t stf = 0, enf = 0;
+#pragma unroll
+	for (uint32_t i = 0; i < PER; ++i) {
+		stf |= static_cast<uint32_t>(sk[i] != sk[i + 1]) << i;
+		enf |= static_cast<uint32_t>(sk[i + 1] != sk[i + 2]) << i;
+	}
+	stf |= j0 == 0 ? 1u : 0u;
+	const uint32_t valid = j0 >= n ? 0u : static_cast<uint32_t>(min<uint64_t>(n - j0, PER));  // keys below n
+	const uint32_t vmask = valid >= 32u ? ~0u : (1u << valid) - 1u;
+	stf &= vmask;
+	enf = (enf | (j0 + valid == n && valid ? 1u << (valid - 1u) : 0u)) & vmask;
+	const uint32_t mx = stf ? static_cast<uint32_t>(j0) + 31u - __clz(stf) : 0u;
+	const uint32_t mn = enf ? static_cast<uint32_t>(j0) + __ffs(enf) - 1u : last;
+	// the largest run start before this thread's keys, the smallest run end after them
+	uint32_t imx = mx, imn = mn;
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1) {
+		const uint32_t a = __shfl_up(imx, o, 64), b = __shfl_down(imn, o, 64);
+		imx = lane >= static_cast<uint32_t>(o) ? max(imx, a) : imx;
+		imn = lane + o < 64u ? min(imn, b) : imn;
+	}
+	if (lane == 63) s_wmax[w] = imx;
+	if (lane == 0) s_wmin[w] = imn;
+	K kv[PER];
+#pragma unroll
+	for (uint32_t i = 0; i < PER; ++i) kv[i] = s_k[pad16(t * PER + i + 1)];
+	__syncthreads();
+	uint32_t before = carry_st, after = carry_end;
+	for (int v = 0; v < NW; ++v) {
+		before = v < static_cast<int>(w) ? max(before, s_wmax[v]) : before;
+		after = v > static_cast<int>(w) ? min(after, s_wmin[v]) : after;
+	}
+	const uint32_t up = __shfl_up(imx, 1, 64), down = __shfl_down(imn, 1, 64);
+	before = lane > 0 ? max(before, up) : before;
+	after = lane < 63 ? min(after, down) : after;
+	uint32_t endpos[PER];  // the ends of this thread's keys' runs, from the last key back
+	{
+		uint32_t e = after;
+#pragma unroll
+		for (int i = static_cast<int>(PER) - 1; i >= 0; --i) {
+			e = (enf >> i) & 1u ? static_cast<uint32_t>(j0 + i) : e;
+			endpos[i] = e;
+		}
+	}
+	uint32_t startpos[PER];
+	{
+		uint32_t a = before;
+#pragma unroll
+		for (uint32_t i = 0; i < PER; ++i) {
+			a = (stf >> i) & 1u ? static_cast<uint32_t>(j0 + i) : a;
+			startpos[i] = a;
+		}
+	}
+	const uint32_t lk = tcount - 1;  // the tile's last key (local index)
+	if (t == 0) { s_run[0] = startpos[0]; s_run[1] = endpos[0]; }
+	{
+		uint32_t sa = 0, sb = 0;
+#pragma unroll
+		for (uint32_t i = 0; i < PER; ++i) {
+			sa = i == lk % PER ? startpos[i] : sa;
+			sb = i == lk % PER ? endpos[i] : sb;
+		}
+		if (t == lk / PER) { s_run[2] = sa; s_run[3] = sb; }
+	}
+	__syncthreads();
+	// the three output ranges in output order (sizes may be 0): slot p of s_k holds output position
+	// orig[r] + (p - off[r]) for range r with off[r] <= p < off[r + 1]
+	const uint32_t a0 = s_run[0], b0 = s_run[1], a1 = s_run[2], b1 = s_run[3];
+	const uint32_t tb = static_cast<uint32_t>(base), te = tb + lk;
+	const bool multi = a1 > b0;  // the first and the last key lie in different runs
+	const uint32_t lenE = min(b0, te) - tb + 1, lenL = multi ? te - a1 + 1 : 0u, lenI = multi ? a1 - 1 - b0 : 0u;
+	uint32_t off1, off2, orig0, orig1, orig2;
+	if (!reverse) {  // the last key's run first, then the inside, then the first key's run
+		off1 = lenL; off2 = lenL + lenI;
+		orig0 = last - b1; orig1 = last - a1 + 1; orig2 = last - b0 + (tb - a0);
+	} else {         // runs keep their places: the first key's run, the inside, the last key's run
+		off1 = lenE; off2 = lenE + lenI;
+		orig0 = a0 + b0 - min(b0, te); orig1 = b0 + 1; orig2 = a1 + b1 - te;
+	}
+#pragma unroll
+	for (uint32_t i = 0; i < PER; ++i) {
+		const uint32_t jj = static_cast<uint32_t>(j0 + i), a = startpos[i], b = endpos[i];
+		const uint32_t o = reverse ? a + (b - jj) : last - b + (jj - a);
+		const bool inE = jj <= b0, inL = multi && jj >= a1;
+		const uint32_t slot = inE ? (reverse ? o - orig0 : off2 + (o - orig2))
+			: (inL ? (reverse ? off2 + (o - orig2) : o - orig0) : off1 + (o - orig1));
+		if (i < valid && slot < kRunTile) s_k[pad16(slot)] = kv[i];
+	}
+	__syncthreads();
+	for (uint32_t p = t; p < tcount; p += THREADS) {
+		const uint32_t o = p < off1 ? orig0 + p : (p < off2 ? orig1 + (p - off1) : orig2 + (p - off2));
+		if (o < n) out[o] = s_k[pad16(p)];  // (always: the ranges partition the output)
+	}
 }
 
 // One block sorts the whole array in LDS (8-bit LSD passes with the block ranking). With PAY a u32
@@ -365,6 +746,7 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 	ctrl->fin = none;
 	for (int p = 0; p < kMaxPasses; ++p) ctrl->pass[p] = none;
 	if (A == 0) {
+		if (pres == 3u) return;  // the run reversal (k_runs_copy) writes the result
 		PassPlan &f = ctrl->fin;
 		f.src = a.src; f.dst = a.fin; f.psrc = a.psrc; f.pdst = a.pfin;
 		f.iota = a.payload && !a.psrc;
@@ -1453,6 +1835,7 @@ __device__ __forceinline__ void reset_control(Control *ctrl)
 		ctrl->tile_counter[threadIdx.x] = 0;
 		ctrl->scan_counter[threadIdx.x] = 0;
 	}
+	if (threadIdx.x < kStopLines) ctrl->check_stop[threadIdx.x][0] = 0;
 	if (threadIdx.x == 0) {
 		ctrl->epoch = ctrl->epoch + 1u;
 		ctrl->check = 0;
@@ -1736,6 +2119,45 @@ constexpr uint32_t kSampleTarget = 4096;
 std::atomic<uint32_t> g_sample_target{0};
 
 // The sample partition plan (rsbd8_sample.hpp) with THREADS x KPT partition tiles.
+// whether a presort check may end in the run reversal: keys only (the plan entry point's sorts), and
+// it writes out from in by positions, so not in place
+inline bool runs_allowed(const SortJob &job)
+{
+	return job.in != job.out && !job.keys_final_any && !job.payload;
+}
+
+// the presort check and, when it finds the input in reverse order with ties, the run reversal (every
+// later launch of the sort then exits at once)
+template <typename K>
+hipError_t launch_check(const SortJob &job, void *ws, const KeyXform &xf, hipStream_t stream)
+{
+	const uint64_t n = job.n;
+	const int cus = devinfo().cus;
+	Control *ctrl = ctrl_of(ws);
+	const K *in = static_cast<const K *>(job.in);
+	const int rev = (job.flags & kRev) ? 1 : 0;
+	const bool runs = runs_allowed(job);
+	const uint32_t tiles = static_cast<uint32_t>((n + kRunTile - 1) / kRunTile);
+	// four blocks per CU (at most kCheckSlots / 4): each wave's verdict goes to a slot of its own
+	const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(tiles, std::min(4u * cus, kCheckSlots / kRunWaves))));
+	uint32_t *counts = counts_of(ws);
+	hipLaunchKernelGGL((k_check<K, 256>), dim3(grid), dim3(256), 0, stream, in, n, xf, ctrl, counts, runs ? 1 : 0, tiles);
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess || n < 2) return e;
+	hipLaunchKernelGGL((k_check_fold<1024>), dim3(1), dim3(1024), 0, stream, ctrl, counts, grid * kRunWaves);
+	if ((e = hipGetLastError()) != hipSuccess || !runs) return e;
+	const uint32_t cblocks = (tiles + kRunBlock - 1) / kRunBlock;  // at most 1024 (n < 2^32)
+	hipLaunchKernelGGL((k_runs_reduce<kRunBlock>), dim3(cblocks), dim3(kRunBlock), 0, stream, ctrl, rev, n, counts, tiles);
+	if ((e = hipGetLastError()) != hipSuccess) return e;
+	hipLaunchKernelGGL((k_runs_carry<kRunBlock>), dim3(cblocks), dim3(kRunBlock), 0, stream, ctrl, rev, n, counts, tiles);
+	if ((e = hipGetLastError()) != hipSuccess) return e;
+	// one block per tile (a tile loop in the block needs four times the registers): when the verdict is
+	// another, the blocks exit at once (about 17 us for 2^16 tiles)
+	hipLaunchKernelGGL((k_runs_copy<K, 256>), dim3(tiles), dim3(256), 0, stream, in, static_cast<K *>(job.out), n, xf, ctrl, rev,
+		counts, tiles);
+	return hipGetLastError();
+}
+
 // Whether the sample partition plan may group instead of rank (PlanArgs::group): keys-only sorts
 // whose transform is one to one (unsigned, signed, the tiered absolute values and the special
 // float order: equal sort keys are equal keys, so no order among them can be seen), and the float
@@ -1774,11 +2196,7 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 	constexpr bool kTiers = sizeof(K) == 8;
 	const int group = sample_group_class(job);
 	hipError_t e;
-	if (job.presort_check) {
-		const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 4095) / 4096, 4u * cus)));
-		hipLaunchKernelGGL((k_check<K, 256>), dim3(grid), dim3(256), 0, stream, in, n, xf, ctrl);
-		if ((e = hipGetLastError()) != hipSuccess) return e;
-	}
+	if (job.presort_check && (e = launch_check<K>(job, ws, xf, stream)) != hipSuccess) return e;
 	{
 		ProfScope ps(0, 0, ws, stream);
 		// chunks of at least kSampleMax keys each, a power of two (every chunk keeps kSampleMax / G samples)
@@ -2114,11 +2532,7 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	// partition writes (X1, never the source), so the number of blocks is bounded by that array's
 	// size (joint_slices).
 	uint32_t *joint_part = static_cast<uint32_t *>(job.buf != job.in ? job.buf : job.out);
-	if (job.presort_check && !fill) {
-		const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>((n + 4095) / 4096, 4u * cus)));
-		hipLaunchKernelGGL((k_check<K, 256>), dim3(grid), dim3(256), 0, stream, static_cast<const K *>(job.in), n, xf, ctrl);
-		if ((e = hipGetLastError()) != hipSuccess) return e;
-	}
+	if (job.presort_check && !fill && (e = launch_check<K>(job, ws, xf, stream)) != hipSuccess) return e;
 	{
 		ProfScope ps(0, 0, ws, stream);
 		// four blocks per CU: measured 0.44 ms for the 2^28 u64 first read against 0.48 with eight
