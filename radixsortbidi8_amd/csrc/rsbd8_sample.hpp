@@ -472,6 +472,7 @@ __global__ __launch_bounds__(256) void k_plan_sample(PlanArgs a, Control *ctrl, 
 		f.iota = a.payload && !a.psrc;
 		f.reverse = (pres == 2u);
 		f.active = (a.src != a.fin) || f.reverse || (a.payload && (f.iota || a.psrc != a.pfin));
+		if (spec_done(ctrl, pres, a.fin, a.payload)) f.active = 0;  // the check already copied it
 		return;
 	}
 	void *X1 = a.other != a.src ? a.other : a.fin;

@@ -126,6 +126,12 @@ __device__ __forceinline__ uint32_t presort_result(uint32_t check, int reverse)
 	return !lt ? 2u : (!gt ? (!eq ? 1u : (runs ? 3u : 0u)) : 0u);
 }
 
+// the verdict is "in order" and the check copied the input to dst while it read it
+__device__ __forceinline__ bool spec_done(const Control *ctrl, uint32_t pres, const void *dst, bool payload)
+{
+	return pres == 1u && (ctrl->check & 32u) && ctrl->spec_dst == dst && !payload;
+}
+
 // ---------------------------------------------------------------------------------------------
 // histogram of every digit in one read
 
@@ -218,13 +224,19 @@ __device__ __forceinline__ K lane0_of(K v)
 // every wave records the last run start and the first run end of its quarter (a run starts right
 // after an end), as bit masks over its keys: bounds kept in the loop by conditional updates lost the
 // first iteration's update in the gfx950 build, so the run kernels use masks and selects only.
+// With spec set (keys-only sorts to another array), a wave also copies its keys to spec while it has
+// seen no descent: when the verdict is "in order" every key is then in place and the final copy is
+// skipped (check bit 5), and on unsorted input the waves stop copying within their first tile.
 template <typename K, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_check(const K *__restrict__ in, uint64_t n, KeyXform xf, Control *ctrl,
-	uint32_t *__restrict__ words, int runs, uint32_t tiles)
+	uint32_t *__restrict__ words, int runs, uint32_t tiles, K *__restrict__ spec)
 {
 	constexpr uint32_t PER = kRunWaveKeys / 64;
 	static_assert(THREADS / 64 == kRunWaves && PER * 64 == kRunWaveKeys, "a wave reads a quarter tile");
-	if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&ctrl->check, runs ? 17u : 1u);
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		if (spec) ctrl->spec_dst = spec;
+		atomicOr(&ctrl->check, (runs ? 17u : 1u) | (spec ? 32u : 0u));
+	}
 	if (n < 2) return;
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	const uint64_t pairs = n - 1;
@@ -252,9 +264,14 @@ __global__ __launch_bounds__(THREADS) void k_check(const K *__restrict__ in, uin
 		if (lane == 63 && qend < n) tail = in[qend];
 		if (lane == 0 && base > 0 && base < n) head = in[base - 1];
 		uint32_t mine = 0, endm = 0, stm = 0;
+		// the copy goes on while the wave's earlier tiles showed no descent (a tile holding the first
+		// descent is still copied: the sort that follows overwrites it), key by key, so no key stays
+		// live past its compare
+		const bool copy = spec && !(seen & 8u);
 #pragma unroll
 		for (uint32_t i = 0; i < PER; ++i) {
 			const uint64_t j = base + i * 64 + lane;
+			if (copy && j < n) spec[j] = k[i];
 			// lane 0's key i + 1, read by the whole wave (the first active lane must be lane 0)
 			const K down = __shfl_down(k[i], 1, 64), first = lane0_of<K>(k[i + 1 < PER ? i + 1 : i]);
 			const K nx = lane < 63 ? down : (i + 1 < PER ? first : tail);
@@ -267,6 +284,7 @@ __global__ __launch_bounds__(THREADS) void k_check(const K *__restrict__ in, uin
 		}
 		seen |= (__ballot((mine & 2u) != 0) ? 2u : 0u) | (__ballot((mine & 4u) != 0) ? 4u : 0u) |
 			(__ballot((mine & 8u) != 0) ? 8u : 0u);
+
 		if (runs) {
 			uint32_t en = endm ? static_cast<uint32_t>(base + (__ffs(endm) - 1u) * 64 + lane) : last;
 			uint32_t st = stm ? static_cast<uint32_t>(base + (31u - __clz(stm)) * 64 + lane + 1u) : 0u;
@@ -753,6 +771,7 @@ __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_
 		// already in order: the result is the input (reversed when it was strictly descending)
 		f.reverse = pres ? (pres == 2u) : a.reverse;
 		f.active = (a.src != a.fin) || f.reverse || (a.payload && (f.iota || a.psrc != a.pfin));
+		if (spec_done(ctrl, pres, a.fin, a.payload)) f.active = 0;  // the check already copied it
 		return;
 	}
 	const void *src = a.src;
@@ -2141,7 +2160,8 @@ hipError_t launch_check(const SortJob &job, void *ws, const KeyXform &xf, hipStr
 	// four blocks per CU (at most kCheckSlots / 4): each wave's verdict goes to a slot of its own
 	const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(tiles, std::min(4u * cus, kCheckSlots / kRunWaves))));
 	uint32_t *counts = counts_of(ws);
-	hipLaunchKernelGGL((k_check<K, 256>), dim3(grid), dim3(256), 0, stream, in, n, xf, ctrl, counts, runs ? 1 : 0, tiles);
+	hipLaunchKernelGGL((k_check<K, 256>), dim3(grid), dim3(256), 0, stream, in, n, xf, ctrl, counts, runs ? 1 : 0, tiles,
+		runs ? static_cast<K *>(job.out) : nullptr);
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess || n < 2) return e;
 	hipLaunchKernelGGL((k_check_fold<1024>), dim3(1), dim3(1024), 0, stream, ctrl, counts, grid * kRunWaves);

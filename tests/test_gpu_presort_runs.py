@@ -62,21 +62,38 @@ def test_reverse_with_ties_takes_the_run_reversal(kt):
                         assert np.array_equal(got, exp), (n, pool, mode, d, plan)
 
 
-@pytest.mark.parametrize("kt", [r.U32, r.F64])
+@pytest.mark.parametrize("kt", [r.U32, r.U64, r.F64])
 def test_presort_other_verdicts_unchanged(kt):
-    """Strictly reversed input (a reversed copy), sorted input (a copy), unsorted input (a sort)."""
+    """Strictly reversed input (a reversed copy), sorted input (a copy, made by the check itself while
+    it reads), unsorted input (a sort), and sorted inputs with one descent early, in the middle or at
+    the very end: the check copies the tiles before the descent, then a real sort overwrites them. The
+    output array keeps the previous sort's keys between calls."""
     rng = np.random.default_rng(950 + kt)
     n = (1 << 18) + 3
     nb = KT[kt]
     raw = rng.permutation(n).astype(UINT[nb])
     if kt == r.F64:
         raw = rng.permutation(n).astype(np.float64).view(np.uint64)
-    flags = r.decode(kt, 64, 0)
-    srt, _ = oracle.radixsort(raw, flags)
-    for bits in (srt[::-1].copy(), srt, raw):
-        exp, _ = oracle.radixsort(bits, flags)
-        got = run_plan(bits, kt, flags, (2, -1, 1))
-        assert np.array_equal(got, exp)
+    for d in (0, 2):
+        flags = r.decode(kt, 64, d)
+        srt, _ = oracle.radixsort(raw, flags)
+        phys = srt if d == 0 else srt[::-1].copy()  # in order when read in the sort's direction
+        cases = [phys[::-1].copy(), phys, raw]
+        for at in (5, n // 2, n - 2):
+            near = phys.copy()
+            near[at], near[at + 1] = near[at + 1], near[at]
+            cases.append(near)
+        for plan in ((2, -1, 1), (4, -1, 1), (1, -1, 1)):
+            a = torch.empty(n, dtype=TORCH[kt], device="cuda")
+            out, buf = torch.empty_like(a), torch.empty_like(a)
+            ws = r.Workspace(kt, n)
+            for bits in cases:
+                exp, _ = oracle.radixsort(bits, flags)
+                a.copy_(torch.from_numpy(bits).view(TORCH[kt]))
+                pl.sort_with_plan(a, out, buf, ws, plan, flags)
+                torch.cuda.synchronize()
+                got = out.cpu().numpy().view(bits.dtype)
+                assert np.array_equal(got, exp), (d, plan)
 
 
 @pytest.mark.parametrize("kt", [r.U8, r.U16, r.U32, r.U64])
