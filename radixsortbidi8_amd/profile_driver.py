@@ -2,10 +2,12 @@
 
   python -m radixsortbidi8_amd.profile_driver [--n-log2 28] [--dtype u64] [--reps 2] [--algo A]
                                                [--tile-config C] [--copy] [--normal] [--payload]
-                                               [--mode M] [--direction D]
+                                               [--mode M] [--direction D] [--dist D] [--presort-check]
                                                [--segmented [--small-cfg S] [--sweep]]
 
-Sorts 2^n seeded uniform keys (the benchmark's distribution) reps times after one warm-up sort,
+Sorts 2^n seeded uniform keys (the benchmark's distribution; --dist picks another of the request
+mix, e.g. sorted or reverse; --presort-check sorts with the plan (algorithm, tile configuration,
+presort check on), algorithm 4 unless --algo says otherwise) reps times after one warm-up sort,
 or (--segmented) sorts a batch of 65,536 segments with log-uniform lengths totalling 2^n keys (int32
 keys, or u64/f64 with --seg-dtype; plan tile configuration --tile-config, LDS tier --small-cfg; --sweep
 times every pair), then, with --copy, copies the same number of bytes device to device reps times: a plain copy of
@@ -30,6 +32,8 @@ def main():
                     help="stable key sort carrying the u32 source index (rsbd8_hip_sort_pairs)")
     ap.add_argument("--mode", type=int, default=64, help="sortingmode (64 native, 65 absolute value, ...)")
     ap.add_argument("--direction", type=int, default=0, help="sortingdirection (0 ascfwdorder ... 3 dscrevorder)")
+    ap.add_argument("--dist", default="uniform", help="key distribution of planner.make_keys (uniform, sorted, reverse, ...)")
+    ap.add_argument("--presort-check", action="store_true", help="sort with the presort check on (a planned sort)")
     ap.add_argument("--segmented", action="store_true",
                     help="instead: 65,536 int32 segments, log-uniform lengths scaled to 2^n keys in total")
     ap.add_argument("--small-cfg", type=int, default=-1, help="segmented: LDS tier of the plan (-1 automatic)")
@@ -49,7 +53,7 @@ def main():
     if args.normal and args.dtype in ("f32", "f64"):
         keys = torch.randn(n, dtype=pl.torch_dtype(args.dtype), device="cuda", generator=gen)
     else:
-        keys = pl.make_keys(args.dtype, n, "uniform", gen)
+        keys = pl.make_keys(args.dtype, n, args.dist, gen)
     out = torch.empty_like(keys)
     buf = torch.empty_like(keys)
     kt = r.keytype_of(keys.dtype)
@@ -63,10 +67,18 @@ def main():
 
         def sort():
             r.sort_pairs(keys, out, buf, None, p_out, p_buf, ws, args.direction, args.mode)
+    elif args.presort_check:
+        plan = (args.algo or 4, args.tile_config, 1)
+        flags = r.decode(kt, args.mode, args.direction)
+
+        def sort():
+            pl.sort_with_plan(keys, out, buf, ws, plan, flags)
     else:
         def sort():
             r.radixsortcopynoalloc(keys, out, buf, ws, args.direction, args.mode)
-    what = f"2^{args.n_log2} {args.dtype}{' N(0,1)' if args.normal else ''}{' + u32 index payload' if args.payload else ''}"
+    what = (f"2^{args.n_log2} {args.dtype}{' N(0,1)' if args.normal else ''}"
+            f"{'' if args.dist == 'uniform' else ' ' + args.dist}{' + u32 index payload' if args.payload else ''}"
+            f"{' with the presort check' if args.presort_check else ''}")
     sort()
     torch.cuda.synchronize()
     print(f"profile_driver: warm-up sort of {what} (mode {args.mode}, direction {args.direction}) done", flush=True)
