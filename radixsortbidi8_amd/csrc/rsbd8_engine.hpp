@@ -38,7 +38,8 @@ struct Control {
 	uint32_t active_mask;
 	uint32_t counts_ready; // bit p: the tile counts of pass p were produced by the histogram read
 	uint32_t check;        // presort check: bit 0 ran, bit 1 some k[j] < k[j+1], bit 2 equal, bit 3 greater,
-	                       // bit 4 the run reversal allowed, bit 5 the input copied to spec_dst where in order
+	                       // bit 4 the run reversal allowed, bits 5 / 6 some wave copied the input to spec_dst
+	                       // in order / reversed
 	uint32_t pad[3];
 	PassPlan pass[kMaxPasses];
 	PassPlan pre;          // copy before the first pass (in-place sorts that would overlap)
@@ -56,7 +57,7 @@ struct Control {
 	uint32_t bucket_count2;// ... and for the larger one
 	uint32_t zero_signs;   // sample partition plan's first read: bit 0 a +0.0 key seen, bit 1 a -0.0 key
 	uint32_t group_parts;  // sample partition plan: 1 when its partitions group (unstable), 0 when they rank
-	void *spec_dst;        // presort check: the array it copied the input to while no descent was seen (check bit 5)
+	void *spec_dst;        // presort check: the array it copied the input to (check bits 5 and 6)
 	// presort check: some wave saw an ascent and a descent (the others may stop): word 0 of 16 cache
 	// lines, block b setting line b % 16 (stores to one word from every wave of the grid queue on one
 	// L2 channel)

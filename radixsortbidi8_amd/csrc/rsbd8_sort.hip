@@ -126,10 +126,12 @@ __device__ __forceinline__ uint32_t presort_result(uint32_t check, int reverse)
 	return !lt ? 2u : (!gt ? (!eq ? 1u : (runs ? 3u : 0u)) : 0u);
 }
 
-// the verdict is "in order" and the check copied the input to dst while it read it
+// the verdict is "in order" (a copy) or "strictly reversed" (a reversed copy), and the check wrote
+// that copy to dst while it read the input (every wave in that direction only)
 __device__ __forceinline__ bool spec_done(const Control *ctrl, uint32_t pres, const void *dst, bool payload)
 {
-	return pres == 1u && (ctrl->check & 32u) && ctrl->spec_dst == dst && !payload;
+	const uint32_t dirs = ctrl->check & 96u;
+	return ((pres == 1u && dirs == 32u) || (pres == 2u && dirs == 64u)) && ctrl->spec_dst == dst && !payload;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -224,25 +226,30 @@ __device__ __forceinline__ K lane0_of(K v)
 // every wave records the last run start and the first run end of its quarter (a run starts right
 // after an end), as bit masks over its keys: bounds kept in the loop by conditional updates lost the
 // first iteration's update in the gfx950 build, so the run kernels use masks and selects only.
-// With spec set (keys-only sorts to another array), a wave also copies its keys to spec while it has
-// seen no descent: when the verdict is "in order" every key is then in place and the final copy is
-// skipped (check bit 5), and on unsorted input the waves stop copying within their first tile.
+// With spec set (keys-only sorts to another array), a wave also copies its keys to spec, in order
+// (spec[j]) while nothing it saw contradicts the verdict "in order", or reversed (spec[n - 1 - j])
+// while nothing contradicts "strictly reversed" (read back to front: reversed with ties), deciding
+// per tile from what it saw before and the tile's first row; its slot records the directions it
+// wrote (bit 5 in order, bit 6 reversed). When the verdict is 1 or 2 and every wave wrote only that
+// verdict's direction, every key is in place and the final copy is skipped; on unsorted input the
+// waves stop copying within their first tile.
 template <typename K, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_check(const K *__restrict__ in, uint64_t n, KeyXform xf, Control *ctrl,
-	uint32_t *__restrict__ words, int runs, uint32_t tiles, K *__restrict__ spec)
+	uint32_t *__restrict__ words, int runs, uint32_t tiles, K *__restrict__ spec, int reverse)
 {
 	constexpr uint32_t PER = kRunWaveKeys / 64;
 	static_assert(THREADS / 64 == kRunWaves && PER * 64 == kRunWaveKeys, "a wave reads a quarter tile");
 	if (blockIdx.x == 0 && threadIdx.x == 0) {
 		if (spec) ctrl->spec_dst = spec;
-		atomicOr(&ctrl->check, (runs ? 17u : 1u) | (spec ? 32u : 0u));
+		atomicOr(&ctrl->check, runs ? 17u : 1u);
 	}
 	if (n < 2) return;
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	const uint64_t pairs = n - 1;
 	const uint32_t last = static_cast<uint32_t>(n - 1);
 	uint32_t *slot = count_word(words, blockIdx.x * kRunWaves + w);
-	uint32_t seen = 0;  // what this wave saw (wave-uniform), stored to its slot when it ends
+	uint32_t seen = 0;   // what this wave saw (wave-uniform), stored to its slot when it ends
+	uint32_t wrote = 0;  // the directions this wave copied in (bits 5 and 6)
 	uint32_t it = 0;
 	for (uint64_t tb = static_cast<uint64_t>(blockIdx.x) * kRunTile; tb < n; tb += static_cast<uint64_t>(gridDim.x) * kRunTile, ++it) {
 		const uint64_t base = tb + w * kRunWaveKeys;
@@ -264,14 +271,23 @@ __global__ __launch_bounds__(THREADS) void k_check(const K *__restrict__ in, uin
 		if (lane == 63 && qend < n) tail = in[qend];
 		if (lane == 0 && base > 0 && base < n) head = in[base - 1];
 		uint32_t mine = 0, endm = 0, stm = 0;
-		// the copy goes on while the wave's earlier tiles showed no descent (a tile holding the first
-		// descent is still copied: the sort that follows overwrites it), key by key, so no key stays
-		// live past its compare
-		const bool copy = spec && !(seen & 8u);
+		// the copy's direction for this tile, from the earlier tiles and the tile's first row (a tile
+		// whose later rows contradict it is still copied: the sort that follows overwrites it), key by
+		// key, so no key stays live past its compare
+		bool fwd = false, bwd = false;
+		if (spec && base < n) {  // (a quarter past the last key records no direction)
+			const auto a0 = order_key<K>(k[0], xf), b0 = order_key<K>(static_cast<K>(__shfl_down(k[0], 1, 64)), xf);
+			const bool pv = lane < 63 && base + lane + 1 < n;
+			const uint32_t pre = seen | (__ballot(pv && a0 < b0) ? 2u : 0u) | (__ballot(pv && a0 == b0) ? 4u : 0u) |
+				(__ballot(pv && a0 > b0) ? 8u : 0u);
+			fwd = !(pre & 8u) && !(reverse && (pre & 4u));
+			bwd = !fwd && !(pre & 2u) && !(!reverse && (pre & 4u));
+			wrote |= (fwd ? 32u : 0u) | (bwd ? 64u : 0u);
+		}
 #pragma unroll
 		for (uint32_t i = 0; i < PER; ++i) {
 			const uint64_t j = base + i * 64 + lane;
-			if (copy && j < n) spec[j] = k[i];
+			if ((fwd || bwd) && j < n) spec[fwd ? j : n - 1 - j] = k[i];
 			// lane 0's key i + 1, read by the whole wave (the first active lane must be lane 0)
 			const K down = __shfl_down(k[i], 1, 64), first = lane0_of<K>(k[i + 1 < PER ? i + 1 : i]);
 			const K nx = lane < 63 ? down : (i + 1 < PER ? first : tail);
@@ -310,7 +326,7 @@ __global__ __launch_bounds__(THREADS) void k_check(const K *__restrict__ in, uin
 		}
 		if (stop) break;
 	}
-	if (lane == 0) *slot = seen;
+	if (lane == 0) *slot = seen | wrote;
 }
 
 // The verdict: every wave's slot of the check, or'd into the control block (one atomic).
@@ -2161,7 +2177,7 @@ hipError_t launch_check(const SortJob &job, void *ws, const KeyXform &xf, hipStr
 	const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(tiles, std::min(4u * cus, kCheckSlots / kRunWaves))));
 	uint32_t *counts = counts_of(ws);
 	hipLaunchKernelGGL((k_check<K, 256>), dim3(grid), dim3(256), 0, stream, in, n, xf, ctrl, counts, runs ? 1 : 0, tiles,
-		runs ? static_cast<K *>(job.out) : nullptr);
+		runs ? static_cast<K *>(job.out) : nullptr, rev);
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess || n < 2) return e;
 	hipLaunchKernelGGL((k_check_fold<1024>), dim3(1), dim3(1024), 0, stream, ctrl, counts, grid * kRunWaves);

@@ -64,10 +64,10 @@ def test_reverse_with_ties_takes_the_run_reversal(kt):
 
 @pytest.mark.parametrize("kt", [r.U32, r.U64, r.F64])
 def test_presort_other_verdicts_unchanged(kt):
-    """Strictly reversed input (a reversed copy), sorted input (a copy, made by the check itself while
-    it reads), unsorted input (a sort), and sorted inputs with one descent early, in the middle or at
-    the very end: the check copies the tiles before the descent, then a real sort overwrites them. The
-    output array keeps the previous sort's keys between calls."""
+    """Strictly reversed input (a reversed copy) and sorted input (a copy), both made by the check
+    itself while it reads; unsorted input (a sort); and sorted or strictly reversed inputs with one
+    swap early, in the middle or at the very end: the check copies the tiles before it, then a real
+    sort overwrites them. The output array keeps the previous sort's keys between calls."""
     rng = np.random.default_rng(950 + kt)
     n = (1 << 18) + 3
     nb = KT[kt]
@@ -80,9 +80,11 @@ def test_presort_other_verdicts_unchanged(kt):
         phys = srt if d == 0 else srt[::-1].copy()  # in order when read in the sort's direction
         cases = [phys[::-1].copy(), phys, raw]
         for at in (5, n // 2, n - 2):
-            near = phys.copy()
-            near[at], near[at + 1] = near[at + 1], near[at]
-            cases.append(near)
+            for base in (phys, phys[::-1]):  # nearly sorted, nearly strictly reversed
+                near = base.copy()
+                near[at], near[at + 1] = near[at + 1], near[at]
+                cases.append(near)
+        cases.append(phys[::-1].copy())  # a reversed copy again after the partial copies
         for plan in ((2, -1, 1), (4, -1, 1), (1, -1, 1)):
             a = torch.empty(n, dtype=TORCH[kt], device="cuda")
             out, buf = torch.empty_like(a), torch.empty_like(a)
