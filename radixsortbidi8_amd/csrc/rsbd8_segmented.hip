@@ -1013,10 +1013,16 @@ hipError_t launch_sort_cfg(int cfg, const void *src, void *dst, const uint64_t *
 	const uint32_t *count, uint32_t count_value, const KeyXform &xf, int reverse, uint32_t *queue, uint32_t *qcount,
 	uint32_t qbase, hipStream_t s)
 {
+	// partition bits per tier: about one key per partition for 4-byte keys in the 4096- and 8192-key
+	// tiers (C4 scaled to 2^28 keys: 2.24 -> 2.04-2.10 ms per graph, 13 bits in the 4096 tier 2.21; at
+	// its stated lengths, where half the buckets of a 2^20-key segment take the 8192 tier, 46.3 ms per
+	// graph warm, 50.4 in a fresh box's first process); 8-byte keys measured the same either way and
+	// keep the smaller counter tables
+	constexpr int PB1 = sizeof(K) == 4 ? 12 : 11, PB2 = sizeof(K) == 4 ? 13 : 11;
 	switch (cfg) {
 	case 0: return launch_sort_t<K, 256, 4, 8>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase, s);
-	case 1: return launch_sort_t<K, 512, 8, 11>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase, s);
-	default: return launch_sort_t<K, 512, 16, 11>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase, s);
+	case 1: return launch_sort_t<K, 512, 8, PB1>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase, s);
+	default: return launch_sort_t<K, 512, 16, PB2>(src, dst, istart, ilen, count, count_value, xf, reverse, queue, qcount, qbase, s);
 	}
 }
 
