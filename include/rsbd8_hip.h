@@ -199,7 +199,10 @@ int rsbd8_hip_set_sample_target(int keys);
  * 3 a single block sorting everything in LDS (keys only, count <= 16384; otherwise automatic),
  * 4 the two-level MSD plan (as for rsbd8_hip_set_algorithm); tile_cfg -1 automatic or 0..4 as for rsbd8_hip_set_tile_config; presort_check first tests
  * whether the input is already in order or strictly reversed (then the sort is one copy), or in
- * reverse order with ties (then one pass moves the runs of equal keys, each in its own order). */
+ * reverse order with ties (then one pass moves the runs of equal keys, each in its own order).
+ * For keys-only sorts to another array the check writes output while it reads (the copy the
+ * verdict will ask for, where the input allows it) and the later work overwrites what it must: the
+ * output array holds no input-dependent data the caller may rely on until the sort completes. */
 typedef struct rsbd8_hip_plan {
 	int algo;
 	int tile_cfg;
