@@ -197,7 +197,10 @@ int rsbd8_hip_set_sample_target(int keys);
 /* A sort plan chosen per call (no reference counterpart; the reference picks its plan from
  * compile-time thresholds, H:12948-13152): algo 0 automatic, 1 onesweep, 2 count + scan + scatter,
  * 3 a single block sorting everything in LDS (keys only, count <= 16384; otherwise automatic),
- * 4 the two-level MSD plan (as for rsbd8_hip_set_algorithm); tile_cfg -1 automatic or 0..4 as for rsbd8_hip_set_tile_config; presort_check first tests
+ * 4 the two-level MSD plan (as for rsbd8_hip_set_algorithm), 5 the two-level MSD plan by splitters
+ * whatever the keys (4- and 8-byte keys with a buffer, else automatic: integer keys with few values
+ * or a skewed distribution get equality buckets instead of failing the exact plan's bucket test);
+ * tile_cfg -1 automatic or 0..4 as for rsbd8_hip_set_tile_config; presort_check first tests
  * whether the input is already in order or strictly reversed (then the sort is one copy), or in
  * reverse order with ties (then one pass moves the runs of equal keys, each in its own order).
  * For keys-only sorts to another array the check writes output while it reads (the copy the
@@ -216,8 +219,8 @@ int rsbd8_hip_radixsortcopynoalloc_plan(int keytype, unsigned flags, size_t coun
 
 /* The candidate plans, indexed in the order of the selector network's outputs (0 <= index <
  * rsbd8_hip_plan_count()): onesweep, count + scan, then the two-level MSD plan, each over the tile
- * configurations, each without and with the presort check; the last is the single-block LDS sort
- * (count <= 16384). */
+ * configurations, each without and with the presort check; then algorithm 5 (4- and 8-byte keys, count
+ * >= 65536); the last is the single-block LDS sort (count <= 16384). */
 int rsbd8_hip_plan_count(void);
 int rsbd8_hip_plan_at(int index, rsbd8_hip_plan *plan);
 

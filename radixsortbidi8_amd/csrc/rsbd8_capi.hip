@@ -723,23 +723,28 @@ void rsbd8_hip_segmented_plan_destroy(rsbd8_hip_segplan *plan)
 namespace {
 
 // The candidate plans in the order of the selector network's outputs: (onesweep, count + scan,
-// two-level MSD) x tile configuration x (no presort check, presort check), then the single-block
-// LDS sort.
+// two-level MSD) x tile configuration x (no presort check, presort check), then the sample
+// partition plan (algorithm 5, its own tiles) and the single-block LDS sort.
 constexpr int kPlanAlgos[3] = {kAlgoOnesweep, kAlgoCountScan, kAlgoHybrid};
-constexpr int kNumPlans = 3 * kNumTileCfgs * 2 + 1;
+constexpr int kNumGridPlans = 3 * kNumTileCfgs * 2;
+constexpr int kNumPlans = kNumGridPlans + 2;
+constexpr int kPlanSample = kNumGridPlans, kPlanLds = kNumGridPlans + 1;
+constexpr size_t kSampleMinCount = size_t{1} << 16;
 static_assert(kNumPlans <= 32, "plans are selected through a 32-bit validity mask");
 
 rsbd8_hip_plan plan_at(int i)
 {
-	if (i == kNumPlans - 1) return rsbd8_hip_plan{kAlgoLds, -1, 0, 0};
+	if (i == kPlanLds) return rsbd8_hip_plan{kAlgoLds, -1, 0, 0};
+	if (i == kPlanSample) return rsbd8_hip_plan{kAlgoSample, -1, 0, 0};
 	return rsbd8_hip_plan{kPlanAlgos[i / (2 * kNumTileCfgs)], (i / 2) % kNumTileCfgs, i & 1, 0};
 }
 
-// plans that can run on count keys, as a bit mask over plan indices
-uint32_t valid_plans(size_t count)
+// plans that can run on count keys of key_bytes bytes, as a bit mask over plan indices
+uint32_t valid_plans(size_t count, int key_bytes)
 {
-	uint32_t m = (1u << (kNumPlans - 1)) - 1u;
-	if (count <= kLdsMax) m |= 1u << (kNumPlans - 1);
+	uint32_t m = (1u << kNumGridPlans) - 1u;
+	if (key_bytes >= 4 && count >= kSampleMinCount) m |= 1u << kPlanSample;
+	if (count <= kLdsMax) m |= 1u << kPlanLds;
 	return m;
 }
 
@@ -753,7 +758,7 @@ int sort_with_plan(int keytype, unsigned flags, size_t count, void *workspace, s
 	if (!input || !output || !workspace) return fail(RSBD8_HIP_EINVAL, "null argument");
 	if (input == output || (buffer && (buffer == input || buffer == output))) return fail(RSBD8_HIP_EINVAL, "arrays alias");
 	if (!buffer && needs_buffer(kb, flags) && kb != 1) return fail(RSBD8_HIP_EINVAL, "buffer required");
-	if (plan.algo < 0 || plan.algo > 4 || plan.tile_cfg < -1 || plan.tile_cfg >= kNumTileCfgs)
+	if (plan.algo < 0 || plan.algo > kAlgoSample || plan.tile_cfg < -1 || plan.tile_cfg >= kNumTileCfgs)
 		return fail(RSBD8_HIP_EINVAL, "plan out of range");
 	if ((r = check_device_arrays({workspace, input, output, buffer}, what))) return r;
 	SortJob job;
@@ -810,7 +815,7 @@ int rsbd8_hip_select_plan(int keytype, unsigned flags, size_t count, const void 
 	if (!slot) return fail(RSBD8_HIP_ENOMEM, "pinned selection slot");
 	*slot = -1;
 	const hipStream_t s = static_cast<hipStream_t>(stream);
-	hipError_t e = select_launch(kb, flags, count, input, hidden, kNumPlans, params, valid_plans(count), slot, s);
+	hipError_t e = select_launch(kb, flags, count, input, hidden, kNumPlans, params, valid_plans(count, kb), slot, s);
 	if (e == hipSuccess) e = hipStreamSynchronize(s);
 	if (e != hipSuccess) return fail_hip(e, "select_plan");
 	const int idx = __atomic_load_n(slot, __ATOMIC_ACQUIRE);

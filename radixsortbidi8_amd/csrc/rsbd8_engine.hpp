@@ -124,11 +124,12 @@ struct SortJob {
 	                               // 3 one block sorts everything in LDS (keys only, n <= kLdsMax),
 	                               // 4 two MSD partitions + per-bucket LDS sort (4/8-byte keys: by the top two
 	                               //   digits for keys-only integer sorts under an injective transform, the
-	                               //   device falling back to algorithm 2 for skewed keys; else by splitters)
+	                               //   device falling back to algorithm 2 for skewed keys; else by splitters),
+	                               // 5 the same by splitters whatever the keys (4/8-byte keys with a buffer; else automatic)
 	bool presort_check = false;    // first test whether the input is already in order (or strictly reversed)
 };
 
-constexpr int kAlgoAuto = 0, kAlgoOnesweep = 1, kAlgoCountScan = 2, kAlgoLds = 3, kAlgoHybrid = 4;
+constexpr int kAlgoAuto = 0, kAlgoOnesweep = 1, kAlgoCountScan = 2, kAlgoLds = 3, kAlgoHybrid = 4, kAlgoSample = 5;
 constexpr uint32_t kLdsMax = 1024 * 16;    // keys-only sorts one LDS block takes
 constexpr uint32_t kLdsPayMax = 512 * 16;  // the same with a u32 payload
 

@@ -2474,6 +2474,14 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	const int cfg = job.cfg >= 0 ? job.cfg : default_cfg(P, n, job.payload, float_keys);
 	const bool keys_only = !job.payload && !job.keys_final_any;
 	int algo = job.algo ? job.algo : default_algo(P, n, keys_only, float_keys);
+	if (algo == kAlgoSample) {
+		// the sample partition plan whatever the keys: skewed and few-valued integer keys, which fail
+		// the exact plan's bucket test, get equality buckets of their repeated values
+		if constexpr (P >= 4) {
+			if (job.buf) return launch_sample<K>(cfg, job, ws, stream);
+		}
+		algo = default_algo(P, n, keys_only, float_keys);
+	}
 	if (!job.algo) {
 		// The automatic plan and the sample partition plan (measured on MI355X, uniform and N(0,1)
 		// keys, against the LSD passes): 8-byte keys take it from 2^22 when they are floats (f64

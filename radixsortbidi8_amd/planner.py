@@ -37,7 +37,8 @@ NUM_FEATURES = 24
 HIDDEN = 32
 LDS_MAX = 16384
 
-PLANS = [(a, c, chk) for a in (1, 2, 4) for c in range(len(TILE_CONFIGS)) for chk in (0, 1)] + [(3, -1, 0)]
+PLANS = [(a, c, chk) for a in (1, 2, 4) for c in range(len(TILE_CONFIGS)) for chk in (0, 1)] + [(5, -1, 0), (3, -1, 0)]
+SAMPLE_MIN = 1 << 16  # algorithm 5 (the sample partition plan whatever the keys): 4- and 8-byte keys from 2^16
 NUM_PLANS = len(PLANS)
 
 DTYPES = ("u8", "u16", "u32", "u64", "f32", "f64")
@@ -74,7 +75,10 @@ def _lib():
     return L
 
 
-def plan_valid(plan, n: int) -> bool:
+def plan_valid(plan, n: int, nbytes: int = 0) -> bool:
+    """Whether plan runs on n keys of nbytes bytes (0: any width), as the library's selection masks it."""
+    if plan[0] == 5:
+        return n >= SAMPLE_MIN and nbytes in (0, 4, 8)
     return plan[0] != 3 or n <= LDS_MAX
 
 
@@ -242,7 +246,7 @@ def time_plans(keys, flags: int, arena: Arena, reps: int = 3):
     out, buf = arena.views(keys)
     res = np.full(NUM_PLANS, np.nan)
     for i, p in enumerate(PLANS):
-        if not plan_valid(p, keys.numel()):
+        if not plan_valid(p, keys.numel(), keys.element_size()):
             continue
         sort_with_plan(keys, out, buf, arena.ws, p, flags)   # warm the plan once
         res[i] = time_call(lambda: sort_with_plan(keys, out, buf, arena.ws, p, flags), reps)
@@ -321,11 +325,11 @@ class Selector:
                                                              ctypes.byref(idx), _stream_ptr(stream)))
         return idx.value
 
-    def choose(self, feats, n: int):
-        """Index of the plan with the lowest predicted time among those valid for n."""
+    def choose(self, feats, n: int, nbytes: int = 0):
+        """Index of the plan with the lowest predicted time among those valid for n keys of nbytes."""
         y = self.forward(feats.view(1, -1))[0].cpu().numpy()
         for i, p in enumerate(PLANS):
-            if not plan_valid(p, n):
+            if not plan_valid(p, n, nbytes):
                 y[i] = np.inf
         return int(np.argmin(y))
 
@@ -425,7 +429,7 @@ def autotune(log2_sizes=range(10, 29, 2), dtypes=("u8", "u16", "u32", "u64"), re
             flags = decode(keytype_of(keys.dtype))
             best, best_t = None, float("inf")
             for p in PLANS:
-                if p[2] or not plan_valid(p, n):
+                if p[2] or not plan_valid(p, n, keys.element_size()):
                     continue
                 out, buf = arena.views(keys)
                 sort_with_plan(keys, out, buf, arena.ws, p, flags)
@@ -446,7 +450,7 @@ def fixed_plan(table: dict, nbytes: int, n: int):
         return (0, -1, 0)
     for b in sorted(keys, key=lambda v: (abs(v - lg), v)):
         p = tuple(table[f"{nbytes}:{b}"])
-        if plan_valid(p, n):
+        if plan_valid(p, n, nbytes):
             return p
     return PLANS[0]
 

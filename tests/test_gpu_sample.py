@@ -262,3 +262,34 @@ def test_sample_presort_check(msd):
         planner.sort_with_plan(a, o, b, ws, (4, 2, 1), r.decode(r.F32, 64, 0))
         exp, _ = oracle.radixsort(keys.view(np.uint32), r.decode(r.F32, 64, 0))
         assert np.array_equal(host(o).view(np.uint32), exp)
+
+
+@pytest.mark.parametrize("kt", [r.U32, r.U64, r.I32, r.I64])
+def test_algorithm5_takes_integer_keys_with_few_or_skewed_values(kt):
+    """Algorithm 5 (the sample partition plan whatever the keys) on keys-only integer sorts that fail
+    the exact plan's bucket test: every key equal, 16 distinct values, and a Zipf-like skew (one value
+    in about a tenth of the keys), in every direction, with and without the presort check."""
+    from radixsortbidi8_amd import planner as pl
+    nb = {r.U32: 4, r.U64: 8, r.I32: 4, r.I64: 8}[kt]
+    udt = np.uint32 if nb == 4 else np.uint64
+    tdt = {r.U32: torch.uint32, r.U64: torch.uint64, r.I32: torch.int32, r.I64: torch.int64}[kt]
+    rng = np.random.default_rng(0x5A3 + kt)
+    n = (1 << 20) + 7
+    ws = r.Workspace(kt, n)
+    for shape in ("equal", "unique16", "zipf"):
+        if shape == "equal":
+            bits = np.full(n, rng.integers(0, np.iinfo(udt).max, dtype=udt), dtype=udt)
+        elif shape == "unique16":
+            bits = rng.integers(0, np.iinfo(udt).max, 16, dtype=udt, endpoint=True)[rng.integers(0, 16, n)]
+        else:
+            pool = rng.integers(0, np.iinfo(udt).max, 4096, dtype=udt, endpoint=True)
+            bits = pool[np.minimum(rng.zipf(1.3, n) - 1, 4095)]
+        a = torch.from_numpy(bits).cuda().view(tdt)
+        out, buf = torch.empty_like(a), torch.empty_like(a)
+        for d in range(4):
+            flags = r.decode(kt, 64, d)
+            exp, _ = oracle.radixsort(bits, flags)
+            for plan in ((5, -1, 0), (5, -1, 1)):
+                pl.sort_with_plan(a, out, buf, ws, plan, flags)
+                torch.cuda.synchronize()
+                assert np.array_equal(out.cpu().numpy().view(udt), exp), (shape, d, plan)

@@ -37,7 +37,7 @@ def test_every_plan_bit_exact(dt):
                 exp, _ = oracle.radixsort(host, flags)
                 out, buf = arena.views(keys)
                 for p in pl.PLANS:
-                    if not pl.plan_valid(p, n):
+                    if not pl.plan_valid(p, n, keys.element_size()):
                         continue
                     out.zero_()
                     pl.sort_with_plan(keys, out, buf, arena.ws, p, flags)

@@ -28,6 +28,8 @@ def test_plan_list():
     assert pl.NUM_PLANS == len(set(pl.PLANS))
     assert (3, -1, 0) in pl.PLANS
     assert all(pl.plan_valid(p, 1 << 20) == (p[0] != 3) for p in pl.PLANS)
+    assert (5, -1, 0) in pl.PLANS and not pl.plan_valid((5, -1, 0), 1 << 20, 2)
+    assert not pl.plan_valid((5, -1, 0), (1 << 16) - 1, 8) and pl.plan_valid((5, -1, 0), 1 << 16, 4)
     table = {"8:20": [2, 4, 0], "8:10": [1, 0, 0]}
     assert pl.fixed_plan(table, 8, 1 << 21) == (2, 4, 0)
     assert pl.fixed_plan(table, 8, 1 << 9) == (1, 0, 0)
