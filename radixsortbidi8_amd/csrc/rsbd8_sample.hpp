@@ -435,7 +435,12 @@ __global__ __launch_bounds__(256) void k_plan_sample(PlanArgs a, Control *ctrl, 
 	const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
 	const uint32_t c = hist[t];
 	hist[t] = 0;
-	const uint32_t tiles = (c + a.tile2 - 1) / a.tile2;
+	// an equality bucket of the first table is final after the first partition (which writes it to
+	// the final array): its segment has no tiles and no length at the second level
+	// (not when the final arrays are the source: an in-place sort reads them while it partitions)
+	const bool direct = a.fin != a.src && !(a.payload && a.psrc && a.psrc == a.pfin);
+	const bool eq = direct && a.split0 && ((a.split0[kSplitEqWord + (t >> 5)] >> (t & 31u)) & 1u);
+	const uint32_t tiles = eq ? 0u : (c + a.tile2 - 1) / a.tile2;
 	const uint32_t incl = wave_inclusive_sum(c), tincl = wave_inclusive_sum(tiles);
 	if (lane == 63) {
 		s_wave[w] = incl;
@@ -453,7 +458,7 @@ __global__ __launch_bounds__(256) void k_plan_sample(PlanArgs a, Control *ctrl, 
 	T.tile_start[t] = tbefore + tincl - tiles;
 	if (t == kRadix - 1) T.tile_start[kRadix] = tbefore + tincl;
 	T.seg_start[t] = start;
-	T.seg_len[t] = c;
+	T.seg_len[t] = eq ? 0u : c;
 	if (t != 0) return;
 	PassPlan none{};
 	for (int h = 0; h < 3; ++h) ctrl->hyb[h] = none;

@@ -715,6 +715,7 @@ struct PlanArgs {
 	int group;          // sample partition plan, keys only: equal sort keys are equal keys (1), or are
 	                    // unless both +0.0 and -0.0 occur (2), so the partitions may group instead of
 	                    // rank; 0: they rank
+	const uint32_t *split0;  // sample partition plan: the first splitter table (its equality buckets skip the second level)
 };
 
 __global__ __launch_bounds__(256) void k_plan(PlanArgs a, Control *ctrl, uint32_t *hist, uint32_t *bucket)
@@ -1360,10 +1361,26 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 		real_tiles = s_ts[kRadix];
 	}
 	uint32_t tab_seg = 0xFFFFFFFFu;  // segment whose table is staged (SPL 2)
+	// The first splitter partition writes the keys (and payload) of an equality bucket straight to
+	// the sort's final arrays, at the same positions: the bucket is in its final order, and the second
+	// level skips its segment. The final arrays are those of the plan's last pass (ppp[2]: hyb[2]).
+	__shared__ uint32_t s_eqm[SPL == 1 ? 8 : 1];
+	K *__restrict__ eq_dst = dst;
+	uint32_t *eq_pdst = pp.pdst;
 	if constexpr (SPL == 1) {
 		split_load<W, THREADS>(split, s_tab);
+		// (the same test as the plan's: not when the final arrays are the source, as in place)
+		const PassPlan fp = ppp[2];
+		const bool direct = fp.dst != pp.src && !(PAY && pp.psrc && pp.psrc == fp.pdst);
+		if (threadIdx.x < 8) s_eqm[threadIdx.x] = direct ? split[kSplitEqWord + threadIdx.x] : 0u;
+		eq_dst = static_cast<K *>(fp.dst);
+		eq_pdst = fp.pdst;
 		__syncthreads();
 	}
+	auto eq_bucket = [&](uint32_t d) {
+		if constexpr (SPL == 1) return ((s_eqm[d >> 5] >> (d & 31u)) & 1u) != 0;
+		else return false;
+	};
 	auto geom = [&](uint32_t tl, uint64_t &base, uint32_t &valid, uint32_t &sgm) {
 		if constexpr (SEG) {
 			seg_tile(s_ts, seg, TILE, tl, base, valid, sgm);
@@ -1496,7 +1513,7 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 				dd = digit_of<K>(kk, xf, shift);
 				if constexpr (PAY) s_dig[i] = static_cast<uint8_t>(dd);
 			}
-			dst[s_glob[dd] + i] = kk;
+			(eq_bucket(dd) ? eq_dst : dst)[s_glob[dd] + i] = kk;
 		}
 		if constexpr (PAY) {
 			__syncthreads();
@@ -1504,7 +1521,10 @@ __global__ __launch_bounds__(THREADS) void k_scatter(const PassPlan *ppp, uint32
 #pragma unroll
 			for (int j = 0; j < KPT; ++j) s_pay[ppos[j]] = pcur[j];
 			__syncthreads();
-			for (uint32_t i = t; i < valid; i += THREADS) pp.pdst[s_glob[s_dig[i]] + i] = s_pay[i];
+			for (uint32_t i = t; i < valid; i += THREADS) {
+				const uint32_t dd = s_dig[i];
+				(eq_bucket(dd) ? eq_pdst : pp.pdst)[s_glob[dd] + i] = s_pay[i];
+			}
 		}
 		__syncthreads();
 		k = k2;
@@ -2263,6 +2283,7 @@ hipError_t launch_sample_cfg(const SortJob &job, void *ws, const KeyXform &xf, h
 	pa.segtab = segtab_of(ws);
 	pa.gate = nullptr;
 	pa.group = group;
+	pa.split0 = split;
 	hipLaunchKernelGGL(k_plan_sample, dim3(1), dim3(256), 0, stream, pa, ctrl, hist, bucket);
 	if ((e = hipGetLastError()) != hipSuccess) return e;
 	{
@@ -2620,6 +2641,7 @@ hipError_t run_sort(const SortJob &job, void *ws, hipStream_t stream)
 	pa.segtab = segtab_of(ws);
 	pa.gate = nullptr;
 	pa.group = 0;
+	pa.split0 = nullptr;
 
 	hipLaunchKernelGGL(k_plan, dim3(1), dim3(256), 0, stream, pa, ctrl, hist, bucket);
 	if ((e = hipGetLastError()) != hipSuccess) return e;
