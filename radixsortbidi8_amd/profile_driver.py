@@ -58,7 +58,8 @@ def main():
     buf = torch.empty_like(keys)
     kt = r.keytype_of(keys.dtype)
     ws = r.Workspace(kt, n, r.LAYOUT_PAIRS if args.payload else r.LAYOUT_KEYS)
-    r.set_algorithm(args.algo)
+    if args.algo != 5:  # algorithm 5 (the sample partition plan whatever the keys) is a plan, not a setting
+        r.set_algorithm(args.algo)
     if args.tile_config >= 0:
         r.set_tile_config(args.tile_config)
     if args.payload:
@@ -67,8 +68,8 @@ def main():
 
         def sort():
             r.sort_pairs(keys, out, buf, None, p_out, p_buf, ws, args.direction, args.mode)
-    elif args.presort_check:
-        plan = (args.algo or 4, args.tile_config, 1)
+    elif args.presort_check or args.algo == 5:
+        plan = (args.algo or 4, args.tile_config, 1 if args.presort_check else 0)
         flags = r.decode(kt, args.mode, args.direction)
 
         def sort():
@@ -78,7 +79,7 @@ def main():
             r.radixsortcopynoalloc(keys, out, buf, ws, args.direction, args.mode)
     what = (f"2^{args.n_log2} {args.dtype}{' N(0,1)' if args.normal else ''}"
             f"{'' if args.dist == 'uniform' else ' ' + args.dist}{' + u32 index payload' if args.payload else ''}"
-            f"{' with the presort check' if args.presort_check else ''}")
+            f"{' with the presort check' if args.presort_check else ''}{' (algorithm 5)' if args.algo == 5 else ''}")
     sort()
     torch.cuda.synchronize()
     print(f"profile_driver: warm-up sort of {what} (mode {args.mode}, direction {args.direction}) done", flush=True)
