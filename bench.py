@@ -148,8 +148,8 @@ def main():
     ap.add_argument("--algo", type=int, default=0, help="0 automatic, 1 onesweep, 2 count + scan, 4 MSD plan")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: every run here has exactly one MI355X; "
-              "--gpus N with N > 1 is not supported (the multi-GPU sharding is described in DESIGN.md)",
+        print("bench.py measures one GPU (one MI355X); --gpus N with N > 1 is not supported: "
+              "the sort runs on one device, and a sort across GPUs would first exchange keys by top-digit ranges",
               file=sys.stderr, flush=True)
         sys.exit(2)
 
